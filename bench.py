@@ -1,0 +1,135 @@
+"""Benchmark of the per-pulse scatterer engine on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): 1,997,824 rain
+scatterers + 1,024 leaf debris (1,998,848 in total), Marshall-Palmer 10 mm/h DSD,
+suctvort-shaped synthetic LES 176x176x80, concept D|B|U, 15 m gates (40 gates),
+1 deg beam, 0.2 us pulse, PRT 1 ms, lambda 0.1 m, PPI 3 deg -12..12 in 0.01 deg.
+A step is one pulse: the advance of every scatterer by one PRT and the per-gate
+complex H/V sums (RS_advance_time + RS_advance_beam + RS_make_pulse), with the
+state resident in HBM.  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps 1000] [--warmup 50] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+REFERENCE_PULSES_PER_S = 222.3   # README.md:58 (instantaneous, 1,998,848 scatterers, GPU not stated)
+
+
+def cpu_baseline(r, cfg, pulses):
+    """The oracle (CPU restatement of the reference kernels, one thread) timed
+    on a bounded sample of the same workload."""
+    from oracle import run as orun
+    st = orun.state_from_radar(r)
+    o = orun.OracleRun(r, cfg, st)
+    sd = r.sim_desc()
+    t0 = time.perf_counter()
+    for _ in range(pulses):
+        o.advance(sd)
+        o.refresh_debris_rcs(sd)
+        o.pulse(sd)
+    dt = time.perf_counter() - t0
+    return pulses / dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-pulses", type=int, default=4)
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--max-blocks", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build drives a single MI355X per process "
+              "(multi-GPU sharding is a design note only); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import simradar_amd as sa
+    from simradar_amd import workloads as wl
+
+    cfg = wl.c2()
+    print("bench: populating %s (%d scatterers)" % (cfg.name, cfg.n_total), file=sys.stderr, flush=True)
+    r = wl.make(cfg)
+    if args.block or args.max_blocks:
+        r.set_launch(args.block, args.max_blocks)
+    R = int(r.params().range_count)
+
+    cpu = None
+    if not args.no_cpu:
+        print("bench: cpu baseline (%d pulses of the oracle)" % args.cpu_pulses, file=sys.stderr, flush=True)
+        cpu_pps = cpu_baseline(r, cfg, args.cpu_pulses)
+        cpu = {"value": cpu_pps, "unit": "pulses/s", "cores": 1, "kind": "port",
+               "sample": "%d pulses of the full %d-scatterer workload, oracle C restatement, one thread"
+                         % (args.cpu_pulses, cfg.n_total)}
+
+    def step():
+        r.make_pulse()
+        r.advance_time()
+        r.advance_beam()
+
+    for _ in range(args.warmup):
+        step()
+    r.sync()
+    r.check()
+    r.profile(True)
+    print("bench: timing %d pulses" % args.steps, file=sys.stderr, flush=True)
+    r.sync()
+    t0 = time.perf_counter()
+    r.timer_start()
+    for _ in range(args.steps):
+        step()
+    r.make_pulse()          # flush the last advance through a fused launch
+    ev_ms = r.timer_stop()
+    r.sync()
+    wall = time.perf_counter() - t0
+    r.check()
+    k_ms, k_n = r.profile_read(3)
+    r.profile(False)
+    steps = args.steps + 1
+    ms_per_step = ev_ms / steps
+    pps = 1000.0 / ms_per_step
+    n = cfg.n_total
+    kern_ms = k_ms / max(1, k_n)
+    rain_bytes = 64 * cfg.n_rain
+    achieved = rain_bytes / (kern_ms * 1e-3) / 1e9
+    line = {
+        "metric": "pulses/s & scatterer-updates/s @2M; surrogate vs table at <=0.5 dB Z err",
+        "value": pps,
+        "unit": "pulses/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "wall_ms_per_step": 1000.0 * wall / steps,
+        "scatterer_updates_per_s": pps * n,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": pps / REFERENCE_PULSES_PER_S,
+        "vs_baseline_ref": "README.md:58 222.3 pulses/s (1,998,848 rain scatterers, 30 m gates, GPU not stated)",
+        "dtype": "f32 state, f64 phase and gate sums",
+        "data": "synthetic (seeded LES / DSD / ADM / RCS tables)",
+        "config": {"workload": "C2: 1,997,824 rain + 1,024 leaf debris, %d gates x 15 m, PPI 3 deg -12:12:0.01, "
+                               "suctvort LES 176x176x80, concept DBU" % R,
+                   "scatterers": n, "range_gates": R, "algorithmic_bytes_per_pulse": cfg.bytes_per_pulse()},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "kernel": "k_rain<advance,pulse>", "kernel_avg_ms": kern_ms, "kernel_launches": k_n,
+                     "bytes_per_launch": rain_bytes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    r.free()
+
+
+if __name__ == "__main__":
+    main()

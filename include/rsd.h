@@ -1,0 +1,145 @@
+/*
+ * rsd.h -- C-ABI of the gfx950 scatterer engine (the device layer under RS_*).
+ *
+ * The reference drives OpenCL directly from rs.c.  Each group of entry points
+ * below replaces one of those call clusters; the RS_* host runtime
+ * (include/rs.h) is written against this interface only.
+ *
+ *   rsd_open / rsd_close            RS_worker_init / RS_worker_free (rs.c:383-620)
+ *   rsd_alloc_state                 RS_worker_malloc buffer creation (rs.c:623-944)
+ *   rsd_upload_state                RS_upload (rs.c:5124-5175)
+ *   rsd_download_state              RS_download, RS_download_position_only,
+ *                                   RS_download_orientation_only (rs.c:4961-5064)
+ *   rsd_set_wind                    RS_set_vel_data image upload (rs.c:2854-2977)
+ *   rsd_set_adm                     RS_set_adm_data (rs.c:3642-3762)
+ *   rsd_set_rcs                     RS_set_rcs_data (rs.c:3852-3972)
+ *   rsd_set_drop_rcs                RS_set_rcs_ellipsoid_table (rs.c:2472-2545)
+ *   rsd_set_range_weight            RS_set_range_weight (rs.c:2548-2618)
+ *   rsd_set_angular_weight          RS_set_angular_weight (rs.c:2626-2698)
+ *   rsd_advance                     enqueue block of RS_advance_time (rs.c:5327-5388)
+ *   rsd_make_pulse                  enqueue block of RS_make_pulse (rs.c:5526-5584)
+ *   rsd_read_pulse                  RS_download_pulse_only read (rs.c:5096-5121)
+ *
+ * Conventions: plain pointers and sizes only; every function returns RSD_OK (0)
+ * or a negative RSD_ERR_* code, never exits; rsd_last_error() gives the text of
+ * the most recent failure on that engine.  Streams are the reference's float4 /
+ * uint4 layouts (16 bytes per scatterer per attribute), descriptors are the
+ * reference's float16 / float4 slot layouts (rs_enum.h).  One engine drives one
+ * GPU; calls on one engine must not overlap.
+ */
+#ifndef RSD_H
+#define RSD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RSD_OK              0
+#define RSD_ERR_NO_DEVICE  -1
+#define RSD_ERR_ALLOC      -2
+#define RSD_ERR_ARG        -3
+#define RSD_ERR_HIP        -4
+#define RSD_ERR_STATE      -5
+
+#define RSD_MAX_TYPES       8
+#define RSD_MAX_TABLES      8
+#define RSD_MAX_GATES       4096
+
+typedef struct rsd_engine rsd_engine;
+
+/* Advance kind of the background (type 0) scatterers. */
+#define RSD_ADVANCE_DRAGGED     1u   /* el_atts (concept D, rs.cl:653-739) */
+#define RSD_ADVANCE_BACKGROUND  0u   /* bg_atts (rs.cl:533-580) */
+
+/* Precision of the phase term of the pulse. */
+#define RSD_PHASE_F64           0u   /* 4 pi r / lambda reduced mod 2 pi in binary64 */
+#define RSD_PHASE_F32           1u   /* r * (4 pi / lambda) in binary32, as rs.cl:1096 */
+
+typedef struct {
+    float    sim_desc[16];           /* reference sim_desc at the RS_advance_time call */
+    int32_t  wind_slot;              /* active LES slot (0 or 1) */
+    int32_t  adm_index[RSD_MAX_TYPES];   /* ADM table of each debris type (entry 0 unused) */
+    int32_t  rcs_index[RSD_MAX_TYPES];   /* RCS table of each debris type (entry 0 unused) */
+    uint32_t background_kind;        /* RSD_ADVANCE_DRAGGED or RSD_ADVANCE_BACKGROUND */
+} rsd_step;
+
+typedef struct {
+    float    sim_desc[16];           /* reference sim_desc at the RS_make_pulse call (beam) */
+    double   two_over_lambda;        /* 2 / lambda in binary64: phase in turns per metre */
+    float    range_start;            /* first gate range (m) */
+    float    range_delta;            /* gate spacing (m) */
+    uint32_t range_count;            /* number of gates R */
+    int32_t  rcs_index[RSD_MAX_TYPES];
+    uint32_t debris_rcs_refresh;     /* 1 when the beam moved since the last pulse (rs.c:5531) */
+    uint32_t phase_mode;             /* RSD_PHASE_F64 or RSD_PHASE_F32 */
+} rsd_pulse;
+
+int  rsd_device_count(void);
+/* Compute units, maximum work-group size and LDS bytes per work-group of a device. */
+int  rsd_device_props(int device_index, int *compute_units, int *max_work_group, size_t *local_mem);
+int  rsd_open(int device_index, rsd_engine **out);
+void rsd_close(rsd_engine *e);
+const char *rsd_last_error(const rsd_engine *e);
+
+/* Scatterer population: counts[k] scatterers of type k at origins[k] (type 0 =
+ * rain at origin 0, debris types contiguous after it), num_scats in total. */
+int rsd_alloc_state(rsd_engine *e, size_t num_scats, const size_t *origins, const size_t *counts);
+
+/* Full-stream transfers (num_scats elements each; any pointer may be NULL on
+ * download).  sig and aux on download are computed for the current positions
+ * with the beam of the most recent pulse (rs.cl:1050-1110 quantities). */
+int rsd_upload_state(rsd_engine *e, const float *pos, const float *vel, const float *ori,
+                     const float *tum, const float *aux, const float *rcs, const uint32_t *rnd);
+int rsd_download_state(rsd_engine *e, float *pos, float *vel, float *ori, float *tum,
+                       float *aux, float *rcs, float *sig, uint32_t *rnd);
+
+/* Tables.  Texels are float4, x fastest.  Descriptors use the reference slots. */
+int rsd_set_wind(rsd_engine *e, int slot, const float *uvwt, int nx, int ny, int nz, const float *les_desc16);
+int rsd_set_adm(rsd_engine *e, int table, const float *cd, const float *cm, int nx, int ny, const float *adm_desc16);
+int rsd_set_rcs(rsd_engine *e, int table, const float *re, const float *im, int nx, int ny, const float *rcs_desc16);
+int rsd_set_drop_rcs(rsd_engine *e, const float *table4, int n, const float *desc4);
+int rsd_set_range_weight(rsd_engine *e, const float *w, int n, const float *desc4);
+int rsd_set_angular_weight(rsd_engine *e, const float *w, int n, const float *desc4);
+
+/* One PRT of every scatterer type.  May be held back and fused with the next
+ * rsd_make_pulse; any transfer or rsd_sync performs it first. */
+int rsd_advance(rsd_engine *e, const rsd_step *step);
+
+/* Per-gate complex sums of all scatterers (before the radar gain) into the
+ * engine's pulse buffer.  Enqueued; rsd_read_pulse waits for it. */
+int rsd_make_pulse(rsd_engine *e, const rsd_pulse *pulse);
+
+/* Blocking read of the last pulse: out[4 k + c] = (Ih, Qh, Iv, Qv) of gate k. */
+int rsd_read_pulse(rsd_engine *e, double *out, uint32_t range_count);
+
+int rsd_sync(rsd_engine *e);
+
+/* Timing of everything enqueued between the two calls, in milliseconds,
+ * measured with HIP events on the engine's stream. */
+int rsd_timer_start(rsd_engine *e);
+int rsd_timer_stop(rsd_engine *e, double *ms);
+
+/* Per-launch HIP-event timing of the background (rain) kernel on the engine's
+ * stream: rsd_profile(e, 1) clears and starts recording; rsd_profile_read sums
+ * the durations of the recorded launches of one kind (1 advance only, 2 pulse
+ * only, 3 fused advance + pulse, -1 all). */
+int rsd_profile(rsd_engine *e, int enable);
+int rsd_profile_read(rsd_engine *e, int kind, double *total_ms, int *count);
+
+/* Launch shape of the scatterer kernels (0 = built-in default). */
+int rsd_set_launch(rsd_engine *e, int block_size, int max_blocks);
+
+/* Self-test: evaluate one of the device's binary32 functions on n inputs
+ * (fn: 0 log1p(a), 1 log(a), 2 sin(a), 3 cos(a), 4 atan2(a, b), 5 acos(a),
+ * 6 sqrt(a), 7 a / b, 8 sin(2 pi a), 9 cos(2 pi a)).  No counterpart in the
+ * reference; it lets a caller check the device arithmetic the advection uses. */
+int rsd_math_probe(int device_index, int fn, const float *a, const float *b, float *out, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,462 @@
+/*
+ * rs_providers.c -- in-memory table providers, scan patterns and logging for
+ * the RS_* host runtime (plain C).
+ *
+ * Interfaces as les.h / adm.h / rcs.h / obj.h / pos.h / log.h of the reference;
+ * the tables themselves are synthetic (see rs_providers.h for the fields).
+ */
+#include "rs_providers.h"
+
+#include <stdarg.h>
+#include <time.h>
+
+/* ------------------------------------------------------------------ logging */
+
+char *now(void) {
+    static char buf[64];
+    struct timeval tv;
+    gettimeofday(&tv, NULL);
+    struct tm tm;
+    localtime_r(&tv.tv_sec, &tm);
+    snprintf(buf, sizeof(buf), "%02d:%02d:%02d.%03d", tm.tm_hour, tm.tm_min, tm.tm_sec, (int)(tv.tv_usec / 1000));
+    return buf;
+}
+
+char *commaint(long long num) {
+    static char bufs[8][64];
+    static int k = 0;
+    char *out = bufs[k];
+    k = (k + 1) % 8;
+    char tmp[64];
+    snprintf(tmp, sizeof(tmp), "%lld", num < 0 ? -num : num);
+    int len = (int)strlen(tmp), j = 0;
+    if (num < 0) out[j++] = '-';
+    for (int i = 0; i < len; i++) {
+        out[j++] = tmp[i];
+        int left = len - 1 - i;
+        if (left > 0 && left % 3 == 0) out[j++] = ',';
+    }
+    out[j] = '\0';
+    return out;
+}
+
+char *commafloat(float num) {
+    static char bufs[8][64];
+    static int k = 0;
+    char *out = bufs[k];
+    k = (k + 1) % 8;
+    long long ip = (long long)num;
+    float frac = fabsf(num - (float)ip);
+    snprintf(out, 64, "%s.%02d", commaint(ip), (int)(frac * 100.0f + 0.5f) % 100);
+    return out;
+}
+
+void rsprint(const char *format, ...) {
+    char msg[4096];
+    va_list args;
+    va_start(args, format);
+    vsnprintf(msg, sizeof(msg), format, args);
+    va_end(args);
+    size_t n = strlen(msg);
+    while (n > 0 && msg[n - 1] == '\n') msg[--n] = '\0';
+    FILE *out = (strncmp(msg, "ERROR", 5) == 0 || strncmp(msg, "WARNING", 7) == 0) ? stderr : stdout;
+    fprintf(out, "%s : RS : %s\n", now(), msg);
+}
+
+/* ------------------------------------------------------- seeded generator */
+
+uint64_t rs_splitmix64(uint64_t *s) {
+    uint64_t z = (*s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+double rs_uniform(uint64_t *s) { return ((double)(rs_splitmix64(s) >> 11) + 0.5) * 0x1.0p-53; }
+
+double rs_gauss(uint64_t *s) {
+    double u1 = rs_uniform(s), u2 = rs_uniform(s);
+    return sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
+}
+
+/* ----------------------------------------------------------------- LES */
+
+typedef struct {
+    uint32_t nx, ny, nz, frames;
+    float ax, rx, az, rz, core, vmax, sigma, tp, tr;
+    uint64_t seed;
+    LESTable **tables;
+} LESMem;
+
+LESHandle LES_init_synthetic(uint32_t nx, uint32_t ny, uint32_t nz, float a_xy, float r_xy, float a_z, float r_z,
+                             float core_m, float vmax, float sigma, uint64_t seed, uint32_t frames) {
+    if (nx < 2 || ny < 2 || nz < 2 || frames < 1) return NULL;
+    LESMem *h = (LESMem *)calloc(1, sizeof(LESMem));
+    if (!h) return NULL;
+    h->nx = nx; h->ny = ny; h->nz = nz; h->frames = frames;
+    h->ax = a_xy; h->rx = r_xy; h->az = a_z; h->rz = r_z;
+    h->core = core_m; h->vmax = vmax; h->sigma = sigma;
+    h->tp = 2.0f;
+    h->tr = 50.0f;
+    h->seed = seed;
+    h->tables = (LESTable **)calloc(frames, sizeof(LESTable *));
+    if (!h->tables) {
+        free(h);
+        return NULL;
+    }
+    return (LESHandle)h;
+}
+
+LESHandle LES_init_with_config_path(const LESConfig config, const char *path) {
+    (void)path;
+    if (config && (!strcmp(config, LESConfigSuctionVortices) || !strcmp(config, LESConfigSuctionVorticesLarge))) {
+        return LES_init_synthetic(176, 176, 80, 2.0f, 1.0212f, 2.0f, 1.05f, 50.0f, 40.0f, 2.0f, 20250101ull, 10);
+    }
+    rsprint("ERROR: LES configuration '%s' needs external data; only '%s' is generated in memory.",
+            config ? config : "(null)", LESConfigSuctionVortices);
+    return NULL;
+}
+
+LESHandle LES_init(void) { return LES_init_with_config_path(LESConfigSuctionVortices, NULL); }
+
+static void les_free_table(LESTable *t) {
+    if (!t) return;
+    free(t->uvwt);
+    free(t->cpxx);
+    free(t);
+}
+
+void LES_free(LESHandle i) {
+    LESMem *h = (LESMem *)i;
+    if (!h) return;
+    for (uint32_t f = 0; f < h->frames; f++) les_free_table(h->tables[f]);
+    free(h->tables);
+    free(h);
+}
+
+/* Position of texel index i along a stretched axis: the stretched map sends
+ * |x| = a (r^m - 1) / (r - 1) to coordinate m + offset and texel i is read at
+ * coordinate i + 0.5 (rs.cl:130-151 with the offsets of rs.c:3058-3061). */
+static double texel_position(int i, double offset, double a, double r) {
+    double m = (double)i + 0.5 - offset;
+    double s = m < 0.0 ? -1.0 : 1.0;
+    return s * a * (pow(r, fabs(m)) - 1.0) / (r - 1.0);
+}
+
+static LESTable *les_make_frame(LESMem *h, uint32_t f) {
+    LESTable *t = (LESTable *)calloc(1, sizeof(LESTable));
+    if (!t) return NULL;
+    t->nx = h->nx; t->ny = h->ny; t->nz = h->nz;
+    t->nn = h->nx * h->ny * h->nz;
+    t->nt = 1;
+    t->nc = h->frames;
+    t->is_stretched = true;
+    t->tr = h->tr;
+    t->tp = h->tp;
+    t->ax = h->ax; t->ay = h->ax; t->az = h->az;
+    t->rx = h->rx; t->ry = h->rx; t->rz = h->rz;
+    t->uvwt = (LESFloat4 *)malloc((size_t)t->nn * sizeof(LESFloat4));
+    if (!t->uvwt) {
+        free(t);
+        return NULL;
+    }
+    const double offx = 0.5 * (double)(h->nx - 1), offy = 0.5 * (double)(h->ny - 1);
+    uint64_t s = h->seed * 0x9E3779B97F4A7C15ull + (uint64_t)(f + 1) * 0xD1B54A32D192ED03ull;
+    size_t idx = 0;
+    for (uint32_t k = 0; k < h->nz; k++) {
+        for (uint32_t j = 0; j < h->ny; j++) {
+            double y = texel_position((int)j, offy, h->ax, h->rx);
+            for (uint32_t i = 0; i < h->nx; i++, idx++) {
+                double x = texel_position((int)i, offx, h->ax, h->rx);
+                double rho = sqrt(x * x + y * y);
+                double vt = rho <= h->core ? h->vmax * rho / h->core : h->vmax * h->core / rho;
+                double u = rho > 0.0 ? -vt * y / rho : 0.0;
+                double v = rho > 0.0 ? vt * x / rho : 0.0;
+                t->uvwt[idx][0] = (float)(u + h->sigma * rs_gauss(&s));
+                t->uvwt[idx][1] = (float)(v + h->sigma * rs_gauss(&s));
+                t->uvwt[idx][2] = (float)(h->sigma * rs_gauss(&s));
+                t->uvwt[idx][3] = 0.0f;
+            }
+        }
+    }
+    return t;
+}
+
+LESTable *LES_get_frame(const LESHandle i, const int n) {
+    LESMem *h = (LESMem *)i;
+    if (!h || n < 0 || (uint32_t)n >= h->frames) return NULL;
+    if (!h->tables[n]) h->tables[n] = les_make_frame(h, (uint32_t)n);
+    return h->tables[n];
+}
+
+float LES_get_table_period(const LESHandle i) { return i ? ((LESMem *)i)->tp : 0.0f; }
+size_t LES_get_table_count(const LESHandle i) { return i ? ((LESMem *)i)->frames : 0; }
+
+/* ----------------------------------------------------------------- ADM */
+
+void ADM_compute_properties(ADMBase *phys) {
+    /* adm.c:279-304: mass, Tachikawa number and inverse inertia lengths,
+     * non-dimensionalised with v0 = 100 m/s and g = 9.8 m/s^2 */
+    const float g = 9.8f, v0 = 100.0f, rho_air = 1.21f;
+    phys->mass = phys->x * phys->y * phys->z * phys->rho;
+    float ix = (phys->y * phys->y + phys->z * phys->z) * phys->mass / 12.0f;
+    float iy = (phys->x * phys->x + phys->z * phys->z) * phys->mass / 12.0f;
+    float iz = (phys->x * phys->x + phys->y * phys->y) * phys->mass / 12.0f;
+    phys->Ta = rho_air * (phys->y * phys->z * v0 * v0) / (2.0f * phys->mass * g);
+    phys->inv_inln_x = (phys->mass * phys->x) / (ix * g * g);
+    phys->inv_inln_y = (phys->mass * phys->y) / (iy * g * g);
+    phys->inv_inln_z = (phys->mass * phys->z) / (iz * g * g);
+    phys->Ta *= g / (v0 * v0);
+}
+
+void ADM_dimension_set(ADMTable *T, const float x, const float y, const float z, const float r) {
+    T->phys.x = x;
+    T->phys.y = y;
+    T->phys.z = z;
+    T->phys.rho = r;
+    ADM_compute_properties(&T->phys);
+}
+
+/* Drag along the body-frame flow direction u = (cos b, sin b cos a, sin b sin a)
+ * with a shape factor C(b) = 1 + 0.3 cos 2b, small torques, plus N(0, noise^2). */
+ADMTable *ADM_table_synthetic(const char *name, uint32_t nb, uint32_t na, float noise, uint64_t seed) {
+    ADMTable *T = (ADMTable *)calloc(1, sizeof(ADMTable));
+    if (!T) return NULL;
+    T->nb = nb;
+    T->na = na;
+    T->nn = nb * na;
+    snprintf(T->name, sizeof(T->name), "%s", name ? name : "synthetic");
+    float **planes[8] = {&T->data.b, &T->data.a, &T->data.cdx, &T->data.cdy, &T->data.cdz, &T->data.cmx, &T->data.cmy, &T->data.cmz};
+    for (int p = 0; p < 8; p++) {
+        *planes[p] = (float *)malloc((size_t)T->nn * sizeof(float));
+        if (!*planes[p]) {
+            ADM_table_free(T);
+            return NULL;
+        }
+    }
+    uint64_t s = seed * 0x9E3779B97F4A7C15ull + 0x243F6A8885A308D3ull;
+    for (uint32_t j = 0; j < na; j++) {
+        double a = M_PI * (double)j / (double)(na - 1);
+        for (uint32_t i = 0; i < nb; i++) {
+            size_t k = (size_t)j * nb + i;
+            double b = -M_PI + 2.0 * M_PI * (double)i / (double)(nb - 1);
+            double C = 1.0 + 0.3 * cos(2.0 * b);
+            T->data.b[k] = (float)b;
+            T->data.a[k] = (float)a;
+            T->data.cdx[k] = (float)(C * cos(b) + noise * rs_gauss(&s));
+            T->data.cdy[k] = (float)(C * sin(b) * cos(a) + noise * rs_gauss(&s));
+            T->data.cdz[k] = (float)(C * sin(b) * sin(a) + noise * rs_gauss(&s));
+            T->data.cmx[k] = (float)(0.02 * sin(2.0 * b) * sin(a) + noise * rs_gauss(&s));
+            T->data.cmy[k] = (float)(-0.05 * sin(2.0 * b) * cos(a) + noise * rs_gauss(&s));
+            T->data.cmz[k] = (float)(0.03 * sin(b) * sin(2.0 * a) + noise * rs_gauss(&s));
+        }
+    }
+    return T;
+}
+
+void ADM_table_free(ADMTable *T) {
+    if (!T) return;
+    free(T->data.b); free(T->data.a); free(T->data.cdx); free(T->data.cdy); free(T->data.cdz);
+    free(T->data.cmx); free(T->data.cmy); free(T->data.cmz);
+    free(T);
+}
+
+/* ----------------------------------------------------------------- RCS */
+
+/* hh/vv/hv amplitudes over alpha in [-pi, pi] (x) and beta in [0, pi] (y):
+ * hh = s (0.6 + 0.4 cos^2 b) e^{j 0.5 sin a}, vv = s (0.5 + 0.3 sin^2 b cos^2 a)
+ * e^{-j 0.3 cos b}, hv = 0.1 s sin 2b sin a, plus N(0, (noise s)^2) per value. */
+RCSTable *RCS_table_synthetic(const char *name, uint32_t na, uint32_t nb, float scale, float noise, uint64_t seed) {
+    RCSTable *T = (RCSTable *)calloc(1, sizeof(RCSTable));
+    if (!T) return NULL;
+    T->na = na;
+    T->nb = nb;
+    T->nn = na * nb;
+    T->lambda = 0.1f;
+    snprintf(T->name, sizeof(T->name), "%s", name ? name : "synthetic");
+    float **planes[8] = {&T->data.a, &T->data.b, &T->data.hh_real, &T->data.vv_real, &T->data.hv_real,
+                         &T->data.hh_imag, &T->data.vv_imag, &T->data.hv_imag};
+    for (int p = 0; p < 8; p++) {
+        *planes[p] = (float *)malloc((size_t)T->nn * sizeof(float));
+        if (!*planes[p]) {
+            RCS_table_free(T);
+            return NULL;
+        }
+    }
+    uint64_t s = seed * 0x9E3779B97F4A7C15ull + 0x13198A2E03707344ull;
+    const double sc = scale, ns = noise * scale;
+    for (uint32_t j = 0; j < nb; j++) {
+        double b = M_PI * (double)j / (double)(nb - 1);
+        for (uint32_t i = 0; i < na; i++) {
+            size_t k = (size_t)j * na + i;
+            double a = -M_PI + 2.0 * M_PI * (double)i / (double)(na - 1);
+            double hh = sc * (0.6 + 0.4 * cos(b) * cos(b)), ph = 0.5 * sin(a);
+            double vv = sc * (0.5 + 0.3 * sin(b) * sin(b) * cos(a) * cos(a)), pv = -0.3 * cos(b);
+            double hv = 0.1 * sc * sin(2.0 * b) * sin(a);
+            T->data.a[k] = (float)a;
+            T->data.b[k] = (float)b;
+            T->data.hh_real[k] = (float)(hh * cos(ph) + ns * rs_gauss(&s));
+            T->data.hh_imag[k] = (float)(hh * sin(ph) + ns * rs_gauss(&s));
+            T->data.vv_real[k] = (float)(vv * cos(pv) + ns * rs_gauss(&s));
+            T->data.vv_imag[k] = (float)(vv * sin(pv) + ns * rs_gauss(&s));
+            T->data.hv_real[k] = (float)(hv + ns * rs_gauss(&s));
+            T->data.hv_imag[k] = (float)(0.3 * hv + ns * rs_gauss(&s));
+        }
+    }
+    return T;
+}
+
+void RCS_table_free(RCSTable *T) {
+    if (!T) return;
+    free(T->data.a); free(T->data.b); free(T->data.hh_real); free(T->data.vv_real); free(T->data.hv_real);
+    free(T->data.hh_imag); free(T->data.vv_imag); free(T->data.hv_imag);
+    free(T);
+}
+
+/* ----------------------------------------------------------------- OBJ */
+
+#define OBJ_MAX 64
+
+typedef struct {
+    int count;
+    OBJTable obj_table[OBJ_MAX];
+} OBJMem;
+
+OBJHandle OBJ_init(void) { return (OBJHandle)calloc(1, sizeof(OBJMem)); }
+
+void OBJ_free(OBJHandle in) {
+    OBJMem *O = (OBJMem *)in;
+    if (!O) return;
+    for (int i = 0; i < O->count; i++) {
+        ADM_table_free(O->obj_table[i].adm_table);
+        RCS_table_free(O->obj_table[i].rcs_table);
+    }
+    free(O);
+}
+
+/* Dimensions (m) and densities (kg m^-3) as obj.c:102-148. */
+OBJTable *OBJ_get_table(const OBJHandle in, OBJConfig type) {
+    OBJMem *O = (OBJMem *)in;
+    if (!O || O->count >= OBJ_MAX) return NULL;
+    OBJTable *t = &O->obj_table[O->count];
+    const char *adm = ADMConfigSquarePlate, *rcs = RCSConfigPlate;
+    float dx = 0.0f, dy = 0.0f, dz = 0.0f, rho = 0.0f, scale = 0.05f;
+    switch (type) {
+        case OBJConfigWoodboard2x4: rcs = RCSConfigWoodBoard; dx = 0.0508f; dy = 0.3048f; dz = 0.1016f; rho = 500.0f; scale = 0.05f; break;
+        case OBJConfigWoodboard4x8: rcs = RCSConfigWoodBoard; dx = 0.1016f; dy = 0.3048f; dz = 0.2032f; rho = 500.0f; scale = 0.08f; break;
+        case OBJConfigLeaf: adm = ADMConfigModelPlate; rcs = RCSConfigLeaf; dx = 0.002f; dy = 0.04f; dz = 0.04f; rho = 1120.0f; scale = 0.01f; break;
+        case OBJConfigLeafBig: adm = ADMConfigModelPlate; rcs = RCSConfigLeaf; dx = 0.001f; dy = 0.08f; dz = 0.06f; rho = 350.0f; scale = 0.02f; break;
+        case OBJConfigMetalSheet: rcs = RCSConfigPlate; dx = 0.001f; dy = 1.0f; dz = 1.0f; rho = 7850.0f; scale = 0.1f; break;
+        case OBJConfigBrick: rcs = RCSConfigBrick; dx = 0.065f; dy = 0.215f; dz = 0.1125f; rho = 2200.0f; scale = 0.05f; break;
+        default: return NULL;
+    }
+    t->adm_table = ADM_table_synthetic(adm, 361, 181, 0.01f, 1000ull + (uint64_t)type);
+    t->rcs_table = RCS_table_synthetic(rcs, 361, 181, scale, 0.02f, 2000ull + (uint64_t)type);
+    if (!t->adm_table || !t->rcs_table) return NULL;
+    ADM_dimension_set(t->adm_table, dx, dy, dz, rho);
+    O->count++;
+    return t;
+}
+
+OBJTable *OBJ_table_at(const OBJHandle in, int t) {
+    OBJMem *O = (OBJMem *)in;
+    if (!O || t < 0 || t >= O->count) return NULL;
+    return &O->obj_table[t];
+}
+
+/* ----------------------------------------------------------------- POS */
+
+POSPattern *POS_init(void) { return POS_init_with_string("P:5,-12:12:0.05/10,-12:12:0.05/15,-12:12:0.05"); }
+
+void POS_free(POSPattern *p) { free(p); }
+
+POSPattern *POS_init_with_string(const char *scan_string) {
+    POSPattern *p = (POSPattern *)calloc(1, sizeof(POSPattern));
+    if (!p) return NULL;
+    POS_parse_from_string(p, scan_string);
+    return p;
+}
+
+/* pos.c:34-56: emit the current position, then step (with repetitions). */
+int POS_get_next_angles(POSPattern *scan) {
+    int k = (int)scan->index;
+    scan->az = scan->positions[k].az;
+    scan->el = scan->positions[k].el;
+    scan->positions[k].index++;
+    if (scan->positions[k].index == scan->positions[k].count) {
+        scan->positions[k].index = 0;
+        scan->index++;
+        if (scan->index == scan->count) scan->index = 0;
+    }
+    scan->tic++;
+    return 0;
+}
+
+/* pos.c:58-184: "P:el,az0:az1:daz/...", "R:az,el0:el1:del/...", "D:az,el,count/...". */
+int POS_parse_from_string(POSPattern *scan, const char *string) {
+    if (!string || strlen(string) < 2 || string[1] != ':') {
+        rsprint("ERROR: Expected : after the scan mode character.");
+        return 1;
+    }
+    char buf[1024];
+    snprintf(buf, sizeof(buf), "%s", string + 2);
+    scan->mode = string[0];
+    int i = 0, j = 0, k = 0;
+    float f1, f2, f3, f4;
+    char *save = NULL;
+    char *token = strtok_r(buf, "/", &save);
+    const char m = (char)(scan->mode | 0x20);
+    if (m == 'p' || m == 'r') {
+        while (token && j < POS_MAX_PATTERN_COUNT && i < POS_MAX_SWEEP_COUNT) {
+            k = sscanf(token, "%f,%f:%f:%f", &f1, &f2, &f3, &f4);
+            POSSweep *sw = &scan->sweeps[i];
+            if (m == 'p') {
+                sw->elStart = f1; sw->elEnd = f1; sw->elDelta = 0.0f;
+                sw->azStart = f2; sw->azEnd = f3; sw->azDelta = f4;
+            } else {
+                sw->azStart = f1; sw->azEnd = f1; sw->azDelta = 0.0f;
+                sw->elStart = f2; sw->elEnd = f3; sw->elDelta = f4;
+            }
+            do {
+                if (m == 'p') {
+                    scan->positions[j].el = f1;
+                    scan->positions[j].az = f2;
+                } else {
+                    scan->positions[j].az = f1;
+                    scan->positions[j].el = f2;
+                }
+                scan->positions[j].index = 0;
+                scan->positions[j].count = 1;
+                f2 += f4;
+                j++;
+            } while (fabs((m == 'p' ? scan->positions[j - 1].az : scan->positions[j - 1].el) - f3) > 0.1f * f4 &&
+                     j < POS_MAX_PATTERN_COUNT);
+            token = strtok_r(NULL, "/", &save);
+            i++;
+        }
+        scan->sweepCount = (uint32_t)i;
+        scan->sweepIndex = 0;
+    } else if (m == 'd') {
+        while (token && j < POS_MAX_PATTERN_COUNT) {
+            k = sscanf(token, "%f,%f,%f", &f1, &f2, &f3);
+            scan->positions[j].az = f1;
+            scan->positions[j].el = f2;
+            scan->positions[j].index = 0;
+            scan->positions[j].count = (uint32_t)f3;
+            token = strtok_r(NULL, "/", &save);
+            j++;
+        }
+    }
+    scan->count = (uint32_t)j;
+    scan->index = 0;
+    if (k > 0) {
+        scan->el = scan->positions[0].el;
+        scan->az = scan->positions[0].az;
+    }
+    return 0;
+}
+
+bool POS_is_ppi(const POSPattern *s) { return s->mode == 'p' || s->mode == 'P'; }
+bool POS_is_rhi(const POSPattern *s) { return s->mode == 'r' || s->mode == 'R'; }
+bool POS_is_dbs(const POSPattern *s) { return s->mode == 'd' || s->mode == 'D'; }
+bool POS_is_empty(const POSPattern *s) { return !s->count; }

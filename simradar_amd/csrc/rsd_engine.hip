@@ -1,0 +1,1324 @@
+// rsd_engine.hip -- gfx950 scatterer engine behind include/rsd.h.
+//
+// Data layout in HBM (one engine = one GPU):
+//   pos, vel, ori, tum, rcs, aux : float4[N]   (structure of float4 streams)
+//   rnd                          : uint4[N]     (four 31-bit LCG lanes)
+//   type ranges                  : [0, n_rain) rain, then each debris type
+//   wind                         : two float4[nz][ny][nx] LES slots (u, v, w, t)
+//   adm cd/cm, rcs re/im         : float4[ny][nx] per table
+//   small tables                 : drop RCS (96 float4), angular / range weights,
+//                                  binary32 gate ranges
+//   partials                     : double[rows][4][R] per-block gate sums
+//
+// Rain keeps its radar cross-section out of memory: it is a function of the
+// drop radius and the position's elevation (rs.cl:268-280) and is recomputed in
+// registers every pulse.  The one case the reference keeps a value that the
+// position no longer determines is a respawn (rs.cl:685-692 leaves rcs as it
+// was); the engine then writes that value to the rcs stream and marks the
+// scatterer by the sign bit of pos.w (the radius, always >= 0).  The flag is
+// cleared by the next advance that does not respawn, and downloads return
+// |pos.w| and the reconstructed rcs, so the streams a caller sees are the
+// reference's.
+//
+// One pulse = one fused kernel per scatterer type (advance of the previous PRT,
+// if pending, then the signal of every scatterer folded into its gates) plus a
+// finalize kernel that sums the per-block gate partials in binary64.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "rsd.h"
+#include "rsd_fmath.h"
+
+#define RSD_MIN_HEIGHT 2.0f
+#define RSD_PI_F 0x1.921fb6p+1f
+#define RSD_PI_2_F 0x1.921fb6p+0f
+#define RSD_SQRT1_2_F 0x1.6a09e6p-1f
+
+struct F16 { float s[16]; };
+struct F4 { float s[4]; };
+struct Tex3 { const float4 *d; int nx, ny, nz; };
+struct Tex2 { const float4 *d; int nx, ny; };
+
+// ---------------------------------------------------------------------------
+// Table sampling: OpenCL linear filter, unnormalised coordinates, clamp to edge
+// (rs.cl:8).  Texel pair floor(u - 0.5), +1, clamped; weight (u - 0.5) - floor;
+// lerp(p, q, a) = fma(a, q, fma(-a, p, p)); x first, then y, then z.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void tex_axis(float u, int n, int &i0, int &i1, float &a) {
+    float uu = u - 0.5f;
+    float fl = __builtin_floorf(uu);
+    a = uu - fl;
+    float fc = __builtin_fminf(__builtin_fmaxf(fl, -1.0f), (float)n);
+    int k = (int)fc;
+    i0 = k < 0 ? 0 : (k > n - 1 ? n - 1 : k);
+    i1 = k + 1 < 0 ? 0 : (k + 1 > n - 1 ? n - 1 : k + 1);
+}
+
+__device__ __forceinline__ float lerpf_(float p, float q, float t) { return rf_fma(t, q, rf_fma(-t, p, p)); }
+__device__ __forceinline__ float4 lerp4(float4 p, float4 q, float t) {
+    return make_float4(lerpf_(p.x, q.x, t), lerpf_(p.y, q.y, t), lerpf_(p.z, q.z, t), lerpf_(p.w, q.w, t));
+}
+
+__device__ __forceinline__ float4 sample3d(const Tex3 &t, float u, float v, float w) {
+    int i0, i1, j0, j1, k0, k1;
+    float a, b, c;
+    tex_axis(u, t.nx, i0, i1, a);
+    tex_axis(v, t.ny, j0, j1, b);
+    tex_axis(w, t.nz, k0, k1, c);
+    const size_t nx = (size_t)t.nx, nxy = (size_t)t.nx * (size_t)t.ny;
+    const float4 *D = t.d;
+    const size_t r00 = k0 * nxy + j0 * nx, r10 = k0 * nxy + j1 * nx;
+    const size_t r01 = k1 * nxy + j0 * nx, r11 = k1 * nxy + j1 * nx;
+    float4 t000 = D[r00 + i0], t100 = D[r00 + i1];
+    float4 t010 = D[r10 + i0], t110 = D[r10 + i1];
+    float4 t001 = D[r01 + i0], t101 = D[r01 + i1];
+    float4 t011 = D[r11 + i0], t111 = D[r11 + i1];
+    float4 x00 = lerp4(t000, t100, a);
+    float4 x10 = lerp4(t010, t110, a);
+    float4 x01 = lerp4(t001, t101, a);
+    float4 x11 = lerp4(t011, t111, a);
+    return lerp4(lerp4(x00, x10, b), lerp4(x01, x11, b), c);
+}
+
+__device__ __forceinline__ float4 sample2d(const Tex2 &t, float u, float v) {
+    int i0, i1, j0, j1;
+    float a, b;
+    tex_axis(u, t.nx, i0, i1, a);
+    tex_axis(v, t.ny, j0, j1, b);
+    const size_t nx = (size_t)t.nx;
+    const float4 *D = t.d;
+    float4 y0 = lerp4(D[j0 * nx + i0], D[j0 * nx + i1], a);
+    float4 y1 = lerp4(D[j1 * nx + i0], D[j1 * nx + i1], a);
+    return lerp4(y0, y1, b);
+}
+
+// rs.cl:123-176: LES table coordinates of a position.
+__device__ __forceinline__ float4 wind_index(float4 pos, const F16 &d, const F16 &sd) {
+    float4 c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    uint32_t spacing = __float_as_uint(d.s[7]);
+    if (spacing == 7u) {
+        float cx = sd.s[8] + 0.5f * sd.s[12];
+        float cy = sd.s[9] + 0.5f * sd.s[13];
+        float rx = pos.x - cx, ry = pos.y - cy, rz = pos.z - 0.0f;
+        c.x = __builtin_copysignf(d.s[0], rx) * rf_log1pf(d.s[4] * __builtin_fabsf(rx)) + d.s[8];
+        c.y = __builtin_copysignf(d.s[1], ry) * rf_log1pf(d.s[5] * __builtin_fabsf(ry)) + d.s[9];
+        c.z = __builtin_copysignf(d.s[2], rz) * rf_log1pf(d.s[6] * __builtin_fabsf(rz)) + d.s[10];
+    } else if (spacing == 0u) {
+        c.x = rf_fma(pos.x, d.s[0], d.s[4]);
+        c.y = rf_fma(pos.y, d.s[1], d.s[5]);
+        c.z = rf_fma(pos.z, d.s[2], d.s[6]);
+    }
+    return c;
+}
+
+// rs.cl:35-44
+__device__ __forceinline__ float4 rand4(uint4 &s) {
+    const float n = 1.0f / 2147483647.0f;
+    s.x = (s.x * 16807u) & 0x7FFFFFFFu;
+    s.y = (s.y * 16807u) & 0x7FFFFFFFu;
+    s.z = (s.z * 16807u) & 0x7FFFFFFFu;
+    s.w = (s.w * 16807u) & 0x7FFFFFFFu;
+    return make_float4((float)s.x * n, (float)s.y * n, (float)s.z * n, (float)s.w * n);
+}
+
+__device__ __forceinline__ bool is_outside(float4 p, const F16 &sd) {
+    float ex = sd.s[8] + sd.s[12], ey = sd.s[9] + sd.s[13], ez = sd.s[10] + sd.s[14];
+    return (p.x <= sd.s[8]) | (p.y <= sd.s[9]) | (p.z <= sd.s[10]) | (p.x >= ex) | (p.y >= ey) | (p.z >= ez);
+}
+
+// rs.cl:268-280 with cos^2(atan2(z, rho)) = rho^2 / (rho^2 + z^2).
+__device__ __forceinline__ float4 ellipsoid_rcs(float4 p, const float4 *table, const F4 &td) {
+    float fidx = rf_clamp(rf_fma(p.w, td.s[0], td.s[1]), 0.0f, td.s[2]);
+    float4 xz = table[(uint32_t)fidx];
+    float rho2 = p.x * p.x + p.y * p.y;
+    float cb2 = rf_div(rho2, rho2 + p.z * p.z);
+    return make_float4(xz.x, xz.y, xz.x + (xz.z - xz.x) * cb2, xz.y + (xz.w - xz.y) * cb2);
+}
+
+// rs.cl:54-85
+__device__ __forceinline__ float4 qmul(float4 l, float4 r) {
+    return make_float4(((l.w * r.x - l.z * r.y) + l.y * r.z) + l.x * r.w,
+                       ((l.w * r.y + l.z * r.x) + l.y * r.w) - l.x * r.z,
+                       ((l.w * r.z + l.z * r.w) - l.y * r.x) + l.x * r.y,
+                       ((l.w * r.w - l.z * r.z) - l.y * r.y) - l.x * r.x);
+}
+__device__ __forceinline__ float4 qconj(float4 q) { return make_float4(-q.x, -q.y, -q.z, q.w); }
+__device__ __forceinline__ float4 qrot(float4 v, float4 q) { return qmul(qmul(q, v), qconj(q)); }
+__device__ __forceinline__ float4 norm4(float4 q) {
+    float n = rf_len4(q.x, q.y, q.z, q.w);
+    return make_float4(rf_div(q.x, n), rf_div(q.y, n), rf_div(q.z, n), rf_div(q.w, n));
+}
+
+// rs.cl:282-359
+__device__ float4 debris_rcs(float4 pos, float4 ori, const Tex2 &rr, const Tex2 &ri, const F16 &d, uint32_t concept) {
+    float el = rf_atan2f(pos.z, rf_len2(pos.x, pos.y));
+    float az = rf_atan2f(pos.x, pos.y);
+    float se, ce, sa, ca;
+    rf_sincosf(0.5f * (el + RSD_PI_2_F), &se, &ce);
+    rf_sincosf(0.5f * az, &sa, &ca);
+    float4 oc = make_float4((-se * ca + ce * sa) * RSD_SQRT1_2_F, (ce * ca + se * sa) * RSD_SQRT1_2_F,
+                            (se * ca + ce * sa) * RSD_SQRT1_2_F, (ce * ca + (-se) * sa) * RSD_SQRT1_2_F);
+    float4 R = qmul(ori, oc);
+    float4 q = make_float4(R.x, R.z, -R.y, R.w);
+    float alpha, beta, gamma;
+    float beta_arg = ((q.w * q.w + q.z * q.z) - q.y * q.y) - q.x * q.x;
+    if (beta_arg > 0.999847f || beta_arg < -0.999847f) {
+        alpha = 0.0f;
+        beta = 0.0f;
+        gamma = (rf_sign(q.z) * rf_acosf(q.w)) * 2.0f;
+    } else {
+        alpha = rf_atan2f(q.y * q.z - q.w * q.x, q.x * q.z + q.w * q.y);
+        beta = rf_acosf(((q.w * q.w + q.z * q.z) - q.y * q.y) - q.x * q.x);
+        gamma = rf_atan2f(q.y * q.z + q.w * q.x, q.w * q.y - q.x * q.z);
+    }
+    float u = rf_fma(alpha, d.s[0], d.s[4]);
+    float v = rf_fma(beta, d.s[1], d.s[5]);
+    float4 re = sample2d(rr, u, v);
+    float4 im = sample2d(ri, u, v);
+    float sg, cg;
+    rf_sincosf(gamma, &sg, &cg);
+    float4 ss;
+    ss.x = cg * (cg * re.x - re.z * sg) - sg * (cg * re.z - re.y * sg);
+    ss.y = cg * (cg * im.x - im.z * sg) - sg * (cg * im.z - im.y * sg);
+    ss.z = cg * (cg * re.y + re.z * sg) + sg * (cg * re.z + re.x * sg);
+    ss.w = cg * (cg * im.y + im.z * sg) + sg * (cg * im.z + im.x * sg);
+    if (concept & 1u) {
+        float hv_r = cg * (cg * re.z + re.x * sg) - sg * (cg * re.y + re.z * sg);
+        float hv_i = cg * (cg * im.z + im.x * sg) - sg * (cg * im.y + im.z * sg);
+        float vh_r = cg * (cg * re.z - re.y * sg) + sg * (cg * re.x - re.z * sg);
+        float vh_i = cg * (cg * im.z - im.y * sg) + sg * (cg * im.x - im.z * sg);
+        ss.x += vh_r;
+        ss.y += vh_i;
+        ss.z += hv_r;
+        ss.w += hv_i;
+    }
+    return ss;
+}
+
+// ---------------------------------------------------------------------------
+// Pulse formation arguments shared by all scatterer kernels.
+// ---------------------------------------------------------------------------
+struct PulseK {
+    F16 sd;                 // sim_desc of the pulse (beam in slots 0-2)
+    double two_over_lambda; // turns per metre of two-way path
+    const float *angw; F4 angw_desc;
+    const float *rangew; F4 rangew_desc;
+    const float *gate_r;    // binary32 gate ranges (start + repeated adds)
+    int R;
+    float gate_start, gate_inv_delta;
+    float sup_lo, sup_hi;   // range-offset interval outside which the weight is 0
+    int dense;              // 1: every gate may be weighted
+    uint32_t phase_mode;
+    double *partials;       // [rows][4][R]
+    int row0;               // first partial row of this launch
+};
+
+__device__ __forceinline__ float table_lerp(float x, const float *W, const F4 &d) {
+    // rs.cl:1067-1093 / 1192-1206: clamp, fract, mix of two table entries
+    float x0 = rf_clamp(rf_fma(x, d.s[0], d.s[1]), 0.0f, d.s[2]);
+    float x1 = rf_clamp(rf_fma(x, d.s[0], d.s[1] + 1.0f), 0.0f, d.s[2]);
+    float ip;
+    float dec = rf_fract(x0, &ip);
+    return rf_mix(W[(uint32_t)ip], W[(uint32_t)__builtin_floorf(x1)], dec);
+}
+
+// Signal x angular weight of one scatterer (rs.cl:1050-1110, 1175-1180).
+__device__ __forceinline__ bool scat_signal(float4 p, float4 rcs, const PulseK &k, float4 &sa, float &r) {
+    r = rf_len3(p.x, p.y, p.z);
+    float nx = rf_div(p.x, r), ny = rf_div(p.y, r), nz = rf_div(p.z, r);
+    float angle = rf_acosf((k.sd.s[0] * nx + k.sd.s[1] * ny) + k.sd.s[2] * nz);
+    float wa = table_lerp(angle, k.angw, k.angw_desc);
+    if (wa == 0.0f) return false;
+    float atten = 1.0f / (r * r);
+    float ss, cc;
+    if (k.phase_mode == RSD_PHASE_F64) {
+        double turns = (double)r * k.two_over_lambda;
+        float t = (float)(turns - floor(turns));
+        rf_sincos_turns(t, &ss, &cc);
+    } else {
+        rf_sincosf(r * k.sd.s[4], &ss, &cc);
+    }
+    sa.x = ((rcs.x * cc - rcs.y * (-ss)) * atten) * wa;
+    sa.y = ((rcs.y * cc + rcs.x * (-ss)) * atten) * wa;
+    sa.z = ((rcs.z * cc - rcs.w * (-ss)) * atten) * wa;
+    sa.w = ((rcs.w * cc + rcs.z * (-ss)) * atten) * wa;
+    return true;
+}
+
+// Fold one scatterer into the block's binary64 gate sums (rs.cl:1182-1217,
+// gate-sparse: only gates whose range offset lies inside the weight's support).
+__device__ __forceinline__ void fold_gates(float r, float4 sa, const PulseK &k, double *acc) {
+    float klo_f, khi_f;
+    if (k.dense) {
+        klo_f = 0.0f;
+        khi_f = (float)(k.R - 1);
+    } else {
+        klo_f = __builtin_fmaxf(__builtin_ceilf((r - k.sup_hi - k.gate_start) * k.gate_inv_delta), 0.0f);
+        khi_f = __builtin_fminf(__builtin_floorf((r - k.sup_lo - k.gate_start) * k.gate_inv_delta), (float)(k.R - 1));
+    }
+    if (!(klo_f <= khi_f)) return;
+    const int klo = (int)klo_f, khi = (int)khi_f;
+    const int R = k.R;
+    for (int g = klo; g <= khi; g++) {
+        float wr = table_lerp(r - k.gate_r[g], k.rangew, k.rangew_desc);
+        if (wr == 0.0f) continue;
+        atomicAdd(&acc[g], (double)(wr * sa.x));
+        atomicAdd(&acc[R + g], (double)(wr * sa.y));
+        atomicAdd(&acc[2 * R + g], (double)(wr * sa.z));
+        atomicAdd(&acc[3 * R + g], (double)(wr * sa.w));
+    }
+}
+
+__device__ __forceinline__ void acc_clear(double *acc, int n) {
+    for (int j = threadIdx.x; j < n; j += blockDim.x) acc[j] = 0.0;
+    __syncthreads();
+}
+
+__device__ __forceinline__ void acc_flush(const double *acc, const PulseK &k) {
+    __syncthreads();
+    double *row = k.partials + (size_t)(k.row0 + blockIdx.x) * 4 * k.R;
+    for (int j = threadIdx.x; j < 4 * k.R; j += blockDim.x) row[j] = acc[j];
+}
+
+// ---------------------------------------------------------------------------
+// Rain (type 0): el_atts (rs.cl:653-739) or bg_atts (rs.cl:533-580), fused with
+// the pulse.  Template flags select advance and/or pulse.
+// ---------------------------------------------------------------------------
+struct RainK {
+    float4 *pos, *vel, *rcs;
+    uint4 *rnd;
+    unsigned long long n;
+    Tex3 wind;
+    F16 les;
+    F16 sd;                 // sim_desc of the advance
+    const float4 *drop;
+    F4 drop_desc;
+    uint32_t dragged;
+    PulseK pk;
+};
+
+// Returns true when the scatterer respawned.
+__device__ __forceinline__ bool rain_advance(const RainK &a, size_t i, float4 &pos, float4 &vel) {
+    const F16 &sd = a.sd;
+    const float dt = sd.s[11];
+    if (a.dragged) {
+        const uint32_t concept = __float_as_uint(sd.s[5]);
+        pos.x = rf_fma(vel.x, dt, pos.x);
+        pos.y = rf_fma(vel.y, dt, pos.y);
+        pos.z = rf_fma(vel.z, dt, pos.z);
+        pos.w = rf_fma(vel.w, 0.0f, pos.w);
+        const float aom = rf_div(0.003006012f, pos.w);
+        if (is_outside(pos, sd)) {
+            uint4 seed = a.rnd[i];
+            float4 r = rand4(seed);
+            a.rnd[i] = seed;
+            pos.x = rf_fma(r.x, sd.s[12], sd.s[8]);
+            pos.y = rf_fma(r.y, sd.s[13], sd.s[9]);
+            pos.z = rf_fma(r.z, sd.s[14], sd.s[10]);
+            vel = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            return true;
+        }
+        float4 c = wind_index(pos, a.les, sd);
+        float4 bg = sample3d(a.wind, c.x, c.y, c.z);
+        float4 dv = make_float4(bg.x - vel.x, bg.y - vel.y, bg.z - vel.z, bg.w - vel.w);
+        float dva = rf_len3(dv.x, dv.y, dv.z);
+        if (dva > 1.0e-3f) {
+            float re = (6.7308e4f * (2.0f * pos.w)) * dva;
+            float cd = (rf_div(24.0f, re) + rf_div(6.0f, 1.0f + rf_sqrt(re))) + 0.4f;
+            float k = (((0.5f * 1.225f) * cd) * aom) * dva;
+            float4 du = make_float4(k * dv.x + 0.0f, k * dv.y + 0.0f, k * dv.z + (-9.8f), k * dv.w + 0.0f);
+            vel.x = vel.x + du.x * dt;
+            vel.y = vel.y + du.y * dt;
+            vel.z = vel.z + du.z * dt;
+            vel.w = vel.w + du.w * 0.0f;
+            if ((concept & 8u) && rf_len4(vel.x, vel.y, vel.z, vel.w) > __builtin_fmaxf(1.0f, 3.0f * rf_len4(bg.x, bg.y, bg.z, bg.w))) {
+                vel.x = bg.x + 0.0f * dt;
+                vel.y = bg.y + 0.0f * dt;
+                vel.z = bg.z + (-9.8f) * dt;
+                vel.w = bg.w + 0.0f * 0.0f;
+            }
+        } else {
+            vel.x = vel.x + 0.0f * dt;
+            vel.y = vel.y + 0.0f * dt;
+            vel.z = vel.z + (-9.8f) * dt;
+            vel.w = vel.w + 0.0f * 0.0f;
+        }
+        return false;
+    }
+    pos.x = pos.x + vel.x * dt;
+    pos.y = pos.y + vel.y * dt;
+    pos.z = pos.z + vel.z * dt;
+    pos.w = pos.w + vel.w * 0.0f;
+    if (is_outside(pos, sd)) {
+        uint4 seed = a.rnd[i];
+        float4 r = rand4(seed);
+        a.rnd[i] = seed;
+        pos.x = r.x * sd.s[12] + sd.s[8];
+        pos.y = r.y * sd.s[13] + sd.s[9];
+        pos.z = r.z * sd.s[14] + sd.s[10];
+        vel = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return true;
+    }
+    float4 c = wind_index(pos, a.les, sd);
+    vel = sample3d(a.wind, c.x, c.y, c.z);
+    return false;
+}
+
+template <bool ADV, bool PULSE>
+__global__ __launch_bounds__(256) void k_rain(RainK a) {
+    extern __shared__ double acc[];
+    if (PULSE) acc_clear(acc, 4 * a.pk.R);
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += stride) {
+        float4 p = a.pos[i];
+        bool stale = __builtin_signbit(p.w);
+        p.w = __builtin_fabsf(p.w);
+        float4 rcs;
+        bool have_rcs = false;
+        if (ADV) {
+            float4 v = a.vel[i];
+            const float4 p_in = p;
+            bool respawned = rain_advance(a, i, p, v);
+            if (respawned) {
+                if (!stale) {
+                    rcs = ellipsoid_rcs(p_in, a.drop, a.drop_desc);
+                    a.rcs[i] = rcs;
+                    stale = true;
+                    have_rcs = true;
+                }
+            } else {
+                stale = false;
+            }
+            a.pos[i] = make_float4(p.x, p.y, p.z, stale ? -p.w : p.w);
+            a.vel[i] = v;
+        }
+        if (PULSE) {
+            if (!have_rcs) rcs = stale ? a.rcs[i] : ellipsoid_rcs(p, a.drop, a.drop_desc);
+            float4 sa;
+            float r;
+            if (scat_signal(p, rcs, a.pk, sa, r)) fold_gates(r, sa, a.pk, acc);
+        }
+    }
+    if (PULSE) acc_flush(acc, a.pk);
+}
+
+// ---------------------------------------------------------------------------
+// Debris (types >= 1): db_atts (rs.cl:744-907, uniform respawn placement) and
+// db_rcs (rs.cl:909-919), fused with the pulse.
+// ---------------------------------------------------------------------------
+struct DebrisK {
+    float4 *pos, *ori, *vel, *tum, *rcs;
+    uint4 *rnd;
+    unsigned long long begin, end;
+    Tex3 wind;
+    F16 les;
+    F16 sd;
+    Tex2 adm_cd, adm_cm;
+    F16 adm_desc;
+    Tex2 rcs_re, rcs_im;    // table of the advance
+    F16 rcs_desc;
+    Tex2 prcs_re, prcs_im;  // table of the pulse's refresh
+    F16 prcs_desc;
+    uint32_t refresh;
+    uint32_t pulse_concept;
+    uint32_t reuse_rcs;     // advance and refresh use the same table and concept
+    PulseK pk;
+};
+
+__device__ bool debris_advance(const DebrisK &a, size_t i, float4 &pos, float4 &ori, float4 &vel, float4 &tum, float4 &rcs) {
+    const F16 &sd = a.sd;
+    const float dt = sd.s[11];
+    const uint32_t concept = __float_as_uint(sd.s[5]);
+    ori = norm4(qmul(ori, tum));
+    pos.x = pos.x + vel.x * dt;
+    pos.y = pos.y + vel.y * dt;
+    pos.z = pos.z + vel.z * dt;
+    pos.w = pos.w + vel.w * 0.0f;
+    if (is_outside(pos, sd)) {
+        uint4 seed = a.rnd[i];
+        float4 r = rand4(seed);
+        pos.x = rf_fma(r.x, sd.s[12], sd.s[8]);
+        pos.y = rf_fma(r.y, sd.s[13], sd.s[9]);
+        pos.z = sd.s[10] + RSD_MIN_HEIGHT;
+        r = rand4(seed);
+        float c0 = rf_sqrt(-2.0f * rf_logf(r.x));
+        float c1 = rf_sqrt(-2.0f * rf_logf(r.y));
+        float c2 = rf_sqrt(-2.0f * rf_logf(r.z));
+        float c3 = r.w;
+        r = rand4(seed);
+        const float two_pi = 2.0f * RSD_PI_F;
+        c0 *= rf_cosf(two_pi * r.x);
+        c1 *= rf_cosf(two_pi * r.y);
+        c2 *= rf_cosf(two_pi * r.z);
+        float sin_th_2, cos_th_2;
+        rf_sincosf(RSD_PI_F * c3, &sin_th_2, &cos_th_2);
+        float sqxy = rf_sqrt(c0 * c0 + c2 * c2);
+        float sqxyz = rf_sqrt((c0 * c0 + c1 * c1) + c2 * c2);
+        float sa, ca;
+        rf_sincosf(0.5f * rf_acosf(rf_div(c1, sqxyz)), &sa, &ca);
+        ori.x = (rf_div((c0 * ca) * sin_th_2, sqxyz) - rf_div(rf_div((c0 * c1) * sin_th_2, sqxyz) * sa, sqxy))
+                + rf_div((c2 * cos_th_2) * sa, sqxy);
+        ori.y = rf_div(sin_th_2, sqxyz) * (sqxy * sa + c1 * ca);
+        ori.z = (rf_div((c2 * ca) * sin_th_2, sqxyz) - rf_div((c0 * cos_th_2) * sa, sqxy))
+                - rf_div(rf_div(((c1 * c2) * sin_th_2) * sa, sqxy), sqxyz);
+        ori.w = cos_th_2 * ca;
+        vel = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        tum = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+        rcs = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        a.rnd[i] = seed;
+        return true;
+    }
+    float4 c = wind_index(pos, a.les, sd);
+    float4 bg = sample3d(a.wind, c.x, c.y, c.z);
+    float4 ur = make_float4(bg.x - vel.x, bg.y - vel.y, bg.z - vel.z, bg.w - vel.w);
+    float4 uh = qrot(norm4(ur), qconj(ori));
+    float beta = rf_acosf(uh.x);
+    float alpha = rf_atan2f(uh.z, uh.y);
+    if (alpha < 0.0f) {
+        alpha = RSD_PI_F + alpha;
+        beta = -beta;
+    }
+    const F16 &ad = a.adm_desc;
+    float cu = rf_fma(beta, ad.s[0], ad.s[4]);
+    float cv = rf_fma(alpha, ad.s[1], ad.s[5]);
+    float4 cd = sample2d(a.adm_cd, cu, cv);
+    float4 cm = sample2d(a.adm_cm, cu, cv);
+    const float Ta = ad.s[15];
+    cd = qrot(cd, ori);
+    float urn = (ur.x * ur.x + ur.y * ur.y) + ur.z * ur.z;
+    float ta_urn = Ta * urn;
+    float4 du = make_float4(ta_urn * cd.x + 0.0f, ta_urn * cd.y + 0.0f, ta_urn * cd.z + (-9.8f), ta_urn * cd.w + 0.0f);
+    float dwx = rf_radians((ta_urn * ad.s[12]) * cm.x);
+    float dwy = rf_radians((ta_urn * ad.s[13]) * cm.y);
+    float dwz = rf_radians((ta_urn * ad.s[14]) * cm.z);
+    if ((concept & 8u) && rf_len2(vel.x + du.x * dt, vel.y + du.y * dt) > 3.0f * rf_len2(bg.x, bg.y)) {
+        vel.x = bg.x;
+        vel.y = bg.y;
+        vel.z = vel.z + du.z * dt;
+    } else {
+        vel.x = vel.x + du.x * dt;
+        vel.y = vel.y + du.y * dt;
+        vel.z = vel.z + du.z * dt;
+        vel.w = vel.w + du.w * 0.0f;
+    }
+    float sx, cx, sy, cy, sz, cz;
+    rf_sincosf(dwx * dt, &sx, &cx);
+    rf_sincosf(dwy * dt, &sy, &cy);
+    rf_sincosf(dwz * dt, &sz, &cz);
+    float4 tq = make_float4((cx * sy) * sz + (sx * cy) * cz, (cx * sy) * cz - (sx * cy) * sz,
+                            (cx * cy) * sz + (sx * sy) * cz, (cx * cy) * cz - (sx * sy) * sz);
+    tum = norm4(tq);
+    rcs = debris_rcs(pos, ori, a.rcs_re, a.rcs_im, a.rcs_desc, concept);
+    return false;
+}
+
+template <bool ADV, bool PULSE>
+__global__ __launch_bounds__(256) void k_debris(DebrisK a) {
+    extern __shared__ double acc[];
+    if (PULSE) acc_clear(acc, 4 * a.pk.R);
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = a.begin + (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.end; i += stride) {
+        float4 p = a.pos[i], o = a.ori[i];
+        float4 rcs;
+        bool rcs_known = false, respawned = false;
+        if (ADV) {
+            float4 v = a.vel[i], t = a.tum[i];
+            respawned = debris_advance(a, i, p, o, v, t, rcs);
+            a.pos[i] = p;
+            a.ori[i] = o;
+            a.vel[i] = v;
+            a.tum[i] = t;
+            rcs_known = true;
+        }
+        if (PULSE) {
+            if (a.refresh && ADV && !respawned && a.reuse_rcs) {
+                // the advance just evaluated the same table at the same pose
+            } else if (a.refresh) {
+                rcs = debris_rcs(p, o, a.prcs_re, a.prcs_im, a.prcs_desc, a.pulse_concept);
+                rcs_known = true;
+            } else if (!rcs_known) {
+                rcs = a.rcs[i];
+            }
+        }
+        if (rcs_known) a.rcs[i] = rcs;
+        if (PULSE) {
+            float4 sa;
+            float r;
+            if (scat_signal(p, rcs, a.pk, sa, r)) fold_gates(r, sa, a.pk, acc);
+        }
+    }
+    if (PULSE) acc_flush(acc, a.pk);
+}
+
+// Sum the per-block partial rows of one pulse: one block per output value.
+__global__ __launch_bounds__(256) void k_finalize(const double *partials, int rows, int n4r, double *out) {
+    __shared__ double red[256];
+    const int j = blockIdx.x;
+    double s = 0.0;
+    for (int r = threadIdx.x; r < rows; r += blockDim.x) s += partials[(size_t)r * n4r + j];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[j] = red[0];
+}
+
+// Device evaluation of the binary32 functions the kernels use (self-test).
+__global__ void k_math_probe(int fn, const float *a, const float *b, float *out, unsigned long long n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float x = a[i], y = b[i], s, c;
+    float r = 0.0f;
+    switch (fn) {
+        case 0: r = rf_log1pf(x); break;
+        case 1: r = rf_logf(x); break;
+        case 2: rf_sincosf(x, &s, &c); r = s; break;
+        case 3: rf_sincosf(x, &s, &c); r = c; break;
+        case 4: r = rf_atan2f(x, y); break;
+        case 5: r = rf_acosf(x); break;
+        case 6: r = rf_sqrt(x); break;
+        case 7: r = rf_div(x, y); break;
+        case 8: rf_sincos_turns(x, &s, &c); r = s; break;
+        case 9: rf_sincos_turns(x, &s, &c); r = c; break;
+        default: break;
+    }
+    out[i] = r;
+}
+
+// Mark every rain scatterer's uploaded rcs as authoritative (sign bit of pos.w).
+__global__ void k_flag_rain(float4 *pos, unsigned long long n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        float4 p = pos[i];
+        p.w = -__builtin_fabsf(p.w);
+        pos[i] = p;
+    }
+}
+
+// Download view of the streams: |pos.w|, reconstructed rain rcs, and the
+// rs.cl:1050-1110 signal / auxiliary quantities for the last pulse's beam.
+struct ViewK {
+    const float4 *pos, *rcs, *aux_in;
+    float4 *pos_out, *rcs_out, *sig_out, *aux_out;
+    unsigned long long n, n_rain;
+    const float4 *drop;
+    F4 drop_desc;
+    PulseK pk;
+    int have_pulse;
+};
+
+__global__ void k_view(ViewK a) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    float4 p = a.pos[i];
+    bool stale = __builtin_signbit(p.w);
+    float4 rcs;
+    if (i < a.n_rain) {
+        p.w = __builtin_fabsf(p.w);
+        rcs = stale ? a.rcs[i] : ellipsoid_rcs(p, a.drop, a.drop_desc);
+    } else {
+        rcs = a.rcs[i];
+    }
+    a.pos_out[i] = p;
+    a.rcs_out[i] = rcs;
+    float4 aux = a.aux_in[i];
+    float4 sig = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (a.have_pulse) {
+        const PulseK &k = a.pk;
+        float r = rf_len3(p.x, p.y, p.z);
+        float nx = rf_div(p.x, r), ny = rf_div(p.y, r), nz = rf_div(p.z, r);
+        float angle = rf_acosf((k.sd.s[0] * nx + k.sd.s[1] * ny) + k.sd.s[2] * nz);
+        float wa = table_lerp(angle, k.angw, k.angw_desc);
+        float atten = 1.0f / (r * r);
+        float ss, cc;
+        if (k.phase_mode == RSD_PHASE_F64) {
+            double turns = (double)r * k.two_over_lambda;
+            rf_sincos_turns((float)(turns - floor(turns)), &ss, &cc);
+        } else {
+            rf_sincosf(r * k.sd.s[4], &ss, &cc);
+        }
+        sig = make_float4((rcs.x * cc - rcs.y * (-ss)) * atten, (rcs.y * cc + rcs.x * (-ss)) * atten,
+                          (rcs.z * cc - rcs.w * (-ss)) * atten, (rcs.w * cc + rcs.z * (-ss)) * atten);
+        aux.x = r;
+        aux.y = aux.y + k.sd.s[15];
+        aux.w = wa;
+    }
+    a.sig_out[i] = sig;
+    a.aux_out[i] = aux;
+}
+
+// ---------------------------------------------------------------------------
+// Host side of the C-ABI.
+// ---------------------------------------------------------------------------
+struct rsd_engine {
+    int device;
+    hipStream_t stream;
+    char err[512];
+    size_t n, n_rain;
+    size_t origins[RSD_MAX_TYPES], counts[RSD_MAX_TYPES];
+    float4 *pos, *vel, *ori, *tum, *rcs, *aux;
+    uint4 *rnd;
+    float4 *view;           // download staging: 4 x n float4
+    float4 *wind[2];
+    int wnx[2], wny[2], wnz[2];
+    F16 les[2];
+    float4 *adm_cd[RSD_MAX_TABLES], *adm_cm[RSD_MAX_TABLES];
+    int adm_nx[RSD_MAX_TABLES], adm_ny[RSD_MAX_TABLES];
+    F16 adm_desc[RSD_MAX_TABLES];
+    float4 *rcs_re[RSD_MAX_TABLES], *rcs_im[RSD_MAX_TABLES];
+    int rcs_nx[RSD_MAX_TABLES], rcs_ny[RSD_MAX_TABLES];
+    F16 rcs_desc[RSD_MAX_TABLES];
+    float4 *drop;
+    F4 drop_desc;
+    float *rangew;
+    float rangew_h[1024];
+    int rangew_n;
+    F4 rangew_desc;
+    float *angw;
+    int angw_n;
+    F4 angw_desc;
+    float *gate_r;
+    int gate_R;
+    float gate_start, gate_delta;
+    double *partials;
+    size_t partial_cap;
+    double *pulse_d;
+    double *pulse_h;
+    int have_pending;
+    rsd_step pending;
+    int have_pulse;
+    rsd_pulse last_pulse;
+    int block, max_blocks;
+    hipEvent_t ev0, ev1;
+    int prof_on, prof_n;
+    hipEvent_t *prof_ev;     // start/stop pairs around the rain kernel
+    int *prof_kind;          // bit 0 advance, bit 1 pulse
+};
+
+#define RSD_PROF_CAP 16384
+
+static int fail(rsd_engine *e, int code, const char *fmt, ...) {
+    if (e) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(e->err, sizeof(e->err), fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+#define HIPCHK(e, call)                                                                          \
+    do {                                                                                         \
+        hipError_t _r = (call);                                                                  \
+        if (_r != hipSuccess) return fail((e), RSD_ERR_HIP, "%s: %s", #call, hipGetErrorString(_r)); \
+    } while (0)
+
+static void dfree(void *p) {
+    if (p) (void)hipFree(p);
+}
+
+extern "C" int rsd_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+extern "C" int rsd_device_props(int device_index, int *cus, int *max_wg, size_t *lds) {
+    hipDeviceProp_t prop;
+    if (device_index < 0 || device_index >= rsd_device_count()) return RSD_ERR_NO_DEVICE;
+    if (hipGetDeviceProperties(&prop, device_index) != hipSuccess) return RSD_ERR_HIP;
+    if (cus) *cus = prop.multiProcessorCount;
+    if (max_wg) *max_wg = prop.maxThreadsPerBlock;
+    if (lds) *lds = prop.sharedMemPerBlock;
+    return RSD_OK;
+}
+
+extern "C" int rsd_open(int device_index, rsd_engine **out) {
+    if (!out) return RSD_ERR_ARG;
+    *out = NULL;
+    int n = rsd_device_count();
+    if (n <= 0 || device_index < 0 || device_index >= n) return RSD_ERR_NO_DEVICE;
+    rsd_engine *e = (rsd_engine *)calloc(1, sizeof(rsd_engine));
+    if (!e) return RSD_ERR_ALLOC;
+    e->device = device_index;
+    e->block = 256;
+    e->max_blocks = 2048;
+    if (hipSetDevice(device_index) != hipSuccess || hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess ||
+        hipHostMalloc((void **)&e->pulse_h, 4 * RSD_MAX_GATES * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void **)&e->pulse_d, 4 * RSD_MAX_GATES * sizeof(double)) != hipSuccess) {
+        free(e);
+        return RSD_ERR_HIP;
+    }
+    *out = e;
+    return RSD_OK;
+}
+
+static void free_state(rsd_engine *e) {
+    dfree(e->pos); dfree(e->vel); dfree(e->ori); dfree(e->tum); dfree(e->rcs); dfree(e->aux);
+    dfree(e->rnd); dfree(e->view);
+    e->pos = e->vel = e->ori = e->tum = e->rcs = e->aux = e->view = NULL;
+    e->rnd = NULL;
+}
+
+extern "C" void rsd_close(rsd_engine *e) {
+    if (!e) return;
+    (void)hipSetDevice(e->device);
+    (void)hipStreamSynchronize(e->stream);
+    free_state(e);
+    for (int s = 0; s < 2; s++) dfree(e->wind[s]);
+    for (int t = 0; t < RSD_MAX_TABLES; t++) {
+        dfree(e->adm_cd[t]); dfree(e->adm_cm[t]); dfree(e->rcs_re[t]); dfree(e->rcs_im[t]);
+    }
+    dfree(e->drop); dfree(e->rangew); dfree(e->angw); dfree(e->gate_r); dfree(e->partials);
+    dfree(e->pulse_d);
+    if (e->pulse_h) (void)hipHostFree(e->pulse_h);
+    if (e->prof_ev) {
+        for (int i = 0; i < 2 * RSD_PROF_CAP; i++) (void)hipEventDestroy(e->prof_ev[i]);
+        free(e->prof_ev);
+        free(e->prof_kind);
+    }
+    (void)hipEventDestroy(e->ev0);
+    (void)hipEventDestroy(e->ev1);
+    (void)hipStreamDestroy(e->stream);
+    free(e);
+}
+
+extern "C" const char *rsd_last_error(const rsd_engine *e) { return e ? e->err : "no engine"; }
+
+extern "C" int rsd_set_launch(rsd_engine *e, int block_size, int max_blocks) {
+    if (!e) return RSD_ERR_ARG;
+    if (block_size == 0) block_size = 256;
+    if (max_blocks == 0) max_blocks = 2048;
+    if (block_size != 64 && block_size != 128 && block_size != 256) return fail(e, RSD_ERR_ARG, "block size %d not in {64,128,256}", block_size);
+    if (max_blocks < 1 || max_blocks > 65536) return fail(e, RSD_ERR_ARG, "max_blocks %d out of range", max_blocks);
+    e->block = block_size;
+    e->max_blocks = max_blocks;
+    return RSD_OK;
+}
+
+extern "C" int rsd_alloc_state(rsd_engine *e, size_t num_scats, const size_t *origins, const size_t *counts) {
+    if (!e || !origins || !counts || num_scats == 0) return fail(e, RSD_ERR_ARG, "bad population");
+    size_t total = 0;
+    for (int k = 0; k < RSD_MAX_TYPES; k++) {
+        if (counts[k] && origins[k] + counts[k] > num_scats) return fail(e, RSD_ERR_ARG, "type %d range beyond num_scats", k);
+        total += counts[k];
+    }
+    if (total != num_scats || origins[0] != 0) return fail(e, RSD_ERR_ARG, "type counts (%zu) != num_scats (%zu)", total, num_scats);
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    free_state(e);
+    e->n = num_scats;
+    e->n_rain = counts[0];
+    memcpy(e->origins, origins, sizeof(e->origins));
+    memcpy(e->counts, counts, sizeof(e->counts));
+    const size_t b = num_scats * sizeof(float4);
+    HIPCHK(e, hipMalloc((void **)&e->pos, b));
+    HIPCHK(e, hipMalloc((void **)&e->vel, b));
+    HIPCHK(e, hipMalloc((void **)&e->ori, b));
+    HIPCHK(e, hipMalloc((void **)&e->tum, b));
+    HIPCHK(e, hipMalloc((void **)&e->rcs, b));
+    HIPCHK(e, hipMalloc((void **)&e->aux, b));
+    HIPCHK(e, hipMalloc((void **)&e->rnd, num_scats * sizeof(uint4)));
+    HIPCHK(e, hipMemsetAsync(e->rcs, 0, b, e->stream));
+    HIPCHK(e, hipMemsetAsync(e->aux, 0, b, e->stream));
+    e->have_pending = 0;
+    e->have_pulse = 0;
+    return RSD_OK;
+}
+
+static int blocks_for(const rsd_engine *e, size_t n) {
+    size_t b = (n + (size_t)e->block - 1) / (size_t)e->block;
+    if (b < 1) b = 1;
+    if (b > (size_t)e->max_blocks) b = (size_t)e->max_blocks;
+    return (int)b;
+}
+
+static int ensure_partials(rsd_engine *e, size_t rows, int R) {
+    size_t need = rows * 4 * (size_t)R;
+    if (need > e->partial_cap) {
+        dfree(e->partials);
+        e->partials = NULL;
+        HIPCHK(e, hipMalloc((void **)&e->partials, need * sizeof(double)));
+        e->partial_cap = need;
+    }
+    return RSD_OK;
+}
+
+static int ensure_gates(rsd_engine *e, float start, float delta, int R) {
+    if (e->gate_r && e->gate_R == R && e->gate_start == start && e->gate_delta == delta) return RSD_OK;
+    float *h = (float *)malloc((size_t)R * sizeof(float));
+    if (!h) return fail(e, RSD_ERR_ALLOC, "gate table");
+    float r = start;
+    for (int k = 0; k < R; k++) {  // rs.cl:1189 / 1214: repeated binary32 additions
+        h[k] = r;
+        r += delta;
+    }
+    dfree(e->gate_r);
+    e->gate_r = NULL;
+    hipError_t st = hipMalloc((void **)&e->gate_r, (size_t)R * sizeof(float));
+    if (st == hipSuccess) st = hipMemcpy(e->gate_r, h, (size_t)R * sizeof(float), hipMemcpyHostToDevice);
+    free(h);
+    if (st != hipSuccess) return fail(e, RSD_ERR_HIP, "gate table: %s", hipGetErrorString(st));
+    e->gate_R = R;
+    e->gate_start = start;
+    e->gate_delta = delta;
+    return RSD_OK;
+}
+
+// Support of the range weight in metres: outside [lo, hi] every gate weight is 0.
+static void range_support(const rsd_engine *e, float delta, float *lo, float *hi, int *dense) {
+    const int n = e->rangew_n;
+    const float *W = e->rangew_h;
+    const float xs = e->rangew_desc.s[0], x0 = e->rangew_desc.s[1];
+    *dense = 0;
+    if (n < 1 || W[0] != 0.0f || W[n - 1] != 0.0f || !(xs > 0.0f)) {
+        *dense = 1;
+        *lo = -INFINITY;
+        *hi = INFINITY;
+        return;
+    }
+    int f = -1, l = -1;
+    for (int i = 0; i < n; i++) {
+        if (W[i] != 0.0f) {
+            if (f < 0) f = i;
+            l = i;
+        }
+    }
+    if (f < 0) {  // all-zero table: no gate is ever weighted
+        *lo = 1.0f;
+        *hi = -1.0f;
+        return;
+    }
+    float pad = 2.0f * fabsf(delta) + 1.0f;
+    *lo = ((float)(f - 1) - x0) / xs - pad;
+    *hi = ((float)(l + 1) - x0) / xs + pad;
+}
+
+static int fill_pulse(rsd_engine *e, const rsd_pulse *p, PulseK &k) {
+    if (!e->angw || !e->rangew) return fail(e, RSD_ERR_STATE, "weight tables not set");
+    int st = ensure_gates(e, p->range_start, p->range_delta, (int)p->range_count);
+    if (st) return st;
+    memcpy(k.sd.s, p->sim_desc, sizeof(k.sd.s));
+    k.two_over_lambda = p->two_over_lambda;
+    k.angw = e->angw;
+    k.angw_desc = e->angw_desc;
+    k.rangew = e->rangew;
+    k.rangew_desc = e->rangew_desc;
+    k.gate_r = e->gate_r;
+    k.R = (int)p->range_count;
+    k.gate_start = p->range_start;
+    k.gate_inv_delta = p->range_delta != 0.0f ? 1.0f / p->range_delta : 0.0f;
+    range_support(e, p->range_delta, &k.sup_lo, &k.sup_hi, &k.dense);
+    if (p->range_delta == 0.0f) k.dense = 1;
+    k.phase_mode = p->phase_mode;
+    k.partials = e->partials;
+    k.row0 = 0;
+    return RSD_OK;
+}
+
+static int fill_rain(rsd_engine *e, const rsd_step *s, RainK &a) {
+    int slot = s ? s->wind_slot : 0;
+    if (s && (!e->wind[slot])) return fail(e, RSD_ERR_STATE, "wind slot %d not set", slot);
+    if (!e->drop) return fail(e, RSD_ERR_STATE, "drop RCS table not set");
+    a.pos = e->pos; a.vel = e->vel; a.rcs = e->rcs; a.rnd = e->rnd;
+    a.n = e->n_rain;
+    a.wind.d = e->wind[slot]; a.wind.nx = e->wnx[slot]; a.wind.ny = e->wny[slot]; a.wind.nz = e->wnz[slot];
+    a.les = e->les[slot];
+    if (s) memcpy(a.sd.s, s->sim_desc, sizeof(a.sd.s));
+    a.drop = e->drop;
+    a.drop_desc = e->drop_desc;
+    a.dragged = s ? (s->background_kind == RSD_ADVANCE_DRAGGED) : 1u;
+    return RSD_OK;
+}
+
+static int fill_debris(rsd_engine *e, const rsd_step *s, const rsd_pulse *p, int type, DebrisK &a) {
+    a.pos = e->pos; a.ori = e->ori; a.vel = e->vel; a.tum = e->tum; a.rcs = e->rcs; a.rnd = e->rnd;
+    a.begin = e->origins[type];
+    a.end = e->origins[type] + e->counts[type];
+    if (s) {
+        int slot = s->wind_slot, ai = s->adm_index[type], ri = s->rcs_index[type];
+        if (!e->wind[slot]) return fail(e, RSD_ERR_STATE, "wind slot %d not set", slot);
+        if (ai < 0 || ai >= RSD_MAX_TABLES || !e->adm_cd[ai]) return fail(e, RSD_ERR_STATE, "ADM table %d not set", ai);
+        if (ri < 0 || ri >= RSD_MAX_TABLES || !e->rcs_re[ri]) return fail(e, RSD_ERR_STATE, "RCS table %d not set", ri);
+        a.wind.d = e->wind[slot]; a.wind.nx = e->wnx[slot]; a.wind.ny = e->wny[slot]; a.wind.nz = e->wnz[slot];
+        a.les = e->les[slot];
+        memcpy(a.sd.s, s->sim_desc, sizeof(a.sd.s));
+        a.adm_cd.d = e->adm_cd[ai]; a.adm_cm.d = e->adm_cm[ai];
+        a.adm_cd.nx = a.adm_cm.nx = e->adm_nx[ai]; a.adm_cd.ny = a.adm_cm.ny = e->adm_ny[ai];
+        a.adm_desc = e->adm_desc[ai];
+        a.rcs_re.d = e->rcs_re[ri]; a.rcs_im.d = e->rcs_im[ri];
+        a.rcs_re.nx = a.rcs_im.nx = e->rcs_nx[ri]; a.rcs_re.ny = a.rcs_im.ny = e->rcs_ny[ri];
+        a.rcs_desc = e->rcs_desc[ri];
+    }
+    if (p) {
+        int ri = p->rcs_index[type];
+        if (ri < 0 || ri >= RSD_MAX_TABLES || !e->rcs_re[ri]) return fail(e, RSD_ERR_STATE, "RCS table %d not set", ri);
+        a.prcs_re.d = e->rcs_re[ri]; a.prcs_im.d = e->rcs_im[ri];
+        a.prcs_re.nx = a.prcs_im.nx = e->rcs_nx[ri]; a.prcs_re.ny = a.prcs_im.ny = e->rcs_ny[ri];
+        a.prcs_desc = e->rcs_desc[ri];
+        a.refresh = p->debris_rcs_refresh;
+        uint32_t pc;
+        memcpy(&pc, &p->sim_desc[5], sizeof(pc));
+        a.pulse_concept = pc;
+        if (s) {
+            uint32_t sc;
+            memcpy(&sc, &s->sim_desc[5], sizeof(sc));
+            a.reuse_rcs = (s->rcs_index[type] == ri && sc == pc) ? 1u : 0u;
+        }
+    }
+    return RSD_OK;
+}
+
+// Launch the kernels of one step: advance (s != NULL) and/or pulse (p != NULL).
+static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p) {
+    HIPCHK(e, hipSetDevice(e->device));
+    const int R = p ? (int)p->range_count : 0;
+    if (p && (R < 1 || R > RSD_MAX_GATES)) return fail(e, RSD_ERR_ARG, "range_count %d out of range", R);
+    // Rows of the partial-sum buffer: one per block of every launch.
+    int rain_blocks = e->n_rain ? blocks_for(e, e->n_rain) : 0;
+    int deb_blocks[RSD_MAX_TYPES] = {0};
+    int rows = rain_blocks;
+    for (int k = 1; k < RSD_MAX_TYPES; k++) {
+        if (e->counts[k]) {
+            deb_blocks[k] = blocks_for(e, e->counts[k]);
+            rows += deb_blocks[k];
+        }
+    }
+    PulseK pk;
+    memset(&pk, 0, sizeof(pk));
+    if (p) {
+        int st = ensure_partials(e, (size_t)rows, R);
+        if (st) return st;
+        st = fill_pulse(e, p, pk);
+        if (st) return st;
+    }
+    const size_t lds = p ? (size_t)4 * R * sizeof(double) : 0;
+    int row = 0;
+    if (e->n_rain) {
+        RainK a;
+        memset(&a, 0, sizeof(a));
+        int st = fill_rain(e, s, a);
+        if (st) return st;
+        a.pk = pk;
+        a.pk.row0 = row;
+        const bool timed = e->prof_on && e->prof_n < RSD_PROF_CAP;
+        if (timed) HIPCHK(e, hipEventRecord(e->prof_ev[2 * e->prof_n], e->stream));
+        if (s && p) hipLaunchKernelGGL((k_rain<true, true>), dim3(rain_blocks), dim3(e->block), lds, e->stream, a);
+        else if (s) hipLaunchKernelGGL((k_rain<true, false>), dim3(rain_blocks), dim3(e->block), 0, e->stream, a);
+        else hipLaunchKernelGGL((k_rain<false, true>), dim3(rain_blocks), dim3(e->block), lds, e->stream, a);
+        HIPCHK(e, hipGetLastError());
+        if (timed) {
+            HIPCHK(e, hipEventRecord(e->prof_ev[2 * e->prof_n + 1], e->stream));
+            e->prof_kind[e->prof_n] = (s ? 1 : 0) | (p ? 2 : 0);
+            e->prof_n++;
+        }
+        row += rain_blocks;
+    }
+    for (int k = 1; k < RSD_MAX_TYPES; k++) {
+        if (!e->counts[k]) continue;
+        DebrisK a;
+        memset(&a, 0, sizeof(a));
+        int st = fill_debris(e, s, p, k, a);
+        if (st) return st;
+        a.pk = pk;
+        a.pk.row0 = row;
+        if (s && p) hipLaunchKernelGGL((k_debris<true, true>), dim3(deb_blocks[k]), dim3(e->block), lds, e->stream, a);
+        else if (s) hipLaunchKernelGGL((k_debris<true, false>), dim3(deb_blocks[k]), dim3(e->block), 0, e->stream, a);
+        else hipLaunchKernelGGL((k_debris<false, true>), dim3(deb_blocks[k]), dim3(e->block), lds, e->stream, a);
+        HIPCHK(e, hipGetLastError());
+        row += deb_blocks[k];
+    }
+    if (p) {
+        hipLaunchKernelGGL(k_finalize, dim3(4 * R), dim3(256), 0, e->stream, (const double *)e->partials, rows, 4 * R, e->pulse_d);
+        HIPCHK(e, hipGetLastError());
+        e->last_pulse = *p;
+        e->have_pulse = 1;
+    }
+    return RSD_OK;
+}
+
+static int flush_pending(rsd_engine *e) {
+    if (!e->have_pending) return RSD_OK;
+    e->have_pending = 0;
+    return launch_step(e, &e->pending, NULL);
+}
+
+extern "C" int rsd_upload_state(rsd_engine *e, const float *pos, const float *vel, const float *ori,
+                                const float *tum, const float *aux, const float *rcs, const uint32_t *rnd) {
+    if (!e || !e->pos) return fail(e, RSD_ERR_STATE, "state not allocated");
+    int st = flush_pending(e);
+    if (st) return st;
+    const size_t b = e->n * sizeof(float4);
+    if (pos) HIPCHK(e, hipMemcpyAsync(e->pos, pos, b, hipMemcpyHostToDevice, e->stream));
+    if (vel) HIPCHK(e, hipMemcpyAsync(e->vel, vel, b, hipMemcpyHostToDevice, e->stream));
+    if (ori) HIPCHK(e, hipMemcpyAsync(e->ori, ori, b, hipMemcpyHostToDevice, e->stream));
+    if (tum) HIPCHK(e, hipMemcpyAsync(e->tum, tum, b, hipMemcpyHostToDevice, e->stream));
+    if (aux) HIPCHK(e, hipMemcpyAsync(e->aux, aux, b, hipMemcpyHostToDevice, e->stream));
+    if (rcs) HIPCHK(e, hipMemcpyAsync(e->rcs, rcs, b, hipMemcpyHostToDevice, e->stream));
+    if (rnd) HIPCHK(e, hipMemcpyAsync(e->rnd, rnd, e->n * sizeof(uint4), hipMemcpyHostToDevice, e->stream));
+    if (pos && e->n_rain) {
+        unsigned nb = (unsigned)((e->n_rain + 255) / 256);
+        hipLaunchKernelGGL(k_flag_rain, dim3(nb), dim3(256), 0, e->stream, e->pos, (unsigned long long)e->n_rain);
+        HIPCHK(e, hipGetLastError());
+    }
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return RSD_OK;
+}
+
+extern "C" int rsd_download_state(rsd_engine *e, float *pos, float *vel, float *ori, float *tum,
+                                  float *aux, float *rcs, float *sig, uint32_t *rnd) {
+    if (!e || !e->pos) return fail(e, RSD_ERR_STATE, "state not allocated");
+    int st = flush_pending(e);
+    if (st) return st;
+    const size_t b = e->n * sizeof(float4);
+    if (pos || rcs || sig || aux) {
+        if (!e->view) HIPCHK(e, hipMalloc((void **)&e->view, 4 * b));
+        ViewK v;
+        memset(&v, 0, sizeof(v));
+        v.pos = e->pos; v.rcs = e->rcs; v.aux_in = e->aux;
+        v.pos_out = e->view; v.rcs_out = e->view + e->n; v.sig_out = e->view + 2 * e->n; v.aux_out = e->view + 3 * e->n;
+        v.n = e->n;
+        v.n_rain = e->n_rain;
+        v.drop = e->drop;
+        v.drop_desc = e->drop_desc;
+        v.have_pulse = 0;
+        if (e->have_pulse) {
+            st = fill_pulse(e, &e->last_pulse, v.pk);
+            if (st) return st;
+            v.have_pulse = 1;
+        }
+        if (e->n_rain && !e->drop) return fail(e, RSD_ERR_STATE, "drop RCS table not set");
+        unsigned nb = (unsigned)((e->n + 255) / 256);
+        hipLaunchKernelGGL(k_view, dim3(nb), dim3(256), 0, e->stream, v);
+        HIPCHK(e, hipGetLastError());
+        if (pos) HIPCHK(e, hipMemcpyAsync(pos, v.pos_out, b, hipMemcpyDeviceToHost, e->stream));
+        if (rcs) HIPCHK(e, hipMemcpyAsync(rcs, v.rcs_out, b, hipMemcpyDeviceToHost, e->stream));
+        if (sig) HIPCHK(e, hipMemcpyAsync(sig, v.sig_out, b, hipMemcpyDeviceToHost, e->stream));
+        if (aux) HIPCHK(e, hipMemcpyAsync(aux, v.aux_out, b, hipMemcpyDeviceToHost, e->stream));
+    }
+    if (vel) HIPCHK(e, hipMemcpyAsync(vel, e->vel, b, hipMemcpyDeviceToHost, e->stream));
+    if (ori) HIPCHK(e, hipMemcpyAsync(ori, e->ori, b, hipMemcpyDeviceToHost, e->stream));
+    if (tum) HIPCHK(e, hipMemcpyAsync(tum, e->tum, b, hipMemcpyDeviceToHost, e->stream));
+    if (rnd) HIPCHK(e, hipMemcpyAsync(rnd, e->rnd, e->n * sizeof(uint4), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return RSD_OK;
+}
+
+static int upload_tex(rsd_engine *e, float4 **dst, const float *src, size_t texels) {
+    dfree(*dst);
+    *dst = NULL;
+    HIPCHK(e, hipMalloc((void **)dst, texels * sizeof(float4)));
+    HIPCHK(e, hipMemcpyAsync(*dst, src, texels * sizeof(float4), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return RSD_OK;
+}
+
+extern "C" int rsd_set_wind(rsd_engine *e, int slot, const float *uvwt, int nx, int ny, int nz, const float *desc) {
+    if (!e || slot < 0 || slot > 1 || !uvwt || !desc || nx < 1 || ny < 1 || nz < 1) return fail(e, RSD_ERR_ARG, "bad wind table");
+    HIPCHK(e, hipSetDevice(e->device));
+    int st = flush_pending(e);
+    if (st) return st;
+    const size_t texels = (size_t)nx * ny * nz;
+    if (e->wind[slot] && e->wnx[slot] == nx && e->wny[slot] == ny && e->wnz[slot] == nz) {
+        HIPCHK(e, hipMemcpyAsync(e->wind[slot], uvwt, texels * sizeof(float4), hipMemcpyHostToDevice, e->stream));
+        HIPCHK(e, hipStreamSynchronize(e->stream));
+    } else {
+        st = upload_tex(e, &e->wind[slot], uvwt, texels);
+        if (st) return st;
+    }
+    e->wnx[slot] = nx; e->wny[slot] = ny; e->wnz[slot] = nz;
+    memcpy(e->les[slot].s, desc, sizeof(e->les[slot].s));
+    return RSD_OK;
+}
+
+extern "C" int rsd_set_adm(rsd_engine *e, int t, const float *cd, const float *cm, int nx, int ny, const float *desc) {
+    if (!e || t < 0 || t >= RSD_MAX_TABLES || !cd || !cm || !desc || nx < 1 || ny < 1) return fail(e, RSD_ERR_ARG, "bad ADM table");
+    HIPCHK(e, hipSetDevice(e->device));
+    int st = flush_pending(e);
+    if (st) return st;
+    st = upload_tex(e, &e->adm_cd[t], cd, (size_t)nx * ny);
+    if (st) return st;
+    st = upload_tex(e, &e->adm_cm[t], cm, (size_t)nx * ny);
+    if (st) return st;
+    e->adm_nx[t] = nx; e->adm_ny[t] = ny;
+    memcpy(e->adm_desc[t].s, desc, sizeof(e->adm_desc[t].s));
+    return RSD_OK;
+}
+
+extern "C" int rsd_set_rcs(rsd_engine *e, int t, const float *re, const float *im, int nx, int ny, const float *desc) {
+    if (!e || t < 0 || t >= RSD_MAX_TABLES || !re || !im || !desc || nx < 1 || ny < 1) return fail(e, RSD_ERR_ARG, "bad RCS table");
+    HIPCHK(e, hipSetDevice(e->device));
+    int st = flush_pending(e);
+    if (st) return st;
+    st = upload_tex(e, &e->rcs_re[t], re, (size_t)nx * ny);
+    if (st) return st;
+    st = upload_tex(e, &e->rcs_im[t], im, (size_t)nx * ny);
+    if (st) return st;
+    e->rcs_nx[t] = nx; e->rcs_ny[t] = ny;
+    memcpy(e->rcs_desc[t].s, desc, sizeof(e->rcs_desc[t].s));
+    return RSD_OK;
+}
+
+extern "C" int rsd_set_drop_rcs(rsd_engine *e, const float *table, int n, const float *desc4) {
+    if (!e || !table || !desc4 || n < 1) return fail(e, RSD_ERR_ARG, "bad drop RCS table");
+    if (!(desc4[2] <= (float)(n - 1))) return fail(e, RSD_ERR_ARG, "drop RCS max index %g beyond %d entries", desc4[2], n);
+    HIPCHK(e, hipSetDevice(e->device));
+    int st = flush_pending(e);
+    if (st) return st;
+    st = upload_tex(e, &e->drop, table, (size_t)n);
+    if (st) return st;
+    memcpy(e->drop_desc.s, desc4, sizeof(e->drop_desc.s));
+    return RSD_OK;
+}
+
+static int upload_floats(rsd_engine *e, float **dst, const float *src, int n) {
+    dfree(*dst);
+    *dst = NULL;
+    HIPCHK(e, hipMalloc((void **)dst, (size_t)n * sizeof(float)));
+    HIPCHK(e, hipMemcpyAsync(*dst, src, (size_t)n * sizeof(float), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return RSD_OK;
+}
+
+extern "C" int rsd_set_range_weight(rsd_engine *e, const float *w, int n, const float *desc4) {
+    if (!e || !w || !desc4 || n < 1 || n > 1024) return fail(e, RSD_ERR_ARG, "bad range weight table");
+    if (!(desc4[2] <= (float)(n - 1))) return fail(e, RSD_ERR_ARG, "range weight max index beyond table");
+    HIPCHK(e, hipSetDevice(e->device));
+    int st = flush_pending(e);
+    if (st) return st;
+    st = upload_floats(e, &e->rangew, w, n);
+    if (st) return st;
+    memcpy(e->rangew_h, w, (size_t)n * sizeof(float));
+    e->rangew_n = n;
+    memcpy(e->rangew_desc.s, desc4, sizeof(e->rangew_desc.s));
+    return RSD_OK;
+}
+
+extern "C" int rsd_set_angular_weight(rsd_engine *e, const float *w, int n, const float *desc4) {
+    if (!e || !w || !desc4 || n < 1) return fail(e, RSD_ERR_ARG, "bad angular weight table");
+    if (!(desc4[2] <= (float)(n - 1))) return fail(e, RSD_ERR_ARG, "angular weight max index beyond table");
+    HIPCHK(e, hipSetDevice(e->device));
+    int st = flush_pending(e);
+    if (st) return st;
+    st = upload_floats(e, &e->angw, w, n);
+    if (st) return st;
+    e->angw_n = n;
+    memcpy(e->angw_desc.s, desc4, sizeof(e->angw_desc.s));
+    return RSD_OK;
+}
+
+extern "C" int rsd_advance(rsd_engine *e, const rsd_step *step) {
+    if (!e || !step) return fail(e, RSD_ERR_ARG, "null step");
+    if (!e->pos) return fail(e, RSD_ERR_STATE, "state not allocated");
+    int st = flush_pending(e);
+    if (st) return st;
+    e->pending = *step;
+    e->have_pending = 1;
+    return RSD_OK;
+}
+
+extern "C" int rsd_make_pulse(rsd_engine *e, const rsd_pulse *p) {
+    if (!e || !p) return fail(e, RSD_ERR_ARG, "null pulse");
+    if (!e->pos) return fail(e, RSD_ERR_STATE, "state not allocated");
+    if (e->have_pending) {
+        e->have_pending = 0;
+        return launch_step(e, &e->pending, p);
+    }
+    return launch_step(e, NULL, p);
+}
+
+extern "C" int rsd_read_pulse(rsd_engine *e, double *out, uint32_t R) {
+    if (!e || !out) return fail(e, RSD_ERR_ARG, "null output");
+    if (!e->have_pulse) return fail(e, RSD_ERR_STATE, "no pulse made yet");
+    if (R != e->last_pulse.range_count) return fail(e, RSD_ERR_ARG, "range_count %u != %u of the last pulse", R, e->last_pulse.range_count);
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipMemcpyAsync(e->pulse_h, e->pulse_d, (size_t)4 * R * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    memcpy(out, e->pulse_h, (size_t)4 * R * sizeof(double));
+    return RSD_OK;
+}
+
+extern "C" int rsd_sync(rsd_engine *e) {
+    if (!e) return RSD_ERR_ARG;
+    HIPCHK(e, hipSetDevice(e->device));
+    int st = flush_pending(e);
+    if (st) return st;
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return RSD_OK;
+}
+
+extern "C" int rsd_math_probe(int device, int fn, const float *a, const float *b, float *out, size_t n) {
+    if (!a || !b || !out || n == 0) return RSD_ERR_ARG;
+    if (device < 0 || device >= rsd_device_count()) return RSD_ERR_NO_DEVICE;
+    if (hipSetDevice(device) != hipSuccess) return RSD_ERR_HIP;
+    float *d = NULL;
+    if (hipMalloc((void **)&d, 3 * n * sizeof(float)) != hipSuccess) return RSD_ERR_ALLOC;
+    int rc = RSD_OK;
+    if (hipMemcpy(d, a, n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d + n, b, n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) rc = RSD_ERR_HIP;
+    if (rc == RSD_OK) {
+        hipLaunchKernelGGL(k_math_probe, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, fn, (const float *)d,
+                           (const float *)(d + n), d + 2 * n, (unsigned long long)n);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = RSD_ERR_HIP;
+    }
+    if (rc == RSD_OK && hipMemcpy(out, d + 2 * n, n * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) rc = RSD_ERR_HIP;
+    (void)hipFree(d);
+    return rc;
+}
+
+extern "C" int rsd_profile(rsd_engine *e, int enable) {
+    if (!e) return RSD_ERR_ARG;
+    HIPCHK(e, hipSetDevice(e->device));
+    if (enable && !e->prof_ev) {
+        e->prof_ev = (hipEvent_t *)calloc(2 * RSD_PROF_CAP, sizeof(hipEvent_t));
+        e->prof_kind = (int *)calloc(RSD_PROF_CAP, sizeof(int));
+        if (!e->prof_ev || !e->prof_kind) return fail(e, RSD_ERR_ALLOC, "profile events");
+        for (int i = 0; i < 2 * RSD_PROF_CAP; i++) HIPCHK(e, hipEventCreate(&e->prof_ev[i]));
+    }
+    e->prof_on = enable ? 1 : 0;
+    e->prof_n = 0;
+    return RSD_OK;
+}
+
+extern "C" int rsd_profile_read(rsd_engine *e, int kind, double *total_ms, int *count) {
+    if (!e || !total_ms || !count) return RSD_ERR_ARG;
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    double t = 0.0;
+    int c = 0;
+    for (int i = 0; i < e->prof_n; i++) {
+        if (kind >= 0 && e->prof_kind[i] != kind) continue;
+        float ms = 0.0f;
+        HIPCHK(e, hipEventElapsedTime(&ms, e->prof_ev[2 * i], e->prof_ev[2 * i + 1]));
+        t += ms;
+        c++;
+    }
+    *total_ms = t;
+    *count = c;
+    return RSD_OK;
+}
+
+extern "C" int rsd_timer_start(rsd_engine *e) {
+    if (!e) return RSD_ERR_ARG;
+    HIPCHK(e, hipEventRecord(e->ev0, e->stream));
+    return RSD_OK;
+}
+
+extern "C" int rsd_timer_stop(rsd_engine *e, double *ms) {
+    if (!e || !ms) return RSD_ERR_ARG;
+    int st = flush_pending(e);
+    if (st) return st;
+    HIPCHK(e, hipEventRecord(e->ev1, e->stream));
+    HIPCHK(e, hipEventSynchronize(e->ev1));
+    float t = 0.0f;
+    HIPCHK(e, hipEventElapsedTime(&t, e->ev0, e->ev1));
+    *ms = (double)t;
+    return RSD_OK;
+}
