@@ -22,6 +22,18 @@ class Tex(ctypes.Structure):
     _fields_ = [("data", ctypes.c_void_p), ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int)]
 
 
+class F4(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_float), ("y", ctypes.c_float), ("z", ctypes.c_float), ("w", ctypes.c_float)]
+
+    def np(self):
+        return np.array([self.x, self.y, self.z, self.w], np.float32)
+
+
+def _f4(v):
+    v = np.asarray(v, np.float32).ravel()
+    return F4(*[float(x) for x in v[:4]])
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -32,7 +44,6 @@ def lib():
         T = ctypes.POINTER(Tex)
         sig = {
             "orc_rand4": (None, [P, P]),
-            "orc_sample3d": (None, []),
             "orc_el_atts": (None, [P, P, P, P, S, S, T, P, P, P, P]),
             "orc_bg_atts": (None, [P, P, P, P, S, S, T, P, P, P, P]),
             "orc_db_atts": (None, [P, P, P, P, P, P, S, S, T, P, T, T, P, T, T, P, P]),
@@ -50,6 +61,13 @@ def lib():
             "orc_el_atts_f64": (None, [P, P, P, S, T, P, P, I]),
             "orc_les_desc": (None, [P, I, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, F, F, F, F, F, F, F]),
             "orc_math": (None, [I, P, P, P, S]),
+            "orc_wind_index": (F4, [F4, P, P]),
+            "orc_sample3d": (F4, [T, F, F, F]),
+            "orc_sample2d": (F4, [T, F, F]),
+            "orc_ellipsoid_rcs": (F4, [F4, P, P]),
+            "orc_debris_rcs": (F4, [F4, F4, T, T, P, P]),
+            "orc_qmul": (F4, [F4, F4]),
+            "orc_quat2abg": (None, [F4, P, P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -83,6 +101,37 @@ def math(fn, a, b=None):
     out = np.empty_like(a)
     lib().orc_math(MATH[fn], _p(a), _p(b), _p(out), a.size)
     return out
+
+
+def wind_index(pos, les_desc, sim_desc):
+    return lib().orc_wind_index(_f4(pos), _p(f32(les_desc)), _p(f32(sim_desc))).np()
+
+
+def sample3d(t, u, v, w):
+    return lib().orc_sample3d(ctypes.byref(t), u, v, w).np()
+
+
+def sample2d(t, u, v):
+    return lib().orc_sample2d(ctypes.byref(t), u, v).np()
+
+
+def ellipsoid_rcs(pos, table, desc):
+    return lib().orc_ellipsoid_rcs(_f4(pos), _p(f32(table)), _p(f32(desc))).np()
+
+
+def debris_rcs(pos, ori, re_tex, im_tex, rcs_desc, sim_desc):
+    return lib().orc_debris_rcs(_f4(pos), _f4(ori), ctypes.byref(re_tex), ctypes.byref(im_tex),
+                                _p(f32(rcs_desc)), _p(f32(sim_desc))).np()
+
+
+def qmul(a, b):
+    return lib().orc_qmul(_f4(a), _f4(b)).np()
+
+
+def quat2abg(q):
+    a, b, g = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+    lib().orc_quat2abg(_f4(q), ctypes.byref(a), ctypes.byref(b), ctypes.byref(g))
+    return a.value, b.value, g.value
 
 
 def rand4(seeds):

@@ -142,6 +142,23 @@ static inline of4 norm4(of4 q) {
     return o;
 }
 
+void orc_quat2abg(of4 q, float *alpha, float *beta, float *gamma) {
+    /* rs.cl:317-332 (matlab/quat2abg.m): 3-2-3 Euler angles, lumped into gamma
+     * within about 1 degree of the poles */
+    float beta_arg = ((q.w * q.w + q.z * q.z) - q.y * q.y) - q.x * q.x;
+    if (beta_arg > 0.999847f || beta_arg < -0.999847f) {
+        *alpha = 0.0f;
+        *beta = 0.0f;
+        *gamma = (of_signf(q.z) * of_acosf(q.w)) * 2.0f;
+    } else {
+        *alpha = of_atan2f(q.y * q.z - q.w * q.x, q.x * q.z + q.w * q.y);
+        *beta = of_acosf(((q.w * q.w + q.z * q.z) - q.y * q.y) - q.x * q.x);
+        *gamma = of_atan2f(q.y * q.z + q.w * q.x, q.w * q.y - q.x * q.z);
+    }
+}
+
+of4 orc_qmul(of4 l, of4 r) { return qmul(l, r); }
+
 of4 orc_debris_rcs(of4 pos, of4 ori, const orc_tex *rr, const orc_tex *ri, const float *d, const float *sd) {
     uint32_t concept = fbits(sd[5]);
     float el = of_atan2f(pos.z, of_len2(pos.x, pos.y));
@@ -157,16 +174,7 @@ of4 orc_debris_rcs(of4 pos, of4 ori, const orc_tex *rr, const orc_tex *ri, const
     of4 R = qmul(ori, oc);
     of4 q = {R.x, R.z, -R.y, R.w};
     float alpha, beta, gamma;
-    float beta_arg = ((q.w * q.w + q.z * q.z) - q.y * q.y) - q.x * q.x;
-    if (beta_arg > 0.999847f || beta_arg < -0.999847f) {
-        alpha = 0.0f;
-        beta = 0.0f;
-        gamma = (of_signf(q.z) * of_acosf(q.w)) * 2.0f;
-    } else {
-        alpha = of_atan2f(q.y * q.z - q.w * q.x, q.x * q.z + q.w * q.y);
-        beta = of_acosf(((q.w * q.w + q.z * q.z) - q.y * q.y) - q.x * q.x);
-        gamma = of_atan2f(q.y * q.z + q.w * q.x, q.w * q.y - q.x * q.z);
-    }
+    orc_quat2abg(q, &alpha, &beta, &gamma);
     float u = fmaf(alpha, d[0], d[4]);
     float v = fmaf(beta, d[1], d[5]);
     of4 re = orc_sample2d(rr, u, v);

@@ -41,6 +41,10 @@ of4 orc_sample2d(const orc_tex *t, float u, float v);
 
 /* rs.cl:268-280 */
 of4 orc_ellipsoid_rcs(of4 pos, const of4 *table, const float *table_desc4);
+/* rs.cl:317-332 -- 3-2-3 Euler angles of a relative quaternion. */
+void orc_quat2abg(of4 q, float *alpha, float *beta, float *gamma);
+/* rs.cl:54-60 */
+of4 orc_qmul(of4 l, of4 r);
 /* rs.cl:282-359 */
 of4 orc_debris_rcs(of4 pos, of4 ori, const orc_tex *rcs_real, const orc_tex *rcs_imag,
                    const float *rcs_desc, const float *sim_desc);

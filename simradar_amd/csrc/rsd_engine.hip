@@ -556,9 +556,11 @@ __global__ __launch_bounds__(256) void k_debris(DebrisK a) {
 }
 
 // Sum the per-block partial rows of one pulse: one block per output value.
+// Rows hold [component][gate]; the pulse buffer is [gate][component].
 __global__ __launch_bounds__(256) void k_finalize(const double *partials, int rows, int n4r, double *out) {
     __shared__ double red[256];
     const int j = blockIdx.x;
+    const int R = n4r / 4;
     double s = 0.0;
     for (int r = threadIdx.x; r < rows; r += blockDim.x) s += partials[(size_t)r * n4r + j];
     red[threadIdx.x] = s;
@@ -567,7 +569,7 @@ __global__ __launch_bounds__(256) void k_finalize(const double *partials, int ro
         if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[j] = red[0];
+    if (threadIdx.x == 0) out[4 * (j % R) + j / R] = red[0];
 }
 
 // Device evaluation of the binary32 functions the kernels use (self-test).
