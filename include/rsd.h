@@ -129,12 +129,17 @@ int rsd_timer_stop(rsd_engine *e, double *ms);
 int rsd_profile(rsd_engine *e, int enable);
 int rsd_profile_read(rsd_engine *e, int kind, double *total_ms, int *count);
 
+/* Bandwidth probe (RS_io_test, rs.c:4389-4427): reps copies of n float4; returns
+ * the achieved read + write rate in GB/s. */
+int rsd_io_test(rsd_engine *e, size_t n, int reps, double *gbps);
+
 /* Launch shape of the scatterer kernels (0 = built-in default). */
 int rsd_set_launch(rsd_engine *e, int block_size, int max_blocks);
 
 /* Self-test: evaluate one of the device's binary32 functions on n inputs
  * (fn: 0 log1p(a), 1 log(a), 2 sin(a), 3 cos(a), 4 atan2(a, b), 5 acos(a),
- * 6 sqrt(a), 7 a / b, 8 sin(2 pi a), 9 cos(2 pi a)).  No counterpart in the
+ * 6 sqrt(a), 7 a / b, 8 sin(2 pi a), 9 cos(2 pi a), 10 the pulse path's fast acos(a)).
+ * No counterpart in the
  * reference; it lets a caller check the device arithmetic the advection uses. */
 int rsd_math_probe(int device_index, int fn, const float *a, const float *b, float *out, size_t n);
 

@@ -241,8 +241,10 @@ void orc_el_atts(of4 *p, of4 *v, of4 *x, ou4 *y, size_t begin, size_t end,
                 vel.y = vel.y + du.y * dt;
                 vel.z = vel.z + du.z * dt;
                 vel.w = vel.w + du.w * 0.0f;
-                if ((concept & 8u) &&
-                    of_len4(vel.x, vel.y, vel.z, vel.w) > fmaxf(1.0f, 3.0f * of_len4(bg.x, bg.y, bg.z, bg.w))) {
+                /* |v| > max(1, 3 |v_bg|) (rs.cl:716), evaluated on squared norms */
+                float v2 = ((vel.x * vel.x + vel.y * vel.y) + vel.z * vel.z) + vel.w * vel.w;
+                float b2 = ((bg.x * bg.x + bg.y * bg.y) + bg.z * bg.z) + bg.w * bg.w;
+                if ((concept & 8u) && v2 > fmaxf(1.0f, 9.0f * b2)) {
                     vel.x = bg.x + 0.0f * dt;
                     vel.y = bg.y + 0.0f * dt;
                     vel.z = bg.z + (-9.8f) * dt;
@@ -375,8 +377,9 @@ void orc_db_atts(of4 *p, of4 *o, of4 *v, of4 *t, of4 *x, ou4 *y, size_t begin, s
         dw.y = of_radiansf((ta_urn * ad[13]) * cm.y);
         dw.z = of_radiansf((ta_urn * ad[14]) * cm.z);
         dw.w = of_radiansf((ta_urn * 0.0f) * cm.w);
-        if ((concept & 8u) &&
-            of_len2(vel.x + du.x * dt, vel.y + du.y * dt) > 3.0f * of_len2(bg.x, bg.y)) {
+        /* |v_xy + dv_xy| > 3 |v_bg,xy| (rs.cl:881), evaluated on squared norms */
+        float nx_ = vel.x + du.x * dt, ny_ = vel.y + du.y * dt;
+        if ((concept & 8u) && nx_ * nx_ + ny_ * ny_ > 9.0f * (bg.x * bg.x + bg.y * bg.y)) {
             vel.x = bg.x;
             vel.y = bg.y;
             vel.z = vel.z + du.z * dt;
@@ -411,11 +414,17 @@ void orc_db_rcs(const of4 *p, const of4 *o, of4 *x, size_t begin, size_t end,
     for (size_t i = begin; i < end; i++) x[i] = orc_debris_rcs(p[i], o[i], rr, ri, rd, sd);
 }
 
-/* Angular weight of a unit-normalised direction (rs.cl:1065-1093). */
+/* Angular weight (rs.cl:1065-1093).  The angle between the beam b and the
+ * position p, acos(b . p / |p|) in rs.cl:1066, is evaluated in the
+ * well-conditioned form atan2(|b x p|, b . p): near the beam axis the cosine form
+ * turns one ulp of the cosine into 3.5e-4 rad, a 1% change of the main-lobe
+ * weight. */
 static inline float ang_weight(of4 pos, const float *w, const float *d4, const float *sd, float *range) {
     float r = of_len3(pos.x, pos.y, pos.z);
-    float nx = pos.x / r, ny = pos.y / r, nz = pos.z / r;
-    float angle = of_acosf((sd[0] * nx + sd[1] * ny) + sd[2] * nz);
+    float cx = sd[1] * pos.z - sd[2] * pos.y;
+    float cy = sd[2] * pos.x - sd[0] * pos.z;
+    float cz = sd[0] * pos.y - sd[1] * pos.x;
+    float angle = of_atan2f(of_len3(cx, cy, cz), (sd[0] * pos.x + sd[1] * pos.y) + sd[2] * pos.z);
     float x0 = of_clampf(fmaf(angle, d4[0], d4[1]), 0.0f, d4[2]);
     float x1 = of_clampf(fmaf(angle, d4[0], d4[1] + 1.0f), 0.0f, d4[2]);
     float ip;
@@ -478,10 +487,9 @@ static inline double range_weight_d(double dr, const float *W, const float *d4) 
 static inline double ang_weight_d(double x, double y, double z, const float *w, const float *d4,
                                   const float *sd, double *range) {
     double r = sqrt(x * x + y * y + z * z);
-    double c = ((double)sd[0] * x + (double)sd[1] * y + (double)sd[2] * z) / r;
-    if (c > 1.0) c = 1.0;
-    if (c < -1.0) c = -1.0;
-    double angle = acos(c);
+    double bx = sd[0], by = sd[1], bz = sd[2];
+    double cx = by * z - bz * y, cy = bz * x - bx * z, cz = bx * y - by * x;
+    double angle = atan2(sqrt(cx * cx + cy * cy + cz * cz), bx * x + by * y + bz * z);
     *range = r;
     return range_weight_d(angle, w, d4);
 }

@@ -79,7 +79,7 @@ EXPORTED_RSD = [
     "rsd_upload_state", "rsd_download_state", "rsd_set_wind", "rsd_set_adm", "rsd_set_rcs", "rsd_set_drop_rcs",
     "rsd_set_range_weight", "rsd_set_angular_weight", "rsd_advance", "rsd_make_pulse", "rsd_read_pulse",
     "rsd_sync", "rsd_timer_start", "rsd_timer_stop", "rsd_set_launch", "rsd_math_probe", "rsd_profile",
-    "rsd_profile_read",
+    "rsd_profile_read", "rsd_io_test",
 ]
 
 
@@ -230,6 +230,7 @@ def _declare(lib):
         "rsd_set_launch": (I, [P, I, I]),
         "rsd_math_probe": (I, [I, I, P, P, P, S]),
         "rsd_profile": (I, [P, I]),
+        "rsd_io_test": (I, [P, S, I, ctypes.POINTER(ctypes.c_double)]),
         "rsd_profile_read": (I, [P, I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]),
     }
     for name, (res, args) in sig.items():

@@ -1024,8 +1024,15 @@ void RS_add_debris(RSHandle *H, OBJConfig type, const size_t count) {
 }
 
 void RS_io_test(RSHandle *H) {
-    (void)H;
-    rsprint("RS_io_test(): bandwidth probes are measured with rocprofv3 on the engine kernels.");
+    if (!has_engine(H)) {
+        rsprint("ERROR: RS_io_test() needs a GPU.");
+        return;
+    }
+    size_t n = H->num_scats ? H->num_scats : ((size_t)1 << 22);
+    double gbps = 0.0;
+    if (engine_ok(H, rsd_io_test(H->workers[0].engine, n, 20, &gbps), "io test")) {
+        rsprint("io test: %s float4 copied 20 times, %.1f GB/s (read + write)", commaint((long long)n), gbps);
+    }
 }
 
 /* ------------------------------------------------------------- drop RCS */

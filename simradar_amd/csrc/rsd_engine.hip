@@ -98,16 +98,16 @@ __device__ __forceinline__ float4 sample2d(const Tex2 &t, float u, float v) {
 }
 
 // rs.cl:123-176: LES table coordinates of a position.
-__device__ __forceinline__ float4 wind_index(float4 pos, const F16 &d, const F16 &sd) {
+__device__ __forceinline__ float4 wind_index(float4 pos, const F16 &d, const F16 &sd, const float *logtab) {
     float4 c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     uint32_t spacing = __float_as_uint(d.s[7]);
     if (spacing == 7u) {
         float cx = sd.s[8] + 0.5f * sd.s[12];
         float cy = sd.s[9] + 0.5f * sd.s[13];
         float rx = pos.x - cx, ry = pos.y - cy, rz = pos.z - 0.0f;
-        c.x = __builtin_copysignf(d.s[0], rx) * rf_log1pf(d.s[4] * __builtin_fabsf(rx)) + d.s[8];
-        c.y = __builtin_copysignf(d.s[1], ry) * rf_log1pf(d.s[5] * __builtin_fabsf(ry)) + d.s[9];
-        c.z = __builtin_copysignf(d.s[2], rz) * rf_log1pf(d.s[6] * __builtin_fabsf(rz)) + d.s[10];
+        c.x = __builtin_copysignf(d.s[0], rx) * rf_log1pf(d.s[4] * __builtin_fabsf(rx), logtab) + d.s[8];
+        c.y = __builtin_copysignf(d.s[1], ry) * rf_log1pf(d.s[5] * __builtin_fabsf(ry), logtab) + d.s[9];
+        c.z = __builtin_copysignf(d.s[2], rz) * rf_log1pf(d.s[6] * __builtin_fabsf(rz), logtab) + d.s[10];
     } else if (spacing == 0u) {
         c.x = rf_fma(pos.x, d.s[0], d.s[4]);
         c.y = rf_fma(pos.y, d.s[1], d.s[5]);
@@ -136,7 +136,7 @@ __device__ __forceinline__ float4 ellipsoid_rcs(float4 p, const float4 *table, c
     float fidx = rf_clamp(rf_fma(p.w, td.s[0], td.s[1]), 0.0f, td.s[2]);
     float4 xz = table[(uint32_t)fidx];
     float rho2 = p.x * p.x + p.y * p.y;
-    float cb2 = rf_div(rho2, rho2 + p.z * p.z);
+    float cb2 = rho2 * rf_rcp_fast(rho2 + p.z * p.z);
     return make_float4(xz.x, xz.y, xz.x + (xz.z - xz.x) * cb2, xz.y + (xz.w - xz.y) * cb2);
 }
 
@@ -227,14 +227,23 @@ __device__ __forceinline__ float table_lerp(float x, const float *W, const F4 &d
     return rf_mix(W[(uint32_t)ip], W[(uint32_t)__builtin_floorf(x1)], dec);
 }
 
+// Angle between the beam and a position: acos(b . p / |p|) of rs.cl:1066 in
+// the well-conditioned form atan2(|b x p|, b . p) (near the axis the cosine
+// form turns one ulp into 3.5e-4 rad).
+__device__ __forceinline__ float beam_angle(float4 p, const F16 &sd) {
+    float cx = sd.s[1] * p.z - sd.s[2] * p.y;
+    float cy = sd.s[2] * p.x - sd.s[0] * p.z;
+    float cz = sd.s[0] * p.y - sd.s[1] * p.x;
+    return rf_atan2_fast(rf_sqrt_fast((cx * cx + cy * cy) + cz * cz), (sd.s[0] * p.x + sd.s[1] * p.y) + sd.s[2] * p.z);
+}
+
 // Signal x angular weight of one scatterer (rs.cl:1050-1110, 1175-1180).
 __device__ __forceinline__ bool scat_signal(float4 p, float4 rcs, const PulseK &k, float4 &sa, float &r) {
     r = rf_len3(p.x, p.y, p.z);
-    float nx = rf_div(p.x, r), ny = rf_div(p.y, r), nz = rf_div(p.z, r);
-    float angle = rf_acosf((k.sd.s[0] * nx + k.sd.s[1] * ny) + k.sd.s[2] * nz);
-    float wa = table_lerp(angle, k.angw, k.angw_desc);
+    const float inv_r = rf_rcp_fast(r);
+    float wa = table_lerp(beam_angle(p, k.sd), k.angw, k.angw_desc);
     if (wa == 0.0f) return false;
-    float atten = 1.0f / (r * r);
+    float atten = inv_r * inv_r;
     float ss, cc;
     if (k.phase_mode == RSD_PHASE_F64) {
         double turns = (double)r * k.two_over_lambda;
@@ -299,11 +308,12 @@ struct RainK {
     const float4 *drop;
     F4 drop_desc;
     uint32_t dragged;
+    const float *logtab;    // global copy of the log1p table (3 x 128 floats)
     PulseK pk;
 };
 
 // Returns true when the scatterer respawned.
-__device__ __forceinline__ bool rain_advance(const RainK &a, size_t i, float4 &pos, float4 &vel) {
+__device__ __forceinline__ bool rain_advance(const RainK &a, size_t i, float4 &pos, float4 &vel, const float *logtab) {
     const F16 &sd = a.sd;
     const float dt = sd.s[11];
     if (a.dragged) {
@@ -323,7 +333,7 @@ __device__ __forceinline__ bool rain_advance(const RainK &a, size_t i, float4 &p
             vel = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             return true;
         }
-        float4 c = wind_index(pos, a.les, sd);
+        float4 c = wind_index(pos, a.les, sd, logtab);
         float4 bg = sample3d(a.wind, c.x, c.y, c.z);
         float4 dv = make_float4(bg.x - vel.x, bg.y - vel.y, bg.z - vel.z, bg.w - vel.w);
         float dva = rf_len3(dv.x, dv.y, dv.z);
@@ -336,7 +346,10 @@ __device__ __forceinline__ bool rain_advance(const RainK &a, size_t i, float4 &p
             vel.y = vel.y + du.y * dt;
             vel.z = vel.z + du.z * dt;
             vel.w = vel.w + du.w * 0.0f;
-            if ((concept & 8u) && rf_len4(vel.x, vel.y, vel.z, vel.w) > __builtin_fmaxf(1.0f, 3.0f * rf_len4(bg.x, bg.y, bg.z, bg.w))) {
+            // |v| > max(1, 3 |v_bg|) (rs.cl:716) on squared norms
+            float v2 = ((vel.x * vel.x + vel.y * vel.y) + vel.z * vel.z) + vel.w * vel.w;
+            float b2 = ((bg.x * bg.x + bg.y * bg.y) + bg.z * bg.z) + bg.w * bg.w;
+            if ((concept & 8u) && v2 > __builtin_fmaxf(1.0f, 9.0f * b2)) {
                 vel.x = bg.x + 0.0f * dt;
                 vel.y = bg.y + 0.0f * dt;
                 vel.z = bg.z + (-9.8f) * dt;
@@ -364,15 +377,22 @@ __device__ __forceinline__ bool rain_advance(const RainK &a, size_t i, float4 &p
         vel = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         return true;
     }
-    float4 c = wind_index(pos, a.les, sd);
+    float4 c = wind_index(pos, a.les, sd, logtab);
     vel = sample3d(a.wind, c.x, c.y, c.z);
     return false;
+}
+
+__device__ __forceinline__ void logtab_load(float *dst, const float *src) {
+    for (int j = threadIdx.x; j < 3 * RF_LOG_TABLE; j += blockDim.x) dst[j] = src[j];
 }
 
 template <bool ADV, bool PULSE>
 __global__ __launch_bounds__(256) void k_rain(RainK a) {
     extern __shared__ double acc[];
+    __shared__ float logtab[3 * RF_LOG_TABLE];
+    if (ADV) logtab_load(logtab, a.logtab);
     if (PULSE) acc_clear(acc, 4 * a.pk.R);
+    else __syncthreads();
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += stride) {
         float4 p = a.pos[i];
@@ -383,7 +403,7 @@ __global__ __launch_bounds__(256) void k_rain(RainK a) {
         if (ADV) {
             float4 v = a.vel[i];
             const float4 p_in = p;
-            bool respawned = rain_advance(a, i, p, v);
+            bool respawned = rain_advance(a, i, p, v, logtab);
             if (respawned) {
                 if (!stale) {
                     rcs = ellipsoid_rcs(p_in, a.drop, a.drop_desc);
@@ -427,10 +447,11 @@ struct DebrisK {
     uint32_t refresh;
     uint32_t pulse_concept;
     uint32_t reuse_rcs;     // advance and refresh use the same table and concept
+    const float *logtab;
     PulseK pk;
 };
 
-__device__ bool debris_advance(const DebrisK &a, size_t i, float4 &pos, float4 &ori, float4 &vel, float4 &tum, float4 &rcs) {
+__device__ bool debris_advance(const DebrisK &a, size_t i, float4 &pos, float4 &ori, float4 &vel, float4 &tum, float4 &rcs, const float *logtab) {
     const F16 &sd = a.sd;
     const float dt = sd.s[11];
     const uint32_t concept = __float_as_uint(sd.s[5]);
@@ -473,7 +494,7 @@ __device__ bool debris_advance(const DebrisK &a, size_t i, float4 &pos, float4 &
         a.rnd[i] = seed;
         return true;
     }
-    float4 c = wind_index(pos, a.les, sd);
+    float4 c = wind_index(pos, a.les, sd, logtab);
     float4 bg = sample3d(a.wind, c.x, c.y, c.z);
     float4 ur = make_float4(bg.x - vel.x, bg.y - vel.y, bg.z - vel.z, bg.w - vel.w);
     float4 uh = qrot(norm4(ur), qconj(ori));
@@ -496,7 +517,9 @@ __device__ bool debris_advance(const DebrisK &a, size_t i, float4 &pos, float4 &
     float dwx = rf_radians((ta_urn * ad.s[12]) * cm.x);
     float dwy = rf_radians((ta_urn * ad.s[13]) * cm.y);
     float dwz = rf_radians((ta_urn * ad.s[14]) * cm.z);
-    if ((concept & 8u) && rf_len2(vel.x + du.x * dt, vel.y + du.y * dt) > 3.0f * rf_len2(bg.x, bg.y)) {
+    // |v_xy + dv_xy| > 3 |v_bg,xy| (rs.cl:881) on squared norms
+    float nvx = vel.x + du.x * dt, nvy = vel.y + du.y * dt;
+    if ((concept & 8u) && nvx * nvx + nvy * nvy > 9.0f * (bg.x * bg.x + bg.y * bg.y)) {
         vel.x = bg.x;
         vel.y = bg.y;
         vel.z = vel.z + du.z * dt;
@@ -520,7 +543,10 @@ __device__ bool debris_advance(const DebrisK &a, size_t i, float4 &pos, float4 &
 template <bool ADV, bool PULSE>
 __global__ __launch_bounds__(256) void k_debris(DebrisK a) {
     extern __shared__ double acc[];
+    __shared__ float logtab[3 * RF_LOG_TABLE];
+    if (ADV) logtab_load(logtab, a.logtab);
     if (PULSE) acc_clear(acc, 4 * a.pk.R);
+    else __syncthreads();
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t i = a.begin + (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.end; i += stride) {
         float4 p = a.pos[i], o = a.ori[i];
@@ -528,7 +554,7 @@ __global__ __launch_bounds__(256) void k_debris(DebrisK a) {
         bool rcs_known = false, respawned = false;
         if (ADV) {
             float4 v = a.vel[i], t = a.tum[i];
-            respawned = debris_advance(a, i, p, o, v, t, rcs);
+            respawned = debris_advance(a, i, p, o, v, t, rcs, logtab);
             a.pos[i] = p;
             a.ori[i] = o;
             a.vel[i] = v;
@@ -572,14 +598,21 @@ __global__ __launch_bounds__(256) void k_finalize(const double *partials, int ro
     if (threadIdx.x == 0) out[4 * (j % R) + j / R] = red[0];
 }
 
+// Bandwidth probe: copy of n float4 (the reference's io kernel, rs.cl:374-379).
+__global__ __launch_bounds__(256) void k_io(const float4 *__restrict__ in, float4 *__restrict__ out, unsigned long long n) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) out[i] = in[i];
+}
+
 // Device evaluation of the binary32 functions the kernels use (self-test).
-__global__ void k_math_probe(int fn, const float *a, const float *b, float *out, unsigned long long n) {
+__global__ void k_math_probe(int fn, const float *a, const float *b, float *out, unsigned long long n, const float *logtab) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float x = a[i], y = b[i], s, c;
     float r = 0.0f;
     switch (fn) {
-        case 0: r = rf_log1pf(x); break;
+        case 0: r = rf_log1pf(x, logtab); break;
+        case 10: r = rf_acos_fast(x); break;
         case 1: r = rf_logf(x); break;
         case 2: rf_sincosf(x, &s, &c); r = s; break;
         case 3: rf_sincosf(x, &s, &c); r = c; break;
@@ -635,10 +668,9 @@ __global__ void k_view(ViewK a) {
     if (a.have_pulse) {
         const PulseK &k = a.pk;
         float r = rf_len3(p.x, p.y, p.z);
-        float nx = rf_div(p.x, r), ny = rf_div(p.y, r), nz = rf_div(p.z, r);
-        float angle = rf_acosf((k.sd.s[0] * nx + k.sd.s[1] * ny) + k.sd.s[2] * nz);
-        float wa = table_lerp(angle, k.angw, k.angw_desc);
-        float atten = 1.0f / (r * r);
+        const float inv_r = rf_rcp_fast(r);
+        float wa = table_lerp(beam_angle(p, k.sd), k.angw, k.angw_desc);
+        float atten = inv_r * inv_r;
         float ss, cc;
         if (k.phase_mode == RSD_PHASE_F64) {
             double turns = (double)r * k.two_over_lambda;
@@ -699,6 +731,7 @@ struct rsd_engine {
     rsd_pulse last_pulse;
     int block, max_blocks;
     hipEvent_t ev0, ev1;
+    float *logtab;           // 3 x 128 floats of the wind-index log1p
     int prof_on, prof_n;
     hipEvent_t *prof_ev;     // start/stop pairs around the rain kernel
     int *prof_kind;          // bit 0 advance, bit 1 pulse
@@ -724,6 +757,28 @@ static int fail(rsd_engine *e, int code, const char *fmt, ...) {
 
 static void dfree(void *p) {
     if (p) (void)hipFree(p);
+}
+
+// Table of the division-free log1p (rsd_fmath.h): g_j = fl(1 / c_j),
+// c_j = 1 + (j + 1/2) / 128, and -log(g_j) split into binary32 head and tail.
+static void make_logtab(float *t) {
+    for (int j = 0; j < RF_LOG_TABLE; j++) {
+        double c = 1.0 + ((double)j + 0.5) / (double)RF_LOG_TABLE;
+        float g = (float)(1.0 / c);
+        double L = -log((double)g);
+        float hi = (float)L;
+        t[j] = g;
+        t[RF_LOG_TABLE + j] = hi;
+        t[2 * RF_LOG_TABLE + j] = (float)(L - (double)hi);
+    }
+}
+
+static hipError_t upload_logtab(float **dst) {
+    float h[3 * RF_LOG_TABLE];
+    make_logtab(h);
+    hipError_t r = hipMalloc((void **)dst, sizeof(h));
+    if (r == hipSuccess) r = hipMemcpy(*dst, h, sizeof(h), hipMemcpyHostToDevice);
+    return r;
 }
 
 extern "C" int rsd_device_count(void) {
@@ -755,7 +810,8 @@ extern "C" int rsd_open(int device_index, rsd_engine **out) {
     if (hipSetDevice(device_index) != hipSuccess || hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess ||
         hipHostMalloc((void **)&e->pulse_h, 4 * RSD_MAX_GATES * sizeof(double), hipHostMallocDefault) != hipSuccess ||
-        hipMalloc((void **)&e->pulse_d, 4 * RSD_MAX_GATES * sizeof(double)) != hipSuccess) {
+        hipMalloc((void **)&e->pulse_d, 4 * RSD_MAX_GATES * sizeof(double)) != hipSuccess ||
+        upload_logtab(&e->logtab) != hipSuccess) {
         free(e);
         return RSD_ERR_HIP;
     }
@@ -781,6 +837,7 @@ extern "C" void rsd_close(rsd_engine *e) {
     }
     dfree(e->drop); dfree(e->rangew); dfree(e->angw); dfree(e->gate_r); dfree(e->partials);
     dfree(e->pulse_d);
+    dfree(e->logtab);
     if (e->pulse_h) (void)hipHostFree(e->pulse_h);
     if (e->prof_ev) {
         for (int i = 0; i < 2 * RSD_PROF_CAP; i++) (void)hipEventDestroy(e->prof_ev[i]);
@@ -938,6 +995,7 @@ static int fill_rain(rsd_engine *e, const rsd_step *s, RainK &a) {
     a.drop = e->drop;
     a.drop_desc = e->drop_desc;
     a.dragged = s ? (s->background_kind == RSD_ADVANCE_DRAGGED) : 1u;
+    a.logtab = e->logtab;
     return RSD_OK;
 }
 
@@ -945,6 +1003,7 @@ static int fill_debris(rsd_engine *e, const rsd_step *s, const rsd_pulse *p, int
     a.pos = e->pos; a.ori = e->ori; a.vel = e->vel; a.tum = e->tum; a.rcs = e->rcs; a.rnd = e->rnd;
     a.begin = e->origins[type];
     a.end = e->origins[type] + e->counts[type];
+    a.logtab = e->logtab;
     if (s) {
         int slot = s->wind_slot, ai = s->adm_index[type], ri = s->rcs_index[type];
         if (!e->wind[slot]) return fail(e, RSD_ERR_STATE, "wind slot %d not set", slot);
@@ -1261,19 +1320,48 @@ extern "C" int rsd_math_probe(int device, int fn, const float *a, const float *b
     if (!a || !b || !out || n == 0) return RSD_ERR_ARG;
     if (device < 0 || device >= rsd_device_count()) return RSD_ERR_NO_DEVICE;
     if (hipSetDevice(device) != hipSuccess) return RSD_ERR_HIP;
-    float *d = NULL;
-    if (hipMalloc((void **)&d, 3 * n * sizeof(float)) != hipSuccess) return RSD_ERR_ALLOC;
+    float *d = NULL, *lt = NULL;
+    if (upload_logtab(&lt) != hipSuccess) return RSD_ERR_HIP;
+    if (hipMalloc((void **)&d, 3 * n * sizeof(float)) != hipSuccess) {
+        (void)hipFree(lt);
+        return RSD_ERR_ALLOC;
+    }
     int rc = RSD_OK;
     if (hipMemcpy(d, a, n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(d + n, b, n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) rc = RSD_ERR_HIP;
     if (rc == RSD_OK) {
         hipLaunchKernelGGL(k_math_probe, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, fn, (const float *)d,
-                           (const float *)(d + n), d + 2 * n, (unsigned long long)n);
+                           (const float *)(d + n), d + 2 * n, (unsigned long long)n, (const float *)lt);
         if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = RSD_ERR_HIP;
     }
     if (rc == RSD_OK && hipMemcpy(out, d + 2 * n, n * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) rc = RSD_ERR_HIP;
     (void)hipFree(d);
+    (void)hipFree(lt);
     return rc;
+}
+
+extern "C" int rsd_io_test(rsd_engine *e, size_t n, int reps, double *gbps) {
+    if (!e || n == 0 || reps < 1 || !gbps) return fail(e, RSD_ERR_ARG, "bad io test arguments");
+    HIPCHK(e, hipSetDevice(e->device));
+    int st = flush_pending(e);
+    if (st) return st;
+    float4 *buf = NULL;
+    HIPCHK(e, hipMalloc((void **)&buf, 2 * n * sizeof(float4)));
+    hipError_t r = hipMemsetAsync(buf, 0, n * sizeof(float4), e->stream);
+    const unsigned blocks = (unsigned)((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536);
+    if (r == hipSuccess) r = hipEventRecord(e->ev0, e->stream);
+    for (int k = 0; k < reps && r == hipSuccess; k++) {
+        hipLaunchKernelGGL(k_io, dim3(blocks), dim3(256), 0, e->stream, (const float4 *)buf, buf + n, (unsigned long long)n);
+        r = hipGetLastError();
+    }
+    if (r == hipSuccess) r = hipEventRecord(e->ev1, e->stream);
+    if (r == hipSuccess) r = hipEventSynchronize(e->ev1);
+    float ms = 0.0f;
+    if (r == hipSuccess) r = hipEventElapsedTime(&ms, e->ev0, e->ev1);
+    (void)hipFree(buf);
+    if (r != hipSuccess) return fail(e, RSD_ERR_HIP, "io test: %s", hipGetErrorString(r));
+    *gbps = 2.0 * (double)n * sizeof(float4) * reps / (ms * 1e-3) / 1e9;
+    return RSD_OK;
 }
 
 extern "C" int rsd_profile(rsd_engine *e, int enable) {

@@ -76,15 +76,81 @@ __device__ __forceinline__ float rf_logf(float x) {
     return (scale != 0.0f) ? r - scale * 0x1.62e430p-1f : r;
 }
 
-__device__ __forceinline__ float rf_log1pf(float x) {
+// log1p for the wind index, without division: u = 1 + x = 2^k m, m in [1, 2);
+// the top 7 mantissa bits select g_j = fl(1 / c_j), c_j = 1 + (j + 1/2) / 128,
+// and L_j = -log(g_j) as binary32 head + tail (built on the host with the C
+// library's binary64 log; passed in `tab` as g[128], head[128], tail[128]);
+// r = fma(m, g_j, -1), log(m) = L_j + r + r^2 (-1/2 + r (1/3 - r/4)); the
+// rounding of u adds (x - (u - 1)) g_j 2^-k.  Arguments below 2^-7 use the
+// series directly; negative ones (not on the wind-index path) the divided form.
+#define RF_LOG_TABLE 128
+__device__ __forceinline__ float rf_log1pf(float x, const float *tab) {
     if (x == 0.0f) return x;
     if (!(x > -1.0f)) return (x == -1.0f) ? -__builtin_inff() : __builtin_nanf("");
     if (__builtin_isinf(x)) return x;
+    if (x < 0.0f) {
+        float u = 1.0f + x;
+        int k;
+        float m = rf_split(u, &k);
+        return rf_log_kernel(m - 1.0f, k, rf_div(x - (u - 1.0f), u));
+    }
+    if (x < 0x1p-7f) return rf_fma(x * x, rf_fma(x, rf_fma(x, -0.25f, 0x1.555556p-2f), -0.5f), x);
     float u = 1.0f + x;
-    float tail = (u < 0x1p+24f) ? rf_div(x - (u - 1.0f), u) : 0.0f;
-    int k;
-    float m = rf_split(u, &k);
-    return rf_log_kernel(m - 1.0f, k, tail);
+    float tail = (u < 0x1p+24f) ? x - (u - 1.0f) : 0.0f;
+    uint32_t iu = __float_as_uint(u);
+    int k = (int)((iu >> 23) & 0xffu) - 127;
+    int j = (int)((iu >> 16) & 0x7fu);
+    float m = __uint_as_float((iu & 0x007fffffu) | 0x3f800000u);
+    float g = tab[j];
+    float r = rf_fma(m, g, -1.0f);
+    float p = rf_fma(r * r, rf_fma(r, rf_fma(r, -0.25f, 0x1.555556p-2f), -0.5f), r);
+    float kf = (float)k;
+    float scale = __uint_as_float((uint32_t)(127 - k) << 23);
+    float lo = rf_fma(kf, RF_LN2_LO, tab[2 * RF_LOG_TABLE + j]) + (tail * g) * scale;
+    float hi = rf_fma(kf, RF_LN2_HI, tab[RF_LOG_TABLE + j]);
+    return hi + (p + lo);
+}
+
+// Faster forms for quantities that only enter the pulse sums (checked against
+// the oracle within the IQ tolerance, not bit for bit): hardware reciprocal
+// (about 1 ulp) and acos from asin(x) = x + x^3 P(x^2) on |x| <= 1/2 (P a
+// degree-4 least-squares fit, relative error 6.4e-9) with the half-angle
+// reduction acos(c) = 2 asin(sqrt((1 - c) / 2)) for |c| > 1/2.
+__device__ __forceinline__ float rf_rcp_fast(float x) { return __builtin_amdgcn_rcpf(x); }
+
+__device__ __forceinline__ float rf_sqrt_fast(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// atan2 with the ratio taken through the hardware reciprocal.
+__device__ __forceinline__ float rf_atan2_fast(float y, float x) {
+    float ax = __builtin_fabsf(x), ay = __builtin_fabsf(y);
+    float mx = __builtin_fmaxf(ax, ay), mn = __builtin_fminf(ax, ay);
+    float a = mx > 0.0f ? mn * rf_rcp_fast(mx) : 0.0f;
+    float t = a, base_hi = 0.0f, base_lo = 0.0f;
+    if (a > RF_TANPI8) {
+        t = (a - 1.0f) * rf_rcp_fast(a + 1.0f);
+        base_hi = RF_PIO4_HI;
+        base_lo = RF_PIO4_LO;
+    }
+    float z = t * t;
+    float p = rf_fma(z, rf_fma(z, rf_fma(z, rf_fma(z, -0x1.f31592p-5f, 0x1.b22d32p-4f), -0x1.23ba22p-3f), 0x1.9991e6p-3f), -0x1.55554ap-2f);
+    float r = base_hi + (rf_fma(t * z, p, t) + base_lo);
+    if (ay > ax) r = RF_PIO2_HI - (r - RF_PIO2_LO);
+    if (__builtin_signbit(x)) r = RF_PI_HI - (r - RF_PI_LO);
+    return __builtin_copysignf(r, y);
+}
+
+__device__ __forceinline__ float rf_asin_half(float x) {
+    float z = x * x;
+    float p = rf_fma(z, rf_fma(z, rf_fma(z, rf_fma(z, 0x1.567920p-5f, 0x1.8fd26ep-6f), 0x1.741946p-5f), 0x1.33063ap-4f), 0x1.5555bap-3f);
+    return rf_fma(x * z, p, x);
+}
+
+__device__ __forceinline__ float rf_acos_fast(float c) {
+    c = __builtin_fminf(__builtin_fmaxf(c, -1.0f), 1.0f);
+    float a = __builtin_fabsf(c);
+    if (a <= 0.5f) return RF_PIO2_HI - (rf_asin_half(c) - RF_PIO2_LO);
+    float s = 2.0f * rf_asin_half(rf_sqrt(0.5f * (1.0f - a)));
+    return c > 0.0f ? s : RF_PI_HI - (s - RF_PI_LO);
 }
 
 __device__ __forceinline__ void rf_sincos_poly(float r, int n, float *s_out, float *c_out) {

@@ -115,3 +115,11 @@ def test_debris_step_parity(gpu_ready):
     errs = [h.rel_err(pg[k], po[k]) for k in range(len(pg))]
     assert max(errs) < IQ_TOL, errs
     r.free()
+
+
+def test_device_fast_acos_accuracy(gpu_ready):
+    """The pulse path's acos (no division) stays within 3e-7 rad of binary64."""
+    g = np.random.default_rng(21)
+    c = np.concatenate([g.uniform(-1, 1, 100000), 1 - np.exp(g.uniform(-20, -1, 50000))]).astype(np.float32)
+    err = np.abs(sa.math_probe(10, c).astype(np.float64) - np.arccos(c.astype(np.float64)))
+    assert err.max() < 3e-7, err.max()
