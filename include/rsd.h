@@ -136,6 +136,32 @@ int rsd_io_test(rsd_engine *e, size_t n, int reps, double *gbps);
 /* Launch shape of the scatterer kernels (0 = built-in default). */
 int rsd_set_launch(rsd_engine *e, int block_size, int max_blocks);
 
+/* Spatial order of the rain streams: every `interval` pulses (default 32,
+ * 0 = never) the engine reorders the stored rain scatterers by range shell
+ * (1/8 gate) and box cell.  Invisible to callers: downloads return the
+ * reference order and uploads restore it.  No counterpart in the reference
+ * (its streams stay in index order); it lets each lane sum its scatterers'
+ * gate terms in registers and keeps wind-table gathers local. */
+int rsd_set_sort(rsd_engine *e, int interval);
+
+/* Strategy of the per-gate sums inside a work-group (results agree to
+ * floating-point reassociation; all end in binary64 LDS and block partials):
+ *   RSD_FOLD_LANE    every lane adds its own terms to the LDS gate sums;
+ *   RSD_FOLD_WAVE64  where a wave's scatterers fall into at most 16 gates, the
+ *   RSD_FOLD_WAVE32  wave sums each gate's terms (binary64 / binary32 partials
+ *                    of 64 terms) and one lane adds them;
+ *   RSD_FOLD_WINDOW  (default) each lane sums its scatterers' terms into a
+ *                    register window of 8 consecutive gates (binary32) and the
+ *                    wave adds the windows once per launch; taken when the range
+ *                    weight spans under 6 gate spacings, else WAVE32.
+ * No counterpart in the reference (one work-item per scatterer, dense gate loop,
+ * rs.cl:1182-1217). */
+#define RSD_FOLD_LANE    0
+#define RSD_FOLD_WAVE64  1
+#define RSD_FOLD_WAVE32  2
+#define RSD_FOLD_WINDOW  3
+int rsd_set_fold(rsd_engine *e, int mode);
+
 /* Self-test: evaluate one of the device's binary32 functions on n inputs
  * (fn: 0 log1p(a), 1 log(a), 2 sin(a), 3 cos(a), 4 atan2(a, b), 5 acos(a),
  * 6 sqrt(a), 7 a / b, 8 sin(2 pi a), 9 cos(2 pi a), 10 the pulse path's fast acos(a)).

@@ -79,7 +79,7 @@ EXPORTED_RSD = [
     "rsd_upload_state", "rsd_download_state", "rsd_set_wind", "rsd_set_adm", "rsd_set_rcs", "rsd_set_drop_rcs",
     "rsd_set_range_weight", "rsd_set_angular_weight", "rsd_advance", "rsd_make_pulse", "rsd_read_pulse",
     "rsd_sync", "rsd_timer_start", "rsd_timer_stop", "rsd_set_launch", "rsd_math_probe", "rsd_profile",
-    "rsd_profile_read", "rsd_io_test",
+    "rsd_profile_read", "rsd_io_test", "rsd_set_sort", "rsd_set_fold",
 ]
 
 
@@ -228,6 +228,8 @@ def _declare(lib):
         "rsd_timer_start": (I, [P]),
         "rsd_timer_stop": (I, [P, ctypes.POINTER(ctypes.c_double)]),
         "rsd_set_launch": (I, [P, I, I]),
+        "rsd_set_sort": (I, [P, I]),
+        "rsd_set_fold": (I, [P, I]),
         "rsd_math_probe": (I, [I, I, P, P, P, S]),
         "rsd_profile": (I, [P, I]),
         "rsd_io_test": (I, [P, S, I, ctypes.POINTER(ctypes.c_double)]),
@@ -440,6 +442,18 @@ class Radar:
     def set_launch(self, block, max_blocks):
         e = self.engine()
         if self.lib.rsd_set_launch(e, block, max_blocks):
+            raise RuntimeError(self.lib.rsd_last_error(e).decode())
+
+    def set_fold(self, mode):
+        """Gate-sum strategy: 0 per-lane, 1 wave binary64, 2 wave binary32."""
+        e = self.engine()
+        if self.lib.rsd_set_fold(e, mode):
+            raise RuntimeError(self.lib.rsd_last_error(e).decode())
+
+    def set_sort(self, interval):
+        """Pulses between spatial reorders of the rain streams (0 = never)."""
+        e = self.engine()
+        if self.lib.rsd_set_sort(e, interval):
             raise RuntimeError(self.lib.rsd_last_error(e).decode())
 
     def sync(self):

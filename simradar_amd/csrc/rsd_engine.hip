@@ -23,7 +23,18 @@
 // One pulse = one fused kernel per scatterer type (advance of the previous PRT,
 // if pending, then the signal of every scatterer folded into its gates) plus a
 // finalize kernel that sums the per-block gate partials in binary64.
+//
+// Spatial order: every sort_interval pulses the rain streams (pos, vel, rcs,
+// rnd) are reordered by range (shells of 1/8 gate) and, inside a shell, by
+// a Morton index of 16^3 cells of the domain box, with id[j] = the reference
+// index of stored scatterer j.  Each scatterer keeps its own state and random
+// stream, so the physics is unchanged; the scatterers a lane visits then fall
+// into the same few range gates (summed in registers, see GateWin) and
+// neighbouring lanes gather neighbouring wind texels.  Downloads scatter through
+// id and uploads first restore the reference order, so callers never see it.
+// aux (read only by downloads) stays in the reference order.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <math.h>
 #include <stdarg.h>
@@ -70,10 +81,10 @@ __device__ __forceinline__ float4 sample3d(const Tex3 &t, float u, float v, floa
     tex_axis(u, t.nx, i0, i1, a);
     tex_axis(v, t.ny, j0, j1, b);
     tex_axis(w, t.nz, k0, k1, c);
-    const size_t nx = (size_t)t.nx, nxy = (size_t)t.nx * (size_t)t.ny;
+    const uint32_t nx = (uint32_t)t.nx, nxy = (uint32_t)t.nx * (uint32_t)t.ny;
     const float4 *D = t.d;
-    const size_t r00 = k0 * nxy + j0 * nx, r10 = k0 * nxy + j1 * nx;
-    const size_t r01 = k1 * nxy + j0 * nx, r11 = k1 * nxy + j1 * nx;
+    const uint32_t r00 = k0 * nxy + j0 * nx, r10 = k0 * nxy + j1 * nx;
+    const uint32_t r01 = k1 * nxy + j0 * nx, r11 = k1 * nxy + j1 * nx;
     float4 t000 = D[r00 + i0], t100 = D[r00 + i1];
     float4 t010 = D[r10 + i0], t110 = D[r10 + i1];
     float4 t001 = D[r01 + i0], t101 = D[r01 + i1];
@@ -208,11 +219,13 @@ struct PulseK {
     double two_over_lambda; // turns per metre of two-way path
     const float *angw; F4 angw_desc;
     const float *rangew; F4 rangew_desc;
+    int rangew_n;           // entries of the range weight table
     const float *gate_r;    // binary32 gate ranges (start + repeated adds)
     int R;
     float gate_start, gate_inv_delta;
     float sup_lo, sup_hi;   // range-offset interval outside which the weight is 0
     int dense;              // 1: every gate may be weighted
+    int fold;               // RSD_FOLD_* strategy of the gate sums
     uint32_t phase_mode;
     double *partials;       // [rows][4][R]
     int row0;               // first partial row of this launch
@@ -259,20 +272,118 @@ __device__ __forceinline__ bool scat_signal(float4 p, float4 rcs, const PulseK &
     return true;
 }
 
-// Fold one scatterer into the block's binary64 gate sums (rs.cl:1182-1217,
-// gate-sparse: only gates whose range offset lies inside the weight's support).
-__device__ __forceinline__ void fold_gates(float r, float4 sa, const PulseK &k, double *acc) {
-    float klo_f, khi_f;
-    if (k.dense) {
-        klo_f = 0.0f;
-        khi_f = (float)(k.R - 1);
-    } else {
-        klo_f = __builtin_fmaxf(__builtin_ceilf((r - k.sup_hi - k.gate_start) * k.gate_inv_delta), 0.0f);
-        khi_f = __builtin_fminf(__builtin_floorf((r - k.sup_lo - k.gate_start) * k.gate_inv_delta), (float)(k.R - 1));
+// Wave-wide binary64 sum through DPP row shifts and row broadcasts (GFX9
+// cross-lane operations): after row_shr 1/2/4/8 lane 15 of each row holds the
+// row sum, row_bcast 15 / 31 fold rows 0+1 and 2+3 into lane 63.
+template <int CTRL, int ROW_MASK, bool BOUND>
+__device__ __forceinline__ double dpp64(double v) {
+    int lo = __double2loint(v), hi = __double2hiint(v);
+    lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, ROW_MASK, 0xf, BOUND);
+    hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, ROW_MASK, 0xf, BOUND);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double wave_sum64(double v) {
+    v += dpp64<0x111, 0xf, true>(v);
+    v += dpp64<0x112, 0xf, true>(v);
+    v += dpp64<0x114, 0xf, true>(v);
+    v += dpp64<0x118, 0xf, true>(v);
+    v += dpp64<0x142, 0xa, false>(v);
+    v += dpp64<0x143, 0xc, false>(v);
+    int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+    int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+    return __hiloint2double(hi, lo);
+}
+
+// Wave-wide binary32 sum, same DPP pattern.
+template <int CTRL, int ROW_MASK, bool BOUND>
+__device__ __forceinline__ float dpp32(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, BOUND));
+}
+
+__device__ __forceinline__ float wave_sum32(float v) {
+    v += dpp32<0x111, 0xf, true>(v);
+    v += dpp32<0x112, 0xf, true>(v);
+    v += dpp32<0x114, 0xf, true>(v);
+    v += dpp32<0x118, 0xf, true>(v);
+    v += dpp32<0x142, 0xa, false>(v);
+    v += dpp32<0x143, 0xc, false>(v);
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
+__device__ __forceinline__ int wave_min_i(int v) {
+    for (int m = 32; m >= 1; m >>= 1) v = min(v, __shfl_xor(v, m));
+    return v;
+}
+
+__device__ __forceinline__ int wave_max_i(int v) {
+    for (int m = 32; m >= 1; m >>= 1) v = max(v, __shfl_xor(v, m));
+    return v;
+}
+
+#define RSD_WAVE_GATES 16
+
+// Fold one scatterer per lane into the block's binary64 gate sums
+// (rs.cl:1182-1217), gate-sparse: only gates whose range offset lies inside the
+// range weight's support.  Called by every lane of the wave (on = has a
+// contribution).  RSD_FOLD_LANE: every lane adds its own terms to LDS.  The wave
+// strategies apply when the wave's gates span at most RSD_WAVE_GATES (spatially
+// sorted scatterers): each gate's four components are summed across the wave
+// (binary64 or binary32 partial sums of 64 terms) and one lane adds them.
+__device__ __forceinline__ void fold_gates(bool on, float r, float4 sa, const PulseK &k, double *acc) {
+    int klo = 0x3fffffff, khi = -1;
+    if (on) {
+        float klo_f, khi_f;
+        if (k.dense) {
+            klo_f = 0.0f;
+            khi_f = (float)(k.R - 1);
+        } else {
+            klo_f = __builtin_fmaxf(__builtin_ceilf((r - k.sup_hi - k.gate_start) * k.gate_inv_delta), 0.0f);
+            khi_f = __builtin_fminf(__builtin_floorf((r - k.sup_lo - k.gate_start) * k.gate_inv_delta), (float)(k.R - 1));
+        }
+        if (klo_f <= khi_f) {
+            klo = (int)klo_f;
+            khi = (int)khi_f;
+        }
     }
-    if (!(klo_f <= khi_f)) return;
-    const int klo = (int)klo_f, khi = (int)khi_f;
     const int R = k.R;
+    if (k.fold != RSD_FOLD_LANE) {
+        const int kmin = wave_min_i(klo), kmax = wave_max_i(khi);
+        if (kmax < kmin) return;
+        if (kmax - kmin < RSD_WAVE_GATES) {
+            const bool lead = (threadIdx.x & 63) == 0;
+            if (k.fold == RSD_FOLD_WAVE32) {
+                for (int g = kmin; g <= kmax; g++) {
+                    float wr = 0.0f;
+                    if (g >= klo && g <= khi) wr = table_lerp(r - k.gate_r[g], k.rangew, k.rangew_desc);
+                    float s0 = wave_sum32(wr * sa.x), s1 = wave_sum32(wr * sa.y);
+                    float s2 = wave_sum32(wr * sa.z), s3 = wave_sum32(wr * sa.w);
+                    if (lead) {
+                        atomicAdd(&acc[g], (double)s0);
+                        atomicAdd(&acc[R + g], (double)s1);
+                        atomicAdd(&acc[2 * R + g], (double)s2);
+                        atomicAdd(&acc[3 * R + g], (double)s3);
+                    }
+                }
+                return;
+            }
+            for (int g = kmin; g <= kmax; g++) {
+                float wr = 0.0f;
+                if (g >= klo && g <= khi) wr = table_lerp(r - k.gate_r[g], k.rangew, k.rangew_desc);
+                double s0 = wave_sum64(wr == 0.0f ? 0.0 : (double)(wr * sa.x));
+                double s1 = wave_sum64(wr == 0.0f ? 0.0 : (double)(wr * sa.y));
+                double s2 = wave_sum64(wr == 0.0f ? 0.0 : (double)(wr * sa.z));
+                double s3 = wave_sum64(wr == 0.0f ? 0.0 : (double)(wr * sa.w));
+                if (lead) {
+                    atomicAdd(&acc[g], s0);
+                    atomicAdd(&acc[R + g], s1);
+                    atomicAdd(&acc[2 * R + g], s2);
+                    atomicAdd(&acc[3 * R + g], s3);
+                }
+            }
+            return;
+        }
+    }
     for (int g = klo; g <= khi; g++) {
         float wr = table_lerp(r - k.gate_r[g], k.rangew, k.rangew_desc);
         if (wr == 0.0f) continue;
@@ -281,6 +392,145 @@ __device__ __forceinline__ void fold_gates(float r, float4 sa, const PulseK &k, 
         atomicAdd(&acc[2 * R + g], (double)(wr * sa.z));
         atomicAdd(&acc[3 * R + g], (double)(wr * sa.w));
     }
+}
+
+// Lane-private window of RSD_WIN consecutive gates [kb, kb + RSD_WIN): binary32
+// partial sums of the scatterers a lane visits (rs.cl:1182-1217 terms).  With
+// the streams in range order the scatterers of one lane, and of one wave, fall
+// into the same gates, so the terms are summed in registers and the wave adds
+// them to LDS once per launch.  A scatterer whose gates leave the window first
+// flushes the lane's window with per-lane LDS atomics.
+#define RSD_WIN 8
+#define RSD_WIN_EMPTY (-(1 << 24))
+
+struct GateWin {
+    float a[RSD_WIN][4];
+    int kb;
+};
+
+__device__ __forceinline__ void win_reset(GateWin &w, int kb) {
+    w.kb = kb;
+#pragma unroll
+    for (int s = 0; s < RSD_WIN; s++) w.a[s][0] = w.a[s][1] = w.a[s][2] = w.a[s][3] = 0.0f;
+}
+
+// Gates whose range offset lies inside the weight's support (klo > khi: none).
+__device__ __forceinline__ void gate_support(float r, const PulseK &k, int &klo, int &khi) {
+    float klo_f, khi_f;
+    if (k.dense) {
+        klo_f = 0.0f;
+        khi_f = (float)(k.R - 1);
+    } else {
+        klo_f = __builtin_fmaxf(__builtin_ceilf((r - k.sup_hi - k.gate_start) * k.gate_inv_delta), 0.0f);
+        khi_f = __builtin_fminf(__builtin_floorf((r - k.sup_lo - k.gate_start) * k.gate_inv_delta), (float)(k.R - 1));
+    }
+    if (klo_f <= khi_f) {
+        klo = (int)klo_f;
+        khi = (int)khi_f;
+    } else {
+        klo = 0x3fffffff;
+        khi = -1;
+    }
+}
+
+__device__ __forceinline__ void win_lane_flush(const GateWin &w, const PulseK &k, double *acc) {
+    const int R = k.R;
+#pragma unroll
+    for (int s = 0; s < RSD_WIN; s++) {
+        const int g = w.kb + s;
+        if (g >= 0 && g < R) {
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+                if (w.a[s][c] != 0.0f) atomicAdd(&acc[c * R + g], (double)w.a[s][c]);
+        }
+    }
+}
+
+// Range weight of offset x from the table in LDS (same clamp / fract / mix as
+// table_lerp; the upper texel is min(ip + 1, n - 1)).  Only used where the
+// table ends in zeros, so offsets outside the support weigh exactly 0.
+__device__ __forceinline__ float lds_weight(float x, const float *W, const F4 &d, int n) {
+    float x0 = rf_clamp(rf_fma(x, d.s[0], d.s[1]), 0.0f, d.s[2]);
+    float ip = __builtin_floorf(x0);
+    float dec = __builtin_fminf(x0 - ip, 0x1.fffffep-1f);
+    int i0 = (int)ip;
+    int i1 = min(i0 + 1, n - 1);
+    return rf_mix(W[i0], W[i1], dec);
+}
+
+__device__ __forceinline__ void win_fold(GateWin &w, bool on, float r, float4 sa, const PulseK &k, double *acc,
+                                         const float *wl) {
+    int klo = 0x3fffffff, khi = -1;
+    if (on) gate_support(r, k, klo, khi);
+    on = klo <= khi;
+    if (on && (klo < w.kb || khi >= w.kb + RSD_WIN)) {
+        if (w.kb != RSD_WIN_EMPTY) win_lane_flush(w, k, acc);
+        win_reset(w, klo - 1);     // one gate of room below
+    }
+    // slots any lane of the wave needs (wave-uniform bounds)
+    const int smin = wave_min_i(on ? klo - w.kb : RSD_WIN);
+    const int smax = wave_max_i(on ? khi - w.kb : -1);
+    const int gmax = k.R - 1;
+#pragma unroll
+    for (int s = 0; s < RSD_WIN; s++) {
+        if (s < smin || s > smax) continue;
+        const int g = min(max(w.kb + s, 0), gmax);
+        float wr = lds_weight(r - k.gate_r[g], wl, k.rangew_desc, k.rangew_n);
+        if (!on || w.kb + s != g) wr = 0.0f;
+        w.a[s][0] = __builtin_fmaf(wr, sa.x, w.a[s][0]);
+        w.a[s][1] = __builtin_fmaf(wr, sa.y, w.a[s][1]);
+        w.a[s][2] = __builtin_fmaf(wr, sa.z, w.a[s][2]);
+        w.a[s][3] = __builtin_fmaf(wr, sa.w, w.a[s][3]);
+    }
+}
+
+// End of launch: the wave sums the windows of lanes that share a base gate and
+// one lane adds them to LDS; lanes whose bases spread wider flush alone.
+__device__ __forceinline__ void win_flush(GateWin &w, const PulseK &k, double *acc) {
+    const int R = k.R;
+    const int lo = wave_min_i(w.kb != RSD_WIN_EMPTY ? w.kb : 0x3fffffff);
+    const int hi = wave_max_i(w.kb);
+    if (hi == RSD_WIN_EMPTY) return;
+    if (hi - lo >= 4) {
+        if (w.kb != RSD_WIN_EMPTY) win_lane_flush(w, k, acc);
+        return;
+    }
+    const bool lead = (threadIdx.x & 63) == 0;
+    for (int v = lo; v <= hi; v++) {
+        const bool mine = w.kb == v;
+        if (__ballot(mine) == 0) continue;
+        float S[RSD_WIN][4];
+#pragma unroll
+        for (int s = 0; s < RSD_WIN; s++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) S[s][c] = wave_sum32(mine ? w.a[s][c] : 0.0f);
+        if (lead) {
+#pragma unroll
+            for (int s = 0; s < RSD_WIN; s++) {
+                if (v + s >= 0 && v + s < R) {
+#pragma unroll
+                    for (int c = 0; c < 4; c++)
+                        if (S[s][c] != 0.0f) atomicAdd(&acc[c * R + v + s], (double)S[s][c]);
+                }
+            }
+        }
+    }
+}
+
+// Contiguous range of scatterers of this block: logical block L owns
+// [L chunk, (L + 1) chunk).  Blocks are dispatched round-robin over the 8 XCDs,
+// so L = (b mod 8) (G / 8) + b / 8 gives each XCD one contiguous eighth of the
+// (spatially sorted) streams and keeps its L2 on one region of the wind table.
+__device__ __forceinline__ void block_range(size_t begin, size_t end, size_t &lo, size_t &hi) {
+    const unsigned G = gridDim.x, b = blockIdx.x;
+    const unsigned L = (G & 7u) == 0u ? (b & 7u) * (G >> 3) + (b >> 3) : b;
+    const size_t n = end - begin;
+    size_t chunk = (n + G - 1) / G;
+    chunk = (chunk + blockDim.x - 1) / blockDim.x * blockDim.x;
+    lo = begin + (size_t)L * chunk;
+    hi = lo + chunk;
+    if (lo > end) lo = end;
+    if (hi > end) hi = end;
 }
 
 __device__ __forceinline__ void acc_clear(double *acc, int n) {
@@ -387,20 +637,31 @@ __device__ __forceinline__ void logtab_load(float *dst, const float *src) {
 }
 
 template <bool ADV, bool PULSE>
-__global__ __launch_bounds__(256) void k_rain(RainK a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_rain(RainK a) {
     extern __shared__ double acc[];
     __shared__ float logtab[3 * RF_LOG_TABLE];
     if (ADV) logtab_load(logtab, a.logtab);
     if (PULSE) acc_clear(acc, 4 * a.pk.R);
     else __syncthreads();
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += stride) {
-        float4 p = a.pos[i];
+    size_t lo, hi;
+    block_range(0, a.n, lo, hi);
+    const bool use_win = PULSE && a.pk.fold == RSD_FOLD_WINDOW;
+    float *wl = (float *)(acc + 4 * a.pk.R);
+    if (use_win) {
+        for (int j = threadIdx.x; j < a.pk.rangew_n; j += blockDim.x) wl[j] = a.pk.rangew[j];
+        __syncthreads();
+    }
+    GateWin win;
+    win_reset(win, RSD_WIN_EMPTY);
+    for (size_t base = lo; base < hi; base += blockDim.x) {
+        const size_t i = base + threadIdx.x;
+        const bool live = i < hi;
+        float4 p = live ? a.pos[i] : make_float4(0.0f, 1.0f, 0.0f, 0.0f);
         bool stale = __builtin_signbit(p.w);
         p.w = __builtin_fabsf(p.w);
         float4 rcs;
         bool have_rcs = false;
-        if (ADV) {
+        if (ADV && live) {
             float4 v = a.vel[i];
             const float4 p_in = p;
             bool respawned = rain_advance(a, i, p, v, logtab);
@@ -418,12 +679,18 @@ __global__ __launch_bounds__(256) void k_rain(RainK a) {
             a.vel[i] = v;
         }
         if (PULSE) {
-            if (!have_rcs) rcs = stale ? a.rcs[i] : ellipsoid_rcs(p, a.drop, a.drop_desc);
-            float4 sa;
-            float r;
-            if (scat_signal(p, rcs, a.pk, sa, r)) fold_gates(r, sa, a.pk, acc);
+            float4 sa = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            float r = 0.0f;
+            bool on = false;
+            if (live) {
+                if (!have_rcs) rcs = stale ? a.rcs[i] : ellipsoid_rcs(p, a.drop, a.drop_desc);
+                on = scat_signal(p, rcs, a.pk, sa, r);
+            }
+            if (use_win) win_fold(win, on, r, sa, a.pk, acc, wl);
+            else fold_gates(on, r, sa, a.pk, acc);
         }
     }
+    if (use_win) win_flush(win, a.pk, acc);
     if (PULSE) acc_flush(acc, a.pk);
 }
 
@@ -547,12 +814,27 @@ __global__ __launch_bounds__(256) void k_debris(DebrisK a) {
     if (ADV) logtab_load(logtab, a.logtab);
     if (PULSE) acc_clear(acc, 4 * a.pk.R);
     else __syncthreads();
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t i = a.begin + (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.end; i += stride) {
-        float4 p = a.pos[i], o = a.ori[i];
+    size_t lo, hi;
+    block_range(a.begin, a.end, lo, hi);
+    const bool use_win = PULSE && a.pk.fold == RSD_FOLD_WINDOW;
+    float *wl = (float *)(acc + 4 * a.pk.R);
+    if (use_win) {
+        for (int j = threadIdx.x; j < a.pk.rangew_n; j += blockDim.x) wl[j] = a.pk.rangew[j];
+        __syncthreads();
+    }
+    GateWin win;
+    win_reset(win, RSD_WIN_EMPTY);
+    for (size_t base = lo; base < hi; base += blockDim.x) {
+        const size_t i = base + threadIdx.x;
+        const bool live = i < hi;
+        float4 p = make_float4(0.0f, 1.0f, 0.0f, 0.0f), o = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+        if (live) {
+            p = a.pos[i];
+            o = a.ori[i];
+        }
         float4 rcs;
         bool rcs_known = false, respawned = false;
-        if (ADV) {
+        if (ADV && live) {
             float4 v = a.vel[i], t = a.tum[i];
             respawned = debris_advance(a, i, p, o, v, t, rcs, logtab);
             a.pos[i] = p;
@@ -561,7 +843,7 @@ __global__ __launch_bounds__(256) void k_debris(DebrisK a) {
             a.tum[i] = t;
             rcs_known = true;
         }
-        if (PULSE) {
+        if (PULSE && live) {
             if (a.refresh && ADV && !respawned && a.reuse_rcs) {
                 // the advance just evaluated the same table at the same pose
             } else if (a.refresh) {
@@ -571,13 +853,16 @@ __global__ __launch_bounds__(256) void k_debris(DebrisK a) {
                 rcs = a.rcs[i];
             }
         }
-        if (rcs_known) a.rcs[i] = rcs;
+        if (rcs_known && live) a.rcs[i] = rcs;
         if (PULSE) {
-            float4 sa;
-            float r;
-            if (scat_signal(p, rcs, a.pk, sa, r)) fold_gates(r, sa, a.pk, acc);
+            float4 sa = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            float r = 0.0f;
+            bool on = live && scat_signal(p, rcs, a.pk, sa, r);
+            if (use_win) win_fold(win, on, r, sa, a.pk, acc, wl);
+            else fold_gates(on, r, sa, a.pk, acc);
         }
     }
+    if (use_win) win_flush(win, a.pk, acc);
     if (PULSE) acc_flush(acc, a.pk);
 }
 
@@ -627,6 +912,91 @@ __global__ void k_math_probe(int fn, const float *a, const float *b, float *out,
     out[i] = r;
 }
 
+// ---------------------------------------------------------------------------
+// Spatial order of the rain streams.
+// ---------------------------------------------------------------------------
+// Spread the low 10 bits of v to bit positions 0, 3, 6, ... 27.
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__device__ __forceinline__ uint32_t cell_of(float x, float o, float inv) {
+    float c = __builtin_fminf(__builtin_fmaxf((x - o) * inv, 0.0f), 1023.0f);
+    return (uint32_t)c;
+}
+
+struct SortK {
+    const float4 *pos;
+    uint32_t *key, *idx;
+    unsigned long long n;
+    float ox, oy, oz, ix, iy, iz;   // box origin, 16 / box size
+    float r0, ir;                    // first shell range, shells per metre
+    uint32_t shells;                 // number of range shells (<= 2^18)
+};
+
+// Key = range shell (high 18 bits) | Morton index of the 16^3 box cell (low 12
+// bits), and the identity permutation.
+__global__ void k_sort_keys(SortK a) {
+    size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= a.n) return;
+    float4 p = a.pos[j];
+    float r = __builtin_sqrtf((p.x * p.x + p.y * p.y) + p.z * p.z);
+    float sh = __builtin_fminf(__builtin_fmaxf((r - a.r0) * a.ir, 0.0f), (float)(a.shells - 1));
+    uint32_t cx = cell_of(p.x, a.ox, a.ix) >> 6, cy = cell_of(p.y, a.oy, a.iy) >> 6, cz = cell_of(p.z, a.oz, a.iz) >> 6;
+    a.key[j] = ((uint32_t)sh << 12) | spread3(cx) | (spread3(cy) << 1) | (spread3(cz) << 2);
+    a.idx[j] = (uint32_t)j;
+}
+
+struct PermK {
+    const float4 *pos, *vel, *rcs;
+    const uint4 *rnd;
+    const uint32_t *id, *perm;
+    float4 *pos2, *vel2, *rcs2;
+    uint4 *rnd2;
+    uint32_t *id2;
+    unsigned long long n, n_rain;
+};
+
+// Gather the permuted streams: stored j <- stored perm[j] for rain, identity
+// for the debris range (copied so the buffers can be swapped whole).  A rain
+// rcs entry is only meaningful while the scatterer is flagged (sign of pos.w).
+__global__ void k_permute(PermK a) {
+    size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= a.n) return;
+    const bool rain = j < a.n_rain;
+    const size_t s = rain ? a.perm[j] : j;
+    const float4 p = a.pos[s];
+    a.pos2[j] = p;
+    a.vel2[j] = a.vel[s];
+    if (!rain || __builtin_signbit(p.w)) a.rcs2[j] = a.rcs[s];
+    a.rnd2[j] = a.rnd[s];
+    if (rain) a.id2[j] = a.id[s];
+}
+
+// Scatter the streams back to the reference order (stored j -> id[j]).
+__global__ void k_unpermute(PermK a) {
+    size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= a.n) return;
+    const bool rain = j < a.n_rain;
+    const size_t o = rain ? a.id[j] : j;
+    const float4 p = a.pos[j];
+    a.pos2[o] = p;
+    a.vel2[o] = a.vel[j];
+    if (!rain || __builtin_signbit(p.w)) a.rcs2[o] = a.rcs[j];
+    a.rnd2[o] = a.rnd[j];
+    if (rain) a.id2[o] = (uint32_t)o;
+}
+
+__global__ void k_iota(uint32_t *id, unsigned long long n) {
+    size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) id[j] = (uint32_t)j;
+}
+
 // Mark every rain scatterer's uploaded rcs as authoritative (sign bit of pos.w).
 __global__ void k_flag_rain(float4 *pos, unsigned long long n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -640,8 +1010,11 @@ __global__ void k_flag_rain(float4 *pos, unsigned long long n) {
 // Download view of the streams: |pos.w|, reconstructed rain rcs, and the
 // rs.cl:1050-1110 signal / auxiliary quantities for the last pulse's beam.
 struct ViewK {
-    const float4 *pos, *rcs, *aux_in;
-    float4 *pos_out, *rcs_out, *sig_out, *aux_out;
+    const float4 *pos, *rcs, *aux_in, *vel;
+    const uint4 *rnd;
+    const uint32_t *id;     // reference index of each stored rain scatterer (NULL: identity)
+    float4 *pos_out, *rcs_out, *sig_out, *aux_out, *vel_out;
+    uint4 *rnd_out;
     unsigned long long n, n_rain;
     const float4 *drop;
     F4 drop_desc;
@@ -652,6 +1025,7 @@ struct ViewK {
 __global__ void k_view(ViewK a) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
+    const size_t o = (a.id && i < a.n_rain) ? a.id[i] : i;
     float4 p = a.pos[i];
     bool stale = __builtin_signbit(p.w);
     float4 rcs;
@@ -661,9 +1035,13 @@ __global__ void k_view(ViewK a) {
     } else {
         rcs = a.rcs[i];
     }
-    a.pos_out[i] = p;
-    a.rcs_out[i] = rcs;
-    float4 aux = a.aux_in[i];
+    a.pos_out[o] = p;
+    a.rcs_out[o] = rcs;
+    if (a.vel_out) {
+        a.vel_out[o] = a.vel[i];
+        a.rnd_out[o] = a.rnd[i];
+    }
+    float4 aux = a.aux_in[o];    // aux stays in the reference order
     float4 sig = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (a.have_pulse) {
         const PulseK &k = a.pk;
@@ -684,8 +1062,8 @@ __global__ void k_view(ViewK a) {
         aux.y = aux.y + k.sd.s[15];
         aux.w = wa;
     }
-    a.sig_out[i] = sig;
-    a.aux_out[i] = aux;
+    a.sig_out[o] = sig;
+    a.aux_out[o] = aux;
 }
 
 // ---------------------------------------------------------------------------
@@ -694,12 +1072,26 @@ __global__ void k_view(ViewK a) {
 struct rsd_engine {
     int device;
     hipStream_t stream;
+    hipStream_t side;       // debris kernels, forked from / joined into stream
+    hipEvent_t ev_fork, ev_join;
     char err[512];
     size_t n, n_rain;
     size_t origins[RSD_MAX_TYPES], counts[RSD_MAX_TYPES];
     float4 *pos, *vel, *ori, *tum, *rcs, *aux;
     uint4 *rnd;
-    float4 *view;           // download staging: 4 x n float4
+    float4 *view;           // download staging: 6 x n float4
+    // spatial order (rain range only): ping-pong streams, reference index,
+    // radix-sort keys / permutation and hipCUB scratch
+    float4 *pos2, *vel2, *rcs2;
+    uint4 *rnd2;
+    uint32_t *id, *id2, *skey, *skey2, *sperm, *sperm2;
+    int sort_bits;          // key bits of the current sort
+    void *sort_tmp;
+    size_t sort_tmp_bytes;
+    int sort_interval;      // pulses between sorts (0 = never)
+    int fold;               // RSD_FOLD_* strategy of the gate sums
+    int since_sort;
+    int permuted;           // stored rain order differs from the reference order
     float4 *wind[2];
     int wnx[2], wny[2], wnz[2];
     F16 les[2];
@@ -721,6 +1113,7 @@ struct rsd_engine {
     float *gate_r;
     int gate_R;
     float gate_start, gate_delta;
+    double gate_drift;      // max |gate_r[k] - (start + k delta)| of the binary32 sums
     double *partials;
     size_t partial_cap;
     double *pulse_d;
@@ -807,7 +1200,12 @@ extern "C" int rsd_open(int device_index, rsd_engine **out) {
     e->device = device_index;
     e->block = 256;
     e->max_blocks = 2048;
+    e->sort_interval = 32;
+    e->fold = RSD_FOLD_WINDOW;
     if (hipSetDevice(device_index) != hipSuccess || hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming) != hipSuccess ||
         hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess ||
         hipHostMalloc((void **)&e->pulse_h, 4 * RSD_MAX_GATES * sizeof(double), hipHostMallocDefault) != hipSuccess ||
         hipMalloc((void **)&e->pulse_d, 4 * RSD_MAX_GATES * sizeof(double)) != hipSuccess ||
@@ -822,8 +1220,17 @@ extern "C" int rsd_open(int device_index, rsd_engine **out) {
 static void free_state(rsd_engine *e) {
     dfree(e->pos); dfree(e->vel); dfree(e->ori); dfree(e->tum); dfree(e->rcs); dfree(e->aux);
     dfree(e->rnd); dfree(e->view);
+    dfree(e->pos2); dfree(e->vel2); dfree(e->rcs2); dfree(e->rnd2);
+    dfree(e->id); dfree(e->id2); dfree(e->skey); dfree(e->skey2); dfree(e->sperm); dfree(e->sperm2);
+    dfree(e->sort_tmp);
     e->pos = e->vel = e->ori = e->tum = e->rcs = e->aux = e->view = NULL;
-    e->rnd = NULL;
+    e->pos2 = e->vel2 = e->rcs2 = NULL;
+    e->rnd = e->rnd2 = NULL;
+    e->id = e->id2 = e->skey = e->skey2 = e->sperm = e->sperm2 = NULL;
+    e->sort_tmp = NULL;
+    e->sort_tmp_bytes = 0;
+    e->permuted = 0;
+    e->since_sort = 0;
 }
 
 extern "C" void rsd_close(rsd_engine *e) {
@@ -846,6 +1253,10 @@ extern "C" void rsd_close(rsd_engine *e) {
     }
     (void)hipEventDestroy(e->ev0);
     (void)hipEventDestroy(e->ev1);
+    (void)hipEventDestroy(e->ev_fork);
+    (void)hipEventDestroy(e->ev_join);
+    (void)hipStreamSynchronize(e->side);
+    (void)hipStreamDestroy(e->side);
     (void)hipStreamDestroy(e->stream);
     free(e);
 }
@@ -916,10 +1327,14 @@ static int ensure_gates(rsd_engine *e, float start, float delta, int R) {
     float *h = (float *)malloc((size_t)R * sizeof(float));
     if (!h) return fail(e, RSD_ERR_ALLOC, "gate table");
     float r = start;
+    double drift = 0.0;
     for (int k = 0; k < R; k++) {  // rs.cl:1189 / 1214: repeated binary32 additions
         h[k] = r;
+        double d = fabs((double)r - ((double)start + (double)k * (double)delta));
+        if (d > drift) drift = d;
         r += delta;
     }
+    e->gate_drift = drift;
     dfree(e->gate_r);
     e->gate_r = NULL;
     hipError_t st = hipMalloc((void **)&e->gate_r, (size_t)R * sizeof(float));
@@ -956,9 +1371,18 @@ static void range_support(const rsd_engine *e, float delta, float *lo, float *hi
         *hi = -1.0f;
         return;
     }
-    float pad = 2.0f * fabsf(delta) + 1.0f;
-    *lo = ((float)(f - 1) - x0) / xs - pad;
-    *hi = ((float)(l + 1) - x0) / xs + pad;
+    if (!(delta > 0.0f)) {     // the gate-index bounds below assume increasing ranges
+        *dense = 1;
+        *lo = -INFINITY;
+        *hi = INFINITY;
+        return;
+    }
+    // Margin for the drift of the summed gate ranges and the rounding of the
+    // binary32 gate-index bounds computed from r in the kernels.
+    const double R = (double)e->gate_R;
+    const double pad = e->gate_drift + 1e-3 * delta + 1e-6 * (fabs((double)e->gate_start) + R * delta) + 1e-3;
+    *lo = (float)(((double)(f - 1) - x0) / xs - pad);
+    *hi = (float)(((double)(l + 1) - x0) / xs + pad);
 }
 
 static int fill_pulse(rsd_engine *e, const rsd_pulse *p, PulseK &k) {
@@ -970,6 +1394,7 @@ static int fill_pulse(rsd_engine *e, const rsd_pulse *p, PulseK &k) {
     k.angw = e->angw;
     k.angw_desc = e->angw_desc;
     k.rangew = e->rangew;
+    k.rangew_n = e->rangew_n;
     k.rangew_desc = e->rangew_desc;
     k.gate_r = e->gate_r;
     k.R = (int)p->range_count;
@@ -978,6 +1403,12 @@ static int fill_pulse(rsd_engine *e, const rsd_pulse *p, PulseK &k) {
     range_support(e, p->range_delta, &k.sup_lo, &k.sup_hi, &k.dense);
     if (p->range_delta == 0.0f) k.dense = 1;
     k.phase_mode = p->phase_mode;
+    k.fold = e->fold;
+    // The window needs every scatterer's gates inside RSD_WIN consecutive gates
+    // (with room to move); wider supports take the wave sums.
+    if (k.fold == RSD_FOLD_WINDOW &&
+        (k.dense || (k.sup_hi - k.sup_lo) * k.gate_inv_delta >= (float)(RSD_WIN - 2)))
+        k.fold = RSD_FOLD_WAVE32;
     k.partials = e->partials;
     k.row0 = 0;
     return RSD_OK;
@@ -1061,15 +1492,37 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p) {
         st = fill_pulse(e, p, pk);
         if (st) return st;
     }
-    const size_t lds = p ? (size_t)4 * R * sizeof(double) : 0;
-    int row = 0;
+    const size_t lds = p ? (size_t)4 * R * sizeof(double) + (pk.fold == RSD_FOLD_WINDOW ? (size_t)e->rangew_n * sizeof(float) : 0) : 0;
+    // Debris types run on the side stream, concurrently with the rain kernel;
+    // partial rows: rain blocks first, then each debris type.
+    const bool any_debris = rows > rain_blocks;
+    if (any_debris) {
+        HIPCHK(e, hipEventRecord(e->ev_fork, e->stream));
+        HIPCHK(e, hipStreamWaitEvent(e->side, e->ev_fork, 0));
+        int row = rain_blocks;
+        for (int k = 1; k < RSD_MAX_TYPES; k++) {
+            if (!e->counts[k]) continue;
+            DebrisK a;
+            memset(&a, 0, sizeof(a));
+            int st = fill_debris(e, s, p, k, a);
+            if (st) return st;
+            a.pk = pk;
+            a.pk.row0 = row;
+            if (s && p) hipLaunchKernelGGL((k_debris<true, true>), dim3(deb_blocks[k]), dim3(e->block), lds, e->side, a);
+            else if (s) hipLaunchKernelGGL((k_debris<true, false>), dim3(deb_blocks[k]), dim3(e->block), 0, e->side, a);
+            else hipLaunchKernelGGL((k_debris<false, true>), dim3(deb_blocks[k]), dim3(e->block), lds, e->side, a);
+            HIPCHK(e, hipGetLastError());
+            row += deb_blocks[k];
+        }
+        HIPCHK(e, hipEventRecord(e->ev_join, e->side));
+    }
     if (e->n_rain) {
         RainK a;
         memset(&a, 0, sizeof(a));
         int st = fill_rain(e, s, a);
         if (st) return st;
         a.pk = pk;
-        a.pk.row0 = row;
+        a.pk.row0 = 0;
         const bool timed = e->prof_on && e->prof_n < RSD_PROF_CAP;
         if (timed) HIPCHK(e, hipEventRecord(e->prof_ev[2 * e->prof_n], e->stream));
         if (s && p) hipLaunchKernelGGL((k_rain<true, true>), dim3(rain_blocks), dim3(e->block), lds, e->stream, a);
@@ -1081,28 +1534,128 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p) {
             e->prof_kind[e->prof_n] = (s ? 1 : 0) | (p ? 2 : 0);
             e->prof_n++;
         }
-        row += rain_blocks;
     }
-    for (int k = 1; k < RSD_MAX_TYPES; k++) {
-        if (!e->counts[k]) continue;
-        DebrisK a;
-        memset(&a, 0, sizeof(a));
-        int st = fill_debris(e, s, p, k, a);
-        if (st) return st;
-        a.pk = pk;
-        a.pk.row0 = row;
-        if (s && p) hipLaunchKernelGGL((k_debris<true, true>), dim3(deb_blocks[k]), dim3(e->block), lds, e->stream, a);
-        else if (s) hipLaunchKernelGGL((k_debris<true, false>), dim3(deb_blocks[k]), dim3(e->block), 0, e->stream, a);
-        else hipLaunchKernelGGL((k_debris<false, true>), dim3(deb_blocks[k]), dim3(e->block), lds, e->stream, a);
-        HIPCHK(e, hipGetLastError());
-        row += deb_blocks[k];
-    }
+    if (any_debris) HIPCHK(e, hipStreamWaitEvent(e->stream, e->ev_join, 0));
     if (p) {
         hipLaunchKernelGGL(k_finalize, dim3(4 * R), dim3(256), 0, e->stream, (const double *)e->partials, rows, 4 * R, e->pulse_d);
         HIPCHK(e, hipGetLastError());
         e->last_pulse = *p;
         e->have_pulse = 1;
     }
+    return RSD_OK;
+}
+
+static int ensure_sort_buffers(rsd_engine *e) {
+    if (e->pos2) return RSD_OK;
+    const size_t b = e->n * sizeof(float4), bi = e->n_rain * sizeof(uint32_t);
+    HIPCHK(e, hipMalloc((void **)&e->pos2, b));
+    HIPCHK(e, hipMalloc((void **)&e->vel2, b));
+    HIPCHK(e, hipMalloc((void **)&e->rcs2, b));
+    HIPCHK(e, hipMalloc((void **)&e->rnd2, e->n * sizeof(uint4)));
+    HIPCHK(e, hipMalloc((void **)&e->id, bi));
+    HIPCHK(e, hipMalloc((void **)&e->id2, bi));
+    HIPCHK(e, hipMalloc((void **)&e->skey, bi));
+    HIPCHK(e, hipMalloc((void **)&e->skey2, bi));
+    HIPCHK(e, hipMalloc((void **)&e->sperm, bi));
+    HIPCHK(e, hipMalloc((void **)&e->sperm2, bi));
+    size_t tmp = 0;
+    HIPCHK(e, hipcub::DeviceRadixSort::SortPairs(NULL, tmp, e->skey, e->skey2, e->sperm, e->sperm2, (int)e->n_rain, 0, 30, e->stream));
+    HIPCHK(e, hipMalloc(&e->sort_tmp, tmp));
+    e->sort_tmp_bytes = tmp;
+    return RSD_OK;
+}
+
+static void swap_streams(rsd_engine *e) {
+    float4 *t;
+    t = e->pos; e->pos = e->pos2; e->pos2 = t;
+    t = e->vel; e->vel = e->vel2; e->vel2 = t;
+    t = e->rcs; e->rcs = e->rcs2; e->rcs2 = t;
+    uint4 *r = e->rnd; e->rnd = e->rnd2; e->rnd2 = r;
+    uint32_t *u = e->id; e->id = e->id2; e->id2 = u;
+}
+
+static PermK perm_args(const rsd_engine *e) {
+    PermK a;
+    a.pos = e->pos; a.vel = e->vel; a.rcs = e->rcs; a.rnd = e->rnd;
+    a.id = e->id; a.perm = e->sperm2;
+    a.pos2 = e->pos2; a.vel2 = e->vel2; a.rcs2 = e->rcs2; a.rnd2 = e->rnd2;
+    a.id2 = e->id2;
+    a.n = e->n;
+    a.n_rain = e->n_rain;
+    return a;
+}
+
+// Reorder the rain streams by range shell and box cell (sim_desc slots 8-10
+// box origin, 12-14 size; shell width = gate spacing / 8).
+static int sort_rain(rsd_engine *e, const float *sd, float gate_delta) {
+    if (e->n_rain < 2 || e->n_rain > 0x7fffffffu) return RSD_OK;
+    if (!(sd[12] > 0.0f && sd[13] > 0.0f && sd[14] > 0.0f)) return RSD_OK;
+    int st = ensure_sort_buffers(e);
+    if (st) return st;
+    const unsigned nb_rain = (unsigned)((e->n_rain + 255) / 256), nb_all = (unsigned)((e->n + 255) / 256);
+    if (!e->permuted) {
+        hipLaunchKernelGGL(k_iota, dim3(nb_rain), dim3(256), 0, e->stream, e->id, (unsigned long long)e->n_rain);
+        HIPCHK(e, hipGetLastError());
+    }
+    SortK k;
+    k.pos = e->pos; k.key = e->skey; k.idx = e->sperm; k.n = e->n_rain;
+    k.ox = sd[8]; k.oy = sd[9]; k.oz = sd[10];
+    k.ix = 1024.0f / sd[12]; k.iy = 1024.0f / sd[13]; k.iz = 1024.0f / sd[14];
+    // nearest and farthest range of the box from the radar at the origin
+    double near2 = 0.0, far2 = 0.0;
+    for (int a = 0; a < 3; a++) {
+        double lo = sd[8 + a], hi = (double)sd[8 + a] + sd[12 + a];
+        double dn = lo > 0.0 ? lo : (hi < 0.0 ? -hi : 0.0);
+        double df = fabs(lo) > fabs(hi) ? fabs(lo) : fabs(hi);
+        near2 += dn * dn;
+        far2 += df * df;
+    }
+    const double rn = sqrt(near2), rf = sqrt(far2);
+    double width = fabs((double)gate_delta) / 8.0;
+    if (!(width > 0.0)) width = (rf - rn) / 4096.0;
+    if ((rf - rn) / width > 262143.0) width = (rf - rn) / 262143.0;
+    k.r0 = (float)rn;
+    k.ir = (float)(1.0 / width);
+    k.shells = (uint32_t)((rf - rn) / width) + 1u;
+    if (k.shells > 262144u) k.shells = 262144u;
+    int bits = 12;
+    while ((1u << (bits - 12)) < k.shells) bits++;
+    hipLaunchKernelGGL(k_sort_keys, dim3(nb_rain), dim3(256), 0, e->stream, k);
+    HIPCHK(e, hipGetLastError());
+    size_t tmp = e->sort_tmp_bytes;
+    HIPCHK(e, hipcub::DeviceRadixSort::SortPairs(e->sort_tmp, tmp, e->skey, e->skey2, e->sperm, e->sperm2, (int)e->n_rain, 0, bits, e->stream));
+    PermK a = perm_args(e);
+    hipLaunchKernelGGL(k_permute, dim3(nb_all), dim3(256), 0, e->stream, a);
+    HIPCHK(e, hipGetLastError());
+    swap_streams(e);
+    e->permuted = 1;
+    e->since_sort = 0;
+    return RSD_OK;
+}
+
+// Restore the reference order of the rain streams.
+static int unsort_rain(rsd_engine *e) {
+    if (!e->permuted) return RSD_OK;
+    const unsigned nb_all = (unsigned)((e->n + 255) / 256);
+    PermK a = perm_args(e);
+    hipLaunchKernelGGL(k_unpermute, dim3(nb_all), dim3(256), 0, e->stream, a);
+    HIPCHK(e, hipGetLastError());
+    swap_streams(e);
+    e->permuted = 0;
+    return RSD_OK;
+}
+
+extern "C" int rsd_set_fold(rsd_engine *e, int mode) {
+    if (!e) return RSD_ERR_ARG;
+    if (mode < RSD_FOLD_LANE || mode > RSD_FOLD_WINDOW) return fail(e, RSD_ERR_ARG, "fold mode %d unknown", mode);
+    e->fold = mode;
+    return RSD_OK;
+}
+
+extern "C" int rsd_set_sort(rsd_engine *e, int interval) {
+    if (!e) return RSD_ERR_ARG;
+    if (interval < 0) return fail(e, RSD_ERR_ARG, "sort interval %d < 0", interval);
+    e->sort_interval = interval;
     return RSD_OK;
 }
 
@@ -1116,6 +1669,9 @@ extern "C" int rsd_upload_state(rsd_engine *e, const float *pos, const float *ve
                                 const float *tum, const float *aux, const float *rcs, const uint32_t *rnd) {
     if (!e || !e->pos) return fail(e, RSD_ERR_STATE, "state not allocated");
     int st = flush_pending(e);
+    if (st) return st;
+    HIPCHK(e, hipSetDevice(e->device));
+    st = unsort_rain(e);
     if (st) return st;
     const size_t b = e->n * sizeof(float4);
     if (pos) HIPCHK(e, hipMemcpyAsync(e->pos, pos, b, hipMemcpyHostToDevice, e->stream));
@@ -1139,13 +1695,20 @@ extern "C" int rsd_download_state(rsd_engine *e, float *pos, float *vel, float *
     if (!e || !e->pos) return fail(e, RSD_ERR_STATE, "state not allocated");
     int st = flush_pending(e);
     if (st) return st;
+    HIPCHK(e, hipSetDevice(e->device));
     const size_t b = e->n * sizeof(float4);
-    if (pos || rcs || sig || aux) {
-        if (!e->view) HIPCHK(e, hipMalloc((void **)&e->view, 4 * b));
+    const bool perm = e->permuted && (vel || rnd);
+    if (pos || rcs || sig || aux || perm) {
+        if (!e->view) HIPCHK(e, hipMalloc((void **)&e->view, 6 * b));
         ViewK v;
         memset(&v, 0, sizeof(v));
-        v.pos = e->pos; v.rcs = e->rcs; v.aux_in = e->aux;
+        v.pos = e->pos; v.rcs = e->rcs; v.aux_in = e->aux; v.vel = e->vel; v.rnd = e->rnd;
+        v.id = e->permuted ? e->id : NULL;
         v.pos_out = e->view; v.rcs_out = e->view + e->n; v.sig_out = e->view + 2 * e->n; v.aux_out = e->view + 3 * e->n;
+        if (perm) {
+            v.vel_out = e->view + 4 * e->n;
+            v.rnd_out = (uint4 *)(e->view + 5 * e->n);
+        }
         v.n = e->n;
         v.n_rain = e->n_rain;
         v.drop = e->drop;
@@ -1164,11 +1727,13 @@ extern "C" int rsd_download_state(rsd_engine *e, float *pos, float *vel, float *
         if (rcs) HIPCHK(e, hipMemcpyAsync(rcs, v.rcs_out, b, hipMemcpyDeviceToHost, e->stream));
         if (sig) HIPCHK(e, hipMemcpyAsync(sig, v.sig_out, b, hipMemcpyDeviceToHost, e->stream));
         if (aux) HIPCHK(e, hipMemcpyAsync(aux, v.aux_out, b, hipMemcpyDeviceToHost, e->stream));
+        if (perm && vel) HIPCHK(e, hipMemcpyAsync(vel, v.vel_out, b, hipMemcpyDeviceToHost, e->stream));
+        if (perm && rnd) HIPCHK(e, hipMemcpyAsync(rnd, v.rnd_out, e->n * sizeof(uint4), hipMemcpyDeviceToHost, e->stream));
     }
-    if (vel) HIPCHK(e, hipMemcpyAsync(vel, e->vel, b, hipMemcpyDeviceToHost, e->stream));
+    if (vel && !perm) HIPCHK(e, hipMemcpyAsync(vel, e->vel, b, hipMemcpyDeviceToHost, e->stream));
     if (ori) HIPCHK(e, hipMemcpyAsync(ori, e->ori, b, hipMemcpyDeviceToHost, e->stream));
     if (tum) HIPCHK(e, hipMemcpyAsync(tum, e->tum, b, hipMemcpyDeviceToHost, e->stream));
-    if (rnd) HIPCHK(e, hipMemcpyAsync(rnd, e->rnd, e->n * sizeof(uint4), hipMemcpyDeviceToHost, e->stream));
+    if (rnd && !perm) HIPCHK(e, hipMemcpyAsync(rnd, e->rnd, e->n * sizeof(uint4), hipMemcpyDeviceToHost, e->stream));
     HIPCHK(e, hipStreamSynchronize(e->stream));
     return RSD_OK;
 }
@@ -1289,6 +1854,12 @@ extern "C" int rsd_advance(rsd_engine *e, const rsd_step *step) {
 extern "C" int rsd_make_pulse(rsd_engine *e, const rsd_pulse *p) {
     if (!e || !p) return fail(e, RSD_ERR_ARG, "null pulse");
     if (!e->pos) return fail(e, RSD_ERR_STATE, "state not allocated");
+    if (e->sort_interval > 0 && e->n_rain > 1 && (!e->permuted || e->since_sort >= e->sort_interval)) {
+        HIPCHK(e, hipSetDevice(e->device));
+        int st = sort_rain(e, e->have_pending ? e->pending.sim_desc : p->sim_desc, p->range_delta);
+        if (st) return st;
+    }
+    e->since_sort++;
     if (e->have_pending) {
         e->have_pending = 0;
         return launch_step(e, &e->pending, p);

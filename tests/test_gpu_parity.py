@@ -48,11 +48,15 @@ def test_device_atan2_div_match(gpu_ready):
     assert np.array_equal(_bits(sa.math_probe(7, y, x)), _bits(y / x))
 
 
-def _run(cfg, pulses):
+def _run(cfg, pulses, sort=None, reupload_at=None, fold=None):
     r = sa.Radar(verb=0)
     h.configure(r, cfg)
     r.populate()
     r.check()
+    if sort is not None:
+        r.set_sort(sort)
+    if fold is not None:
+        r.set_fold(fold)
     st = h.state_from_radar(r)     # host streams still hold the populated state
     o = h.OracleRun(r, cfg, st)
     sd_adv = r.sim_desc().copy()   # populate's zero-length step (rs.c:4941-4955)
@@ -61,6 +65,12 @@ def _run(cfg, pulses):
     o.refresh_debris_rcs(r.sim_desc())
     pg, po = [], []
     for k in range(pulses):
+        if k == reupload_at:
+            # reference-order round trip of every stream through the host
+            r.download()
+            r.check()
+            r.upload()
+            r.check()
         if k > 0:
             r.make_pulse()
             o.refresh_debris_rcs(r.sim_desc())
@@ -112,6 +122,50 @@ def test_debris_step_parity(gpu_ready):
     _check_state(r, o)
     n0 = o.counts[0]
     np.testing.assert_allclose(r.stream("rcs")[n0:], o.st.rcs[n0:], rtol=1e-5, atol=1e-9)
+    errs = [h.rel_err(pg[k], po[k]) for k in range(len(pg))]
+    assert max(errs) < IQ_TOL, errs
+    r.free()
+
+
+def test_rain_sorted_every_pulse(gpu_ready):
+    """Spatial reorder before every pulse: streams still come back bit-exact in
+    the reference order and the gate sums are unchanged."""
+    cfg = h.Config(n_rain=8192, debris=[(sa.OBJ_LEAF, 512)], scan="P:3.0,-1:1:0.01")
+    r, o, pg, po = _run(cfg, 8, sort=1)
+    _check_state(r, o)
+    errs = [h.rel_err(pg[k], po[k]) for k in range(len(pg))]
+    assert max(errs) < IQ_TOL, errs
+    r.free()
+
+
+def test_sorted_state_upload_roundtrip(gpu_ready):
+    """Download + upload while the rain streams are reordered restores the
+    reference order first; the run continues bit-exact."""
+    cfg = h.Config(n_rain=8192, debris=[(sa.OBJ_LEAF, 256)], scan="P:3.0,-1:1:0.01")
+    r, o, pg, po = _run(cfg, 10, sort=3, reupload_at=5)
+    _check_state(r, o)
+    errs = [h.rel_err(pg[k], po[k]) for k in range(len(pg))]
+    assert max(errs) < IQ_TOL, errs
+    r.free()
+
+
+@pytest.mark.parametrize("fold", [0, 1, 2, 3])
+@pytest.mark.parametrize("sort", [0, 1])
+def test_fold_strategies(gpu_ready, fold, sort):
+    """Every gate-sum strategy, on range-sorted and on reference-order streams."""
+    cfg = h.Config(n_rain=16384, debris=[(sa.OBJ_LEAF, 256)], scan="P:3.0,-1:1:0.01")
+    r, o, pg, po = _run(cfg, 4, sort=sort, fold=fold)
+    _check_state(r, o)
+    errs = [h.rel_err(pg[k], po[k]) for k in range(len(pg))]
+    assert max(errs) < IQ_TOL, errs
+    r.free()
+
+
+def test_unsorted_engine_parity(gpu_ready):
+    """Sorting switched off: the stored order is the reference order throughout."""
+    cfg = h.Config(n_rain=8192, scan="P:3.0,-1:1:0.01")
+    r, o, pg, po = _run(cfg, 6, sort=0)
+    _check_state(r, o, ("pos", "vel"))
     errs = [h.rel_err(pg[k], po[k]) for k in range(len(pg))]
     assert max(errs) < IQ_TOL, errs
     r.free()
