@@ -8,8 +8,22 @@ tables kept under profiles/.
 kernel-trace statistics); `counters` prints the per-dispatch average of every
 PMC counter collected, per kernel.
 """
+import re
 import sqlite3
 import sys
+
+
+def short(name):
+    """Readable name of a kernel: library template instantiations are reduced
+    to the library's kernel function (plus the lambda index that tells two
+    dispatches of it apart)."""
+    if len(name) <= 100:
+        return name
+    m = re.search(r"detail::(radix_sort_onesweep_(?:iteration|global_offsets))", name) or \
+        re.search(r"detail::(radix_sort_\w+|device_\w+|\w+_kernel)", name)
+    lam = re.findall(r"lambda\(auto:1\)#(\d)", name)
+    base = name.split("::")[0] + "::" + (m.group(1) if m else name[:60])
+    return base + (" #" + lam[-1] if lam else "")
 
 
 def kernels(dbs):
@@ -17,7 +31,7 @@ def kernels(dbs):
     for db in dbs:
         c = sqlite3.connect(db)
         for name, calls, total, avg in c.execute("select name, total_calls, total_duration, average from top_kernels"):
-            r = rows.setdefault(name, [0, 0.0])
+            r = rows.setdefault(short(name), [0, 0.0])
             r[0] += calls
             r[1] += total
     total_all = sum(v[1] for v in rows.values()) or 1.0
@@ -34,7 +48,7 @@ def counters(dbs):
         c = sqlite3.connect(db)
         q = "select kernel_name, counter_name, value, duration from counters_collection"
         for name, counter, value, dur in c.execute(q):
-            r = rows.setdefault((name, counter), [0, 0.0, 0.0])
+            r = rows.setdefault((short(name), counter), [0, 0.0, 0.0])
             r[0] += 1
             r[1] += value
             r[2] += dur
