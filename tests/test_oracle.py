@@ -385,3 +385,122 @@ def test_el_atts_respawn_draw():
     np.testing.assert_array_equal(st.pos[0, :3], np.array(want, np.float32))
     np.testing.assert_array_equal(st.vel[0], [0, 0, 0, 0])
     np.testing.assert_array_equal(st.rcs[0], [7.0, 8.0, 9.0, 10.0])
+
+
+# ---------------------------------------------------------------- background and debris (rs.cl:533-580, 744-919)
+
+def _box_sd(dt, concept=0):
+    sd = np.zeros(16, np.float32)
+    sd[8:11] = [-1e4, -1e4, -1e4]
+    sd[12:15] = [2e4, 2e4, 2e4]
+    sd[11] = dt
+    sd[5] = np.array([concept], np.uint32).view(np.float32)[0]
+    return sd
+
+
+def test_bg_atts_is_advect_then_sample():
+    """bg_atts: pos += vel * dt without fma (rs.cl:551), then vel = wind at the
+    new position (rs.cl:573); rcs from the drop table."""
+    t, keep, les = _uniform_wind(3.0, -2.0, 0.5)
+    drop = np.zeros((96, 4), np.float32)
+    dd = np.array([20000.0, -5.0, 95.0, 1.0], np.float32)
+    sd = _box_sd(0.01)
+    v0 = np.array([[1.25, 0.5, -4.0, 0.0]], np.float32)
+    p0 = np.array([[10.0, 20.0, 30.0, 1e-3]], np.float32)
+    st = orc.State(p0, v0, [[0, 0, 0, 1]], [[0, 0, 0, 1]], [[0, 0, 0, 0]], np.array([[1, 2, 3, 4]], np.uint32))
+    orc.bg_atts(st, 0, 1, t, les, drop, dd, sd)
+    dt = np.float32(0.01)
+    want = p0[0, :3] + np.float32(v0[0, :3] * dt)          # binary32 product, then binary32 sum
+    assert np.array_equal(st.pos[0, :3], want.astype(np.float32))
+    np.testing.assert_array_equal(st.vel[0, :3], [3.0, -2.0, 0.5])
+
+
+def _const_tex2(val, nx=9, ny=7):
+    return orc.tex(np.tile(np.array(val, np.float32), (ny, nx, 1)))
+
+
+def _debris_step(ori, cd, cm, u=(6.0, 0.0, 0.0), dt=0.01, ta=0.05, inln=(2.0, 3.0, 4.0), concept=0):
+    wt, wk, les = _uniform_wind(*u)
+    cdt, cdk = _const_tex2(cd)
+    cmt, cmk = _const_tex2(cm)
+    ret, rek = _const_tex2([0.0, 0.0, 0.0, 0.0])
+    imt, imk = _const_tex2([0.0, 0.0, 0.0, 0.0])
+    ad = np.zeros(16, np.float32)
+    ad[0:2] = 1.0
+    ad[4:6] = 1.0
+    ad[8:10] = [8, 6]
+    ad[12:15] = inln
+    ad[15] = ta
+    rd = np.zeros(16, np.float32)
+    rd[0:2] = 1.0
+    rd[4:6] = 1.0
+    sd = _box_sd(dt, concept)
+    st = orc.State([[5.0, 100.0, 50.0, 0.0]], [[0, 0, 0, 0]], [ori], [[0, 0, 0, 1]], [[0, 0, 0, 0]],
+                   np.array([[11, 12, 13, 14]], np.uint32))
+    orc.db_atts(st, 0, 1, wt, les, cdt, cmt, ad, ret, imt, rd, sd)
+    return st
+
+
+def test_db_atts_force_is_the_drag_coefficient_rotated_into_the_world():
+    """dudt = Ta |u_r|^2 rot(cd, ori) + g (rs.cl:195-261) and pos moves with the
+    old velocity: a body at rest in wind u = (6, 0, 0) with a constant table."""
+    ta, dt, c = 0.05, 0.01, 0.3
+    st = _debris_step([0, 0, 0, 1], [c, 0, 0, 0], [0, 0, 0, 0], ta=ta, dt=dt)
+    np.testing.assert_allclose(st.vel[0, :3], [ta * 36 * c * dt, 0.0, -9.8 * dt], rtol=1e-6)
+    np.testing.assert_array_equal(st.pos[0, :3], [5.0, 100.0, 50.0])
+    # the same body yawed +90 deg about z: body x is world y (q v q*, rs.cl:81-85)
+    s = np.float32(np.sqrt(0.5))
+    st = _debris_step([0, 0, s, s], [c, 0, 0, 0], [0, 0, 0, 0], ta=ta, dt=dt)
+    np.testing.assert_allclose(st.vel[0, :3], [0.0, ta * 36 * c * dt, -9.8 * dt], rtol=1e-5, atol=1e-9)
+
+
+def test_db_atts_torque_tumbles_by_radians_of_the_moment():
+    """dw = radians(Ta |u_r|^2 inv_inln cm) (rs.cl:243) and the tumble quaternion
+    from sincos(dw dt) (rs.cl:889-897): a pure x moment gives (sin a, 0, 0, cos a)."""
+    ta, dt, m, inl = 0.05, 0.01, 0.7, 2.0
+    st = _debris_step([0, 0, 0, 1], [0, 0, 0, 0], [m, 0, 0, 0], ta=ta, dt=dt, inln=(inl, 0.0, 0.0))
+    a = np.radians(ta * 36 * inl * m) * dt
+    np.testing.assert_allclose(st.tum[0], [np.sin(a), 0.0, 0.0, np.cos(a)], rtol=1e-5, atol=1e-7)
+
+
+def test_db_atts_respawn_resets_the_body():
+    """Leaving the box: xy from the LCG over the box, z = z0 + 2 m, unit random
+    orientation, zero velocity, identity tumble, rcs 0 (rs.cl:790-873)."""
+    wt, wk, les = _uniform_wind(0.0, 0.0, 0.0)
+    z = _const_tex2([0.0, 0.0, 0.0, 0.0])
+    ad = np.zeros(16, np.float32)
+    ad[8:10] = [8, 6]
+    sd = np.zeros(16, np.float32)
+    sd[8:11] = [0.0, 0.0, 0.0]
+    sd[12:15] = [100.0, 200.0, 50.0]
+    sd[11] = 0.01
+    st = orc.State([[50.0, 100.0, 60.0, 0.0]], [[1, 2, 3, 0]], [[0, 0, 0, 1]], [[0.1, 0, 0, 0.99]], [[1, 1, 1, 1]],
+                   np.array([[11, 12, 13, 14]], np.uint32))
+    orc.db_atts(st, 0, 1, wt, les, z[0], z[0], ad, z[0], z[0], ad, sd)
+    seeds, r = orc.rand4(np.array([11, 12, 13, 14], np.uint32))
+    np.testing.assert_array_equal(st.pos[0, :2], [np.float32(r[0] * 100.0), np.float32(r[1] * 200.0)])
+    assert st.pos[0, 2] == 2.0
+    np.testing.assert_array_equal(st.vel[0], 0.0)
+    np.testing.assert_array_equal(st.tum[0], [0, 0, 0, 1])
+    np.testing.assert_array_equal(st.rcs[0], 0.0)
+    assert abs(np.linalg.norm(st.ori[0].astype(np.float64)) - 1.0) < 1e-5
+    assert not np.array_equal(st.rnd[0], [11, 12, 13, 14])
+
+
+def test_debris_rcs_of_an_isotropic_scatterer_ignores_orientation():
+    """hh = vv = c, hv = 0 is invariant under the gamma rotation of the
+    scattering matrix (rs.cl:340-347): every pose returns (c, c) in H and V."""
+    g = np.random.default_rng(3)
+    re_t, rk = _const_tex2([0.7, 0.7, 0.0, 0.0])
+    im_t, ik = _const_tex2([-0.2, -0.2, 0.0, 0.0])
+    rd = np.zeros(16, np.float32)
+    rd[0:2] = 1.0
+    rd[4:6] = 1.0
+    sd = _box_sd(0.01)
+    for _ in range(50):
+        q = g.normal(size=4).astype(np.float32)
+        q /= np.linalg.norm(q)
+        p = g.uniform(-1e3, 1e3, 4).astype(np.float32)
+        p[2] = abs(p[2])
+        out = orc.debris_rcs(p, q, re_t, im_t, rd, sd)
+        np.testing.assert_allclose(out, [0.7, -0.2, 0.7, -0.2], rtol=2e-6, atol=1e-7)
