@@ -316,6 +316,10 @@ void RS_set_phase_precision(RSHandle *H, uint32_t phase_mode);
 /* addition: binary64 gate sums of the last pulse (R x 4, before the gain);
  * returns the number of gates. */
 int RS_get_pulse_f64(RSHandle *H, double *out);
+/* addition: radar moments (Z, V, ZDR, rho_hv) over the last `pulses` pulses,
+ * computed on the device from its IQ history (no counterpart in the reference) */
+void RS_set_moment_pulses(RSHandle *H, int pulses);
+int RS_get_moments(RSHandle *H, int pulses, float *out);
 /* addition: descriptor accessors (which: 0 sim, 1 les, 2 range weight,
  * 3 angular weight, 4 drop rcs, 16+t adm table t, 32+t rcs table t);
  * returns the number of floats written (4 or 16). */

@@ -162,6 +162,28 @@ int rsd_set_sort(rsd_engine *e, int interval);
 #define RSD_FOLD_WINDOW  3
 int rsd_set_fold(rsd_engine *e, int mode);
 
+/* Radar moments (no counterpart in the reference, which has no estimator; an
+ * unused zdr() exists at rs.c:376-378).  rsd_set_iq_ring keeps the last
+ * `pulses` pulses of `range_count` gates on the device (0 frees it); every
+ * rsd_make_pulse with that gate count appends to it.  rsd_moments computes, per
+ * gate over the most recent `pulses` pulses (2 <= pulses <= ring size), with
+ * H = Ih + j Qh and V = Iv + j Qv scaled by `gain`:
+ *   R0h = <|H_k|^2>, R0v = <|V_k|^2>, R1 = <conj(H_k) H_k+1>, C = <H_k conj(V_k)>
+ *   Z   = 10 log10(R0h) + 20 log10(r_g)   (dB; range-corrected power, radar constant 0)
+ *   V   = -lambda / (4 pi PRT) arg(R1)    (m/s, positive away from the radar)
+ *   ZDR = 10 log10(R0h / R0v)             (dB)
+ *   rho = |C| / sqrt(R0h R0v)
+ * with r_g = range_start + g range_delta; out[4 g + (0..3)] = (Z, V, ZDR, rho). */
+typedef struct {
+    double lambda;          /* wavelength (m) */
+    double prt;             /* pulse repetition time (s) */
+    double gain;            /* amplitude gain applied to the IQ (RS radar gain) */
+    double range_start;     /* range of gate 0 (m) */
+    double range_delta;     /* gate spacing (m) */
+} rsd_moment_params;
+int rsd_set_iq_ring(rsd_engine *e, int pulses, int range_count);
+int rsd_moments(rsd_engine *e, int pulses, const rsd_moment_params *params, float *out);
+
 /* Self-test: evaluate one of the device's binary32 functions on n inputs
  * (fn: 0 log1p(a), 1 log(a), 2 sin(a), 3 cos(a), 4 atan2(a, b), 5 acos(a),
  * 6 sqrt(a), 7 a / b, 8 sin(2 pi a), 9 cos(2 pi a), 10 the pulse path's fast acos(a)).

@@ -61,6 +61,7 @@ def lib():
             "orc_el_atts_f64": (None, [P, P, P, S, T, P, P, I]),
             "orc_les_desc": (None, [P, I, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, F, F, F, F, F, F, F]),
             "orc_math": (None, [I, P, P, P, S]),
+            "orc_moments": (None, [P, I, I, D, D, D, D, D, P]),
             "orc_wind_index": (F4, [F4, P, P]),
             "orc_sample3d": (F4, [T, F, F, F]),
             "orc_sample2d": (F4, [T, F, F]),
@@ -100,6 +101,16 @@ def math(fn, a, b=None):
     b = a if b is None else f32(b).ravel()
     out = np.empty_like(a)
     lib().orc_math(MATH[fn], _p(a), _p(b), _p(out), a.size)
+    return out
+
+
+def moments(iq, lam, prt, gain, range_start, range_delta):
+    """Pulse-pair moments of iq[M, R, 4] (Ih, Qh, Iv, Qv): (R, 4) = (Z dB, V m/s,
+    ZDR dB, rho_hv), binary64."""
+    q = np.ascontiguousarray(iq, np.float64)
+    M, R = q.shape[0], q.shape[1]
+    out = np.zeros((R, 4), np.float64)
+    lib().orc_moments(_p(q), M, R, lam, prt, gain, range_start, range_delta, _p(out))
     return out
 
 

@@ -1379,6 +1379,26 @@ int RS_get_pulse_f64(RSHandle *H, double *out) {
     return (int)R;
 }
 
+/* Moment estimator over the device IQ history (rsd_set_iq_ring / rsd_moments):
+ * RS_set_moment_pulses keeps the last `pulses` pulses of the current gate count;
+ * RS_get_moments fills out[4 g + (Z dB, V m/s, ZDR dB, rho_hv)] and returns R. */
+void RS_set_moment_pulses(RSHandle *H, int pulses) {
+    if (!has_engine(H)) return;
+    engine_ok(H, rsd_set_iq_ring(H->workers[0].engine, pulses, (int)H->params.range_count), "IQ ring");
+}
+
+int RS_get_moments(RSHandle *H, int pulses, float *out) {
+    if (!has_engine(H)) return 0;
+    rsd_moment_params mp;
+    mp.lambda = H->params.lambda;
+    mp.prt = H->params.prt;
+    mp.gain = RS_radar_gain(H);
+    mp.range_start = H->params.range_start;
+    mp.range_delta = H->params.range_delta;
+    if (!engine_ok(H, rsd_moments(H->workers[0].engine, pulses, &mp, out), "moments")) return 0;
+    return (int)H->params.range_count;
+}
+
 int RS_get_descriptor(RSHandle *H, int which, float *out) {
     const RSWorker *C = &H->workers[0];
     if (which == 0) { memcpy(out, H->sim_desc.s, 64); return 16; }

@@ -868,7 +868,7 @@ __global__ __launch_bounds__(256) void k_debris(DebrisK a) {
 
 // Sum the per-block partial rows of one pulse: one block per output value.
 // Rows hold [component][gate]; the pulse buffer is [gate][component].
-__global__ __launch_bounds__(256) void k_finalize(const double *partials, int rows, int n4r, double *out) {
+__global__ __launch_bounds__(256) void k_finalize(const double *partials, int rows, int n4r, double *out, double *out2) {
     __shared__ double red[256];
     const int j = blockIdx.x;
     const int R = n4r / 4;
@@ -880,7 +880,60 @@ __global__ __launch_bounds__(256) void k_finalize(const double *partials, int ro
         if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[4 * (j % R) + j / R] = red[0];
+    if (threadIdx.x == 0) {
+        out[4 * (j % R) + j / R] = red[0];
+        if (out2) out2[4 * (j % R) + j / R] = red[0];
+    }
+}
+
+// Pulse-pair moments of one gate over the last M pulses of the IQ ring
+// (slots oldest .. newest): lag-0 powers, lag-1 autocorrelation of H and the
+// H-V cross-correlation, all in binary64.
+struct MomK {
+    const double *ring;     // [ring_m][4 R], gate-major (Ih, Qh, Iv, Qv)
+    int ring_m, first, m, R;
+    double gain2;           // radar gain squared (the IQ are stored before the gain)
+    double vfac;            // -lambda / (4 pi PRT)
+    double range_start, range_delta;
+    float4 *out;            // (Z dB, V m/s, ZDR dB, rho_hv) per gate
+};
+
+__global__ __launch_bounds__(256) void k_moments(MomK a) {
+    __shared__ double red[6][256];
+    const int g = blockIdx.x;
+    double p_h = 0.0, p_v = 0.0, r1_re = 0.0, r1_im = 0.0, c_re = 0.0, c_im = 0.0;
+    for (int k = threadIdx.x; k < a.m; k += blockDim.x) {
+        const double *q = a.ring + (size_t)((a.first + k) % a.ring_m) * 4 * a.R + 4 * g;
+        const double hi = q[0], hq = q[1], vi = q[2], vq = q[3];
+        p_h += hi * hi + hq * hq;
+        p_v += vi * vi + vq * vq;
+        c_re += hi * vi + hq * vq;      // H conj(V)
+        c_im += hq * vi - hi * vq;
+        if (k + 1 < a.m) {
+            const double *n = a.ring + (size_t)((a.first + k + 1) % a.ring_m) * 4 * a.R + 4 * g;
+            r1_re += hi * n[0] + hq * n[1];  // conj(H_k) H_k+1
+            r1_im += hi * n[1] - hq * n[0];
+        }
+    }
+    red[0][threadIdx.x] = p_h; red[1][threadIdx.x] = p_v; red[2][threadIdx.x] = r1_re;
+    red[3][threadIdx.x] = r1_im; red[4][threadIdx.x] = c_re; red[5][threadIdx.x] = c_im;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int c = 0; c < 6; c++) red[c][threadIdx.x] += red[c][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double m = (double)a.m;
+        const double r0h = red[0][0] / m * a.gain2, r0v = red[1][0] / m * a.gain2;
+        const double cxr = red[4][0] / m * a.gain2, cxi = red[5][0] / m * a.gain2;
+        const double rg = a.range_start + (double)g * a.range_delta;
+        const double z = 10.0 * log10(r0h) + 20.0 * log10(fabs(rg));
+        const double v = a.vfac * atan2(red[3][0], red[2][0]);
+        const double zdr = 10.0 * log10(r0h / r0v);
+        const double rho = sqrt(cxr * cxr + cxi * cxi) / sqrt(r0h * r0v);
+        a.out[g] = make_float4((float)z, (float)v, (float)zdr, (float)rho);
+    }
 }
 
 // Bandwidth probe: copy of n float4 (the reference's io kernel, rs.cl:374-379).
@@ -1090,6 +1143,10 @@ struct rsd_engine {
     size_t sort_tmp_bytes;
     int sort_interval;      // pulses between sorts (0 = never)
     int fold;               // RSD_FOLD_* strategy of the gate sums
+    // IQ history for the moment estimator: ring of ring_m pulses of 4 R doubles
+    double *ring;
+    int ring_m, ring_R, ring_next, ring_count;
+    float4 *mom_d;
     int since_sort;
     int permuted;           // stored rain order differs from the reference order
     float4 *wind[2];
@@ -1245,6 +1302,8 @@ extern "C" void rsd_close(rsd_engine *e) {
     dfree(e->drop); dfree(e->rangew); dfree(e->angw); dfree(e->gate_r); dfree(e->partials);
     dfree(e->pulse_d);
     dfree(e->logtab);
+    dfree(e->ring);
+    dfree(e->mom_d);
     if (e->pulse_h) (void)hipHostFree(e->pulse_h);
     if (e->prof_ev) {
         for (int i = 0; i < 2 * RSD_PROF_CAP; i++) (void)hipEventDestroy(e->prof_ev[i]);
@@ -1537,7 +1596,14 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p) {
     }
     if (any_debris) HIPCHK(e, hipStreamWaitEvent(e->stream, e->ev_join, 0));
     if (p) {
-        hipLaunchKernelGGL(k_finalize, dim3(4 * R), dim3(256), 0, e->stream, (const double *)e->partials, rows, 4 * R, e->pulse_d);
+        double *slot = NULL;
+        if (e->ring && e->ring_R == R) {
+            slot = e->ring + (size_t)e->ring_next * 4 * R;
+            e->ring_next = (e->ring_next + 1) % e->ring_m;
+            if (e->ring_count < e->ring_m) e->ring_count++;
+        }
+        hipLaunchKernelGGL(k_finalize, dim3(4 * R), dim3(256), 0, e->stream, (const double *)e->partials, rows, 4 * R,
+                           e->pulse_d, slot);
         HIPCHK(e, hipGetLastError());
         e->last_pulse = *p;
         e->have_pulse = 1;
@@ -1642,6 +1708,53 @@ static int unsort_rain(rsd_engine *e) {
     HIPCHK(e, hipGetLastError());
     swap_streams(e);
     e->permuted = 0;
+    return RSD_OK;
+}
+
+static int flush_pending(rsd_engine *e);
+
+extern "C" int rsd_set_iq_ring(rsd_engine *e, int pulses, int range_count) {
+    if (!e) return RSD_ERR_ARG;
+    if (pulses < 0 || range_count < 0 || range_count > RSD_MAX_GATES) return fail(e, RSD_ERR_ARG, "bad IQ ring size");
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    dfree(e->ring);
+    dfree(e->mom_d);
+    e->ring = NULL;
+    e->mom_d = NULL;
+    e->ring_m = e->ring_R = e->ring_next = e->ring_count = 0;
+    if (pulses == 0) return RSD_OK;
+    if (pulses < 2 || range_count < 1) return fail(e, RSD_ERR_ARG, "IQ ring needs >= 2 pulses and >= 1 gate");
+    HIPCHK(e, hipMalloc((void **)&e->ring, (size_t)pulses * 4 * range_count * sizeof(double)));
+    HIPCHK(e, hipMalloc((void **)&e->mom_d, (size_t)range_count * sizeof(float4)));
+    e->ring_m = pulses;
+    e->ring_R = range_count;
+    return RSD_OK;
+}
+
+extern "C" int rsd_moments(rsd_engine *e, int pulses, const rsd_moment_params *mp, float *out) {
+    if (!e || !mp || !out) return RSD_ERR_ARG;
+    if (!e->ring) return fail(e, RSD_ERR_STATE, "no IQ ring (rsd_set_iq_ring)");
+    int st = flush_pending(e);
+    if (st) return st;
+    if (pulses < 2 || pulses > e->ring_m) return fail(e, RSD_ERR_ARG, "moments over %d pulses: ring holds %d", pulses, e->ring_m);
+    if (e->ring_count < pulses) return fail(e, RSD_ERR_STATE, "only %d pulses recorded", e->ring_count);
+    HIPCHK(e, hipSetDevice(e->device));
+    MomK a;
+    a.ring = e->ring;
+    a.ring_m = e->ring_m;
+    a.first = (e->ring_next - pulses + e->ring_m) % e->ring_m;
+    a.m = pulses;
+    a.R = e->ring_R;
+    a.gain2 = mp->gain * mp->gain;
+    a.vfac = -mp->lambda / (4.0 * M_PI * mp->prt);
+    a.range_start = mp->range_start;
+    a.range_delta = mp->range_delta;
+    a.out = e->mom_d;
+    hipLaunchKernelGGL(k_moments, dim3(e->ring_R), dim3(256), 0, e->stream, a);
+    HIPCHK(e, hipGetLastError());
+    HIPCHK(e, hipMemcpyAsync(out, e->mom_d, (size_t)e->ring_R * sizeof(float4), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
     return RSD_OK;
 }
 
