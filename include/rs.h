@@ -180,6 +180,7 @@ struct _rs_handle {
     POSHandle           P;
     /* additions */
     size_t              population_override;    /* RS_set_population */
+    float               range_cap;              /* RS_set_range_cap (reference: 10 km) */
     uint32_t            phase_mode;             /* RSD_PHASE_F64 (default) or RSD_PHASE_F32 */
     double              *pulse_acc;             /* binary64 gate sums of the last pulse, R x 4 */
     int                 last_error;             /* RSD_* code of the last failed engine call */
@@ -311,6 +312,8 @@ RSHandle *RS_init_host_only(const char verb);
 /* addition: exact background population, bypassing the rounding of
  * RS_revise_population (rs.c:2172-2178) to device-preferred multiples. */
 void RS_set_population(RSHandle *H, size_t rain_count);
+/* addition: maximum padded range of the scan extent (default 10 km, rs.c:1561) */
+void RS_set_range_cap(RSHandle *H, float r_max);
 /* addition: phase precision of the pulse (RSD_PHASE_F64 default). */
 void RS_set_phase_precision(RSHandle *H, uint32_t phase_mode);
 /* addition: binary64 gate sums of the last pulse (R x 4, before the gain);

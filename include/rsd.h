@@ -162,6 +162,16 @@ int rsd_set_sort(rsd_engine *e, int interval);
 #define RSD_FOLD_WINDOW  3
 int rsd_set_fold(rsd_engine *e, int mode);
 
+/* Position precision (no counterpart in the reference, which keeps binary32
+ * positions): hilo = 1 carries every position as a pair of binary32 values
+ * hi + lo (hi in the pos stream, lo in a stream of its own, reset to 0 on
+ * upload and on respawn); the advance adds v dt to the pair with TwoProduct /
+ * TwoSum and the pulse forms the range for the phase in binary64 from hi + lo.
+ * The physics (wind lookup, drag, box test) reads hi.  Downloads return hi in
+ * pos; rsd_download_pos_lo returns lo in the reference order. */
+int rsd_set_position_precision(rsd_engine *e, int hilo);
+int rsd_download_pos_lo(rsd_engine *e, float *lo);
+
 /* Radar moments (no counterpart in the reference, which has no estimator; an
  * unused zdr() exists at rs.c:376-378).  rsd_set_iq_ring keeps the last
  * `pulses` pulses of `range_count` gates on the device (0 frees it); every

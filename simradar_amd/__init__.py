@@ -71,7 +71,7 @@ EXPORTED_RS = [
     "RS_set_rcs_data_to_RCS_table", "RS_set_rcs_data_to_unity", "RS_clear_rcs_data",
     "RS_set_rcs_ellipsoid_table", "RS_revise_population", "RS_update_origins_offsets", "RS_merge_pulse_tmp",
     "RS_init_host_only", "RS_set_population", "RS_set_phase_precision", "RS_get_pulse_f64",
-    "RS_set_moment_pulses", "RS_get_moments",
+    "RS_set_moment_pulses", "RS_get_moments", "RS_set_range_cap",
     "RS_get_descriptor", "RS_radar_gain", "RS_engine", "RS_get_info", "RS_get_host_stream", "RS_get_params",
     "RS_get_dsd", "RS_suggest_scan_domain_values", "RS_set_scan_box_values", "RS_get_les_frame", "RS_get_debris_table",
 ]
@@ -81,6 +81,7 @@ EXPORTED_RSD = [
     "rsd_set_range_weight", "rsd_set_angular_weight", "rsd_advance", "rsd_make_pulse", "rsd_read_pulse",
     "rsd_sync", "rsd_timer_start", "rsd_timer_stop", "rsd_set_launch", "rsd_math_probe", "rsd_profile",
     "rsd_profile_read", "rsd_io_test", "rsd_set_sort", "rsd_set_fold", "rsd_set_iq_ring", "rsd_moments",
+    "rsd_set_position_precision", "rsd_download_pos_lo",
 ]
 
 
@@ -198,6 +199,7 @@ def _declare(lib):
         "RS_set_phase_precision": (None, [P, ctypes.c_uint32]),
         "RS_get_pulse_f64": (I, [P, P]),
         "RS_set_moment_pulses": (None, [P, I]),
+        "RS_set_range_cap": (None, [P, F]),
         "RS_get_moments": (I, [P, I, P]),
         "RS_get_descriptor": (I, [P, I, P]),
         "RS_radar_gain": (F, [P]),
@@ -235,6 +237,8 @@ def _declare(lib):
         "rsd_set_fold": (I, [P, I]),
         "rsd_set_iq_ring": (I, [P, I, I]),
         "rsd_moments": (I, [P, I, P, P]),
+        "rsd_set_position_precision": (I, [P, I]),
+        "rsd_download_pos_lo": (I, [P, P]),
         "rsd_math_probe": (I, [I, I, P, P, P, S]),
         "rsd_profile": (I, [P, I]),
         "rsd_io_test": (I, [P, S, I, ctypes.POINTER(ctypes.c_double)]),
@@ -448,6 +452,21 @@ class Radar:
         e = self.engine()
         if self.lib.rsd_set_launch(e, block, max_blocks):
             raise RuntimeError(self.lib.rsd_last_error(e).decode())
+
+    def set_position_precision(self, hilo):
+        """1: carry positions as binary32 hi/lo pairs (binary64 range for the phase)."""
+        e = self.engine()
+        if self.lib.rsd_set_position_precision(e, int(hilo)):
+            raise RuntimeError(self.lib.rsd_last_error(e).decode())
+
+    def pos_lo(self):
+        """(N, 4) low parts of the hi/lo positions, reference order."""
+        e = self.engine()
+        n = int(self.info().num_scats)
+        out = np.zeros((n, 4), np.float32)
+        if self.lib.rsd_download_pos_lo(e, _ptr(out)):
+            raise RuntimeError(self.lib.rsd_last_error(e).decode())
+        return out
 
     def set_moment_pulses(self, pulses):
         """Keep the last `pulses` pulses on the device for the moment estimator."""

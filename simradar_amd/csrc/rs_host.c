@@ -83,6 +83,7 @@ static RSHandle *alloc_handle(RSMethod method) {
     H->method = method;
     H->random_seed = 19760520;
     H->phase_mode = RSD_PHASE_F64;
+    H->range_cap = 10.0e3f;
     for (int i = 0; i < RS_MAX_GPU_DEVICE; i++) H->workers[i].name = (char)i;
     return H;
 }
@@ -392,7 +393,7 @@ void RS_set_scan_extent(RSHandle *H,
     const bool is_full_sweep = fabs(azimuth_start - 0.0f) < 0.01f && fabs(azimuth_end - 360.0f) < 0.01f;
     const RSParams *P = &H->params;
     const RSfloat r_lo = floor(MAX(P->range_delta, P->range_start - P->domain_pad_factor * P->dr) / P->range_delta) * P->range_delta;
-    const RSfloat r_hi = ceil(MIN(10.0e3f, P->range_end + P->domain_pad_factor * P->dr) / P->range_delta) * P->range_delta;
+    const RSfloat r_hi = ceil(MIN(H->range_cap, P->range_end + P->domain_pad_factor * P->dr) / P->range_delta) * P->range_delta;
     const RSfloat az_lo = is_full_sweep ? 0.0f : floor(P->azimuth_start_deg - P->domain_pad_factor * P->antenna_bw_deg) / P->antenna_bw_deg * P->antenna_bw_deg;
     const RSfloat az_hi = is_full_sweep ? 360.0f : ceil(P->azimuth_end_deg + P->domain_pad_factor * P->antenna_bw_deg) / P->antenna_bw_deg * P->antenna_bw_deg;
     const RSfloat el_lo = floor(MAX(0.0f, P->elevation_start_deg - P->domain_pad_factor * P->antenna_bw_deg) / P->antenna_bw_deg) * P->antenna_bw_deg;
@@ -499,6 +500,12 @@ void RS_set_debris_count(RSHandle *H, const int debris_id, const size_t count) {
         if (H->counts[i] > 0) H->num_types++;
     }
     if (H->status & RSStatusPopulationDefined) RS_revise_population(H);
+}
+
+/* addition: the reference caps the padded scan extent at 10 km (rs.c:1561);
+ * domains beyond it (a box at 10 km range) lift the cap before RS_set_scan_box. */
+void RS_set_range_cap(RSHandle *H, float r_max) {
+    H->range_cap = r_max > 0.0f ? r_max : 10.0e3f;
 }
 
 void RS_set_population(RSHandle *H, size_t rain_count) {

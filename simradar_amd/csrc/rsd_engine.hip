@@ -251,14 +251,29 @@ __device__ __forceinline__ float beam_angle(float4 p, const F16 &sd) {
 }
 
 // Signal x angular weight of one scatterer (rs.cl:1050-1110, 1175-1180).
-__device__ __forceinline__ bool scat_signal(float4 p, float4 rcs, const PulseK &k, float4 &sa, float &r) {
-    r = rf_len3(p.x, p.y, p.z);
+// With HILO the position is the pair hi + lo (binary32 each) and the range that
+// enters the phase is formed in binary64 from it.
+template <bool HILO = false>
+__device__ __forceinline__ bool scat_signal(float4 p, float4 rcs, const PulseK &k, float4 &sa, float &r,
+                                            float4 lo = make_float4(0.0f, 0.0f, 0.0f, 0.0f)) {
+    double r64 = 0.0;
+    if (HILO) {
+        const double x = (double)p.x + (double)lo.x, y = (double)p.y + (double)lo.y, z = (double)p.z + (double)lo.z;
+        r64 = sqrt((x * x + y * y) + z * z);
+        r = (float)r64;
+    } else {
+        r = rf_len3(p.x, p.y, p.z);
+    }
     const float inv_r = rf_rcp_fast(r);
     float wa = table_lerp(beam_angle(p, k.sd), k.angw, k.angw_desc);
     if (wa == 0.0f) return false;
     float atten = inv_r * inv_r;
     float ss, cc;
-    if (k.phase_mode == RSD_PHASE_F64) {
+    if (HILO) {
+        double turns = r64 * k.two_over_lambda;
+        float t = (float)(turns - floor(turns));
+        rf_sincos_turns(t, &ss, &cc);
+    } else if (k.phase_mode == RSD_PHASE_F64) {
         double turns = (double)r * k.two_over_lambda;
         float t = (float)(turns - floor(turns));
         rf_sincos_turns(t, &ss, &cc);
@@ -550,6 +565,7 @@ __device__ __forceinline__ void acc_flush(const double *acc, const PulseK &k) {
 // ---------------------------------------------------------------------------
 struct RainK {
     float4 *pos, *vel, *rcs;
+    float4 *plo;            // low parts of the positions (hi/lo mode), else NULL
     uint4 *rnd;
     unsigned long long n;
     Tex3 wind;
@@ -562,15 +578,38 @@ struct RainK {
     PulseK pk;
 };
 
-// Returns true when the scatterer respawned.
-__device__ __forceinline__ bool rain_advance(const RainK &a, size_t i, float4 &pos, float4 &vel, const float *logtab) {
+// (hi, lo) += v dt carried as a pair of binary32 values: the product's rounding
+// error from fma (TwoProduct), the sum's from TwoSum, then renormalised so that
+// |lo| <= ulp(hi) / 2.
+__device__ __forceinline__ void hilo_step(float &hi, float &lo, float v, float dt) {
+    const float d = v * dt;
+    const float de = __builtin_fmaf(v, dt, -d);
+    const float s = hi + d;
+    const float bp = s - hi;
+    const float err = (hi - (s - bp)) + (d - bp);
+    const float l = lo + (err + de);
+    hi = s + l;
+    lo = l - (hi - s);
+}
+
+// Returns true when the scatterer respawned.  HILO: positions are hi + lo pairs
+// (lo in its own stream); the physics reads hi.
+template <bool HILO = false>
+__device__ __forceinline__ bool rain_advance(const RainK &a, size_t i, float4 &pos, float4 &vel, const float *logtab,
+                                             float4 &lo) {
     const F16 &sd = a.sd;
     const float dt = sd.s[11];
     if (a.dragged) {
         const uint32_t concept = __float_as_uint(sd.s[5]);
-        pos.x = rf_fma(vel.x, dt, pos.x);
-        pos.y = rf_fma(vel.y, dt, pos.y);
-        pos.z = rf_fma(vel.z, dt, pos.z);
+        if (HILO) {
+            hilo_step(pos.x, lo.x, vel.x, dt);
+            hilo_step(pos.y, lo.y, vel.y, dt);
+            hilo_step(pos.z, lo.z, vel.z, dt);
+        } else {
+            pos.x = rf_fma(vel.x, dt, pos.x);
+            pos.y = rf_fma(vel.y, dt, pos.y);
+            pos.z = rf_fma(vel.z, dt, pos.z);
+        }
         pos.w = rf_fma(vel.w, 0.0f, pos.w);
         const float aom = rf_div(0.003006012f, pos.w);
         if (is_outside(pos, sd)) {
@@ -581,6 +620,7 @@ __device__ __forceinline__ bool rain_advance(const RainK &a, size_t i, float4 &p
             pos.y = rf_fma(r.y, sd.s[13], sd.s[9]);
             pos.z = rf_fma(r.z, sd.s[14], sd.s[10]);
             vel = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            lo = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             return true;
         }
         float4 c = wind_index(pos, a.les, sd, logtab);
@@ -613,9 +653,15 @@ __device__ __forceinline__ bool rain_advance(const RainK &a, size_t i, float4 &p
         }
         return false;
     }
-    pos.x = pos.x + vel.x * dt;
-    pos.y = pos.y + vel.y * dt;
-    pos.z = pos.z + vel.z * dt;
+    if (HILO) {
+        hilo_step(pos.x, lo.x, vel.x, dt);
+        hilo_step(pos.y, lo.y, vel.y, dt);
+        hilo_step(pos.z, lo.z, vel.z, dt);
+    } else {
+        pos.x = pos.x + vel.x * dt;
+        pos.y = pos.y + vel.y * dt;
+        pos.z = pos.z + vel.z * dt;
+    }
     pos.w = pos.w + vel.w * 0.0f;
     if (is_outside(pos, sd)) {
         uint4 seed = a.rnd[i];
@@ -625,6 +671,7 @@ __device__ __forceinline__ bool rain_advance(const RainK &a, size_t i, float4 &p
         pos.y = r.y * sd.s[13] + sd.s[9];
         pos.z = r.z * sd.s[14] + sd.s[10];
         vel = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        lo = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         return true;
     }
     float4 c = wind_index(pos, a.les, sd, logtab);
@@ -636,7 +683,7 @@ __device__ __forceinline__ void logtab_load(float *dst, const float *src) {
     for (int j = threadIdx.x; j < 3 * RF_LOG_TABLE; j += blockDim.x) dst[j] = src[j];
 }
 
-template <bool ADV, bool PULSE>
+template <bool ADV, bool PULSE, bool HILO>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_rain(RainK a) {
     extern __shared__ double acc[];
     __shared__ float logtab[3 * RF_LOG_TABLE];
@@ -661,10 +708,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         p.w = __builtin_fabsf(p.w);
         float4 rcs;
         bool have_rcs = false;
+        float4 plo = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (HILO && live) plo = a.plo[i];
         if (ADV && live) {
             float4 v = a.vel[i];
             const float4 p_in = p;
-            bool respawned = rain_advance(a, i, p, v, logtab);
+            bool respawned = rain_advance<HILO>(a, i, p, v, logtab, plo);
+            if (HILO) a.plo[i] = plo;
             if (respawned) {
                 if (!stale) {
                     rcs = ellipsoid_rcs(p_in, a.drop, a.drop_desc);
@@ -684,7 +734,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             bool on = false;
             if (live) {
                 if (!have_rcs) rcs = stale ? a.rcs[i] : ellipsoid_rcs(p, a.drop, a.drop_desc);
-                on = scat_signal(p, rcs, a.pk, sa, r);
+                on = scat_signal<HILO>(p, rcs, a.pk, sa, r, plo);
             }
             if (use_win) win_fold(win, on, r, sa, a.pk, acc, wl);
             else fold_gates(on, r, sa, a.pk, acc);
@@ -1006,10 +1056,10 @@ __global__ void k_sort_keys(SortK a) {
 }
 
 struct PermK {
-    const float4 *pos, *vel, *rcs;
+    const float4 *pos, *vel, *rcs, *plo;
     const uint4 *rnd;
     const uint32_t *id, *perm;
-    float4 *pos2, *vel2, *rcs2;
+    float4 *pos2, *vel2, *rcs2, *plo2;
     uint4 *rnd2;
     uint32_t *id2;
     unsigned long long n, n_rain;
@@ -1028,6 +1078,7 @@ __global__ void k_permute(PermK a) {
     a.vel2[j] = a.vel[s];
     if (!rain || __builtin_signbit(p.w)) a.rcs2[j] = a.rcs[s];
     a.rnd2[j] = a.rnd[s];
+    if (a.plo) a.plo2[j] = a.plo[s];
     if (rain) a.id2[j] = a.id[s];
 }
 
@@ -1042,7 +1093,17 @@ __global__ void k_unpermute(PermK a) {
     a.vel2[o] = a.vel[j];
     if (!rain || __builtin_signbit(p.w)) a.rcs2[o] = a.rcs[j];
     a.rnd2[o] = a.rnd[j];
+    if (a.plo) a.plo2[o] = a.plo[j];
     if (rain) a.id2[o] = (uint32_t)o;
+}
+
+// Low position parts in the reference order (hi/lo mode downloads).
+__global__ void k_lo_view(const float4 *plo, const uint32_t *id, unsigned long long n_rain, unsigned long long n,
+                          float4 *out) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t o = (id && i < n_rain) ? id[i] : i;
+    out[o] = plo[i];
 }
 
 __global__ void k_iota(uint32_t *id, unsigned long long n) {
@@ -1139,6 +1200,8 @@ struct rsd_engine {
     uint4 *rnd2;
     uint32_t *id, *id2, *skey, *skey2, *sperm, *sperm2;
     int sort_bits;          // key bits of the current sort
+    float4 *plo, *plo2;     // hi/lo positions: low parts (and ping-pong copy)
+    int pos_hilo;           // 1: positions carried as hi/lo pairs
     void *sort_tmp;
     size_t sort_tmp_bytes;
     int sort_interval;      // pulses between sorts (0 = never)
@@ -1280,6 +1343,8 @@ static void free_state(rsd_engine *e) {
     dfree(e->pos2); dfree(e->vel2); dfree(e->rcs2); dfree(e->rnd2);
     dfree(e->id); dfree(e->id2); dfree(e->skey); dfree(e->skey2); dfree(e->sperm); dfree(e->sperm2);
     dfree(e->sort_tmp);
+    dfree(e->plo); dfree(e->plo2);
+    e->plo = e->plo2 = NULL;
     e->pos = e->vel = e->ori = e->tum = e->rcs = e->aux = e->view = NULL;
     e->pos2 = e->vel2 = e->rcs2 = NULL;
     e->rnd = e->rnd2 = NULL;
@@ -1358,6 +1423,10 @@ extern "C" int rsd_alloc_state(rsd_engine *e, size_t num_scats, const size_t *or
     HIPCHK(e, hipMalloc((void **)&e->rnd, num_scats * sizeof(uint4)));
     HIPCHK(e, hipMemsetAsync(e->rcs, 0, b, e->stream));
     HIPCHK(e, hipMemsetAsync(e->aux, 0, b, e->stream));
+    if (e->pos_hilo) {
+        HIPCHK(e, hipMalloc((void **)&e->plo, b));
+        HIPCHK(e, hipMemsetAsync(e->plo, 0, b, e->stream));
+    }
     e->have_pending = 0;
     e->have_pulse = 0;
     return RSD_OK;
@@ -1584,9 +1653,16 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p) {
         a.pk.row0 = 0;
         const bool timed = e->prof_on && e->prof_n < RSD_PROF_CAP;
         if (timed) HIPCHK(e, hipEventRecord(e->prof_ev[2 * e->prof_n], e->stream));
-        if (s && p) hipLaunchKernelGGL((k_rain<true, true>), dim3(rain_blocks), dim3(e->block), lds, e->stream, a);
-        else if (s) hipLaunchKernelGGL((k_rain<true, false>), dim3(rain_blocks), dim3(e->block), 0, e->stream, a);
-        else hipLaunchKernelGGL((k_rain<false, true>), dim3(rain_blocks), dim3(e->block), lds, e->stream, a);
+        a.plo = e->plo;
+        if (e->plo) {
+            if (s && p) hipLaunchKernelGGL((k_rain<true, true, true>), dim3(rain_blocks), dim3(e->block), lds, e->stream, a);
+            else if (s) hipLaunchKernelGGL((k_rain<true, false, true>), dim3(rain_blocks), dim3(e->block), 0, e->stream, a);
+            else hipLaunchKernelGGL((k_rain<false, true, true>), dim3(rain_blocks), dim3(e->block), lds, e->stream, a);
+        } else {
+            if (s && p) hipLaunchKernelGGL((k_rain<true, true, false>), dim3(rain_blocks), dim3(e->block), lds, e->stream, a);
+            else if (s) hipLaunchKernelGGL((k_rain<true, false, false>), dim3(rain_blocks), dim3(e->block), 0, e->stream, a);
+            else hipLaunchKernelGGL((k_rain<false, true, false>), dim3(rain_blocks), dim3(e->block), lds, e->stream, a);
+        }
         HIPCHK(e, hipGetLastError());
         if (timed) {
             HIPCHK(e, hipEventRecord(e->prof_ev[2 * e->prof_n + 1], e->stream));
@@ -1618,6 +1694,7 @@ static int ensure_sort_buffers(rsd_engine *e) {
     HIPCHK(e, hipMalloc((void **)&e->vel2, b));
     HIPCHK(e, hipMalloc((void **)&e->rcs2, b));
     HIPCHK(e, hipMalloc((void **)&e->rnd2, e->n * sizeof(uint4)));
+    if (e->plo) HIPCHK(e, hipMalloc((void **)&e->plo2, b));
     HIPCHK(e, hipMalloc((void **)&e->id, bi));
     HIPCHK(e, hipMalloc((void **)&e->id2, bi));
     HIPCHK(e, hipMalloc((void **)&e->skey, bi));
@@ -1638,13 +1715,14 @@ static void swap_streams(rsd_engine *e) {
     t = e->rcs; e->rcs = e->rcs2; e->rcs2 = t;
     uint4 *r = e->rnd; e->rnd = e->rnd2; e->rnd2 = r;
     uint32_t *u = e->id; e->id = e->id2; e->id2 = u;
+    t = e->plo; e->plo = e->plo2; e->plo2 = t;
 }
 
 static PermK perm_args(const rsd_engine *e) {
     PermK a;
-    a.pos = e->pos; a.vel = e->vel; a.rcs = e->rcs; a.rnd = e->rnd;
+    a.pos = e->pos; a.vel = e->vel; a.rcs = e->rcs; a.rnd = e->rnd; a.plo = e->plo;
     a.id = e->id; a.perm = e->sperm2;
-    a.pos2 = e->pos2; a.vel2 = e->vel2; a.rcs2 = e->rcs2; a.rnd2 = e->rnd2;
+    a.pos2 = e->pos2; a.vel2 = e->vel2; a.rcs2 = e->rcs2; a.rnd2 = e->rnd2; a.plo2 = e->plo2;
     a.id2 = e->id2;
     a.n = e->n;
     a.n_rain = e->n_rain;
@@ -1712,6 +1790,45 @@ static int unsort_rain(rsd_engine *e) {
 }
 
 static int flush_pending(rsd_engine *e);
+
+extern "C" int rsd_set_position_precision(rsd_engine *e, int hilo) {
+    if (!e || (hilo != 0 && hilo != 1)) return fail(e, RSD_ERR_ARG, "position precision %d unknown", hilo);
+    HIPCHK(e, hipSetDevice(e->device));
+    int st = flush_pending(e);
+    if (st) return st;
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    e->pos_hilo = hilo;
+    if (!e->pos) return RSD_OK;
+    const size_t b = e->n * sizeof(float4);
+    if (hilo && !e->plo) {
+        HIPCHK(e, hipMalloc((void **)&e->plo, b));
+        HIPCHK(e, hipMemsetAsync(e->plo, 0, b, e->stream));
+        if (e->pos2 && !e->plo2) HIPCHK(e, hipMalloc((void **)&e->plo2, b));
+    } else if (!hilo && e->plo) {
+        dfree(e->plo); dfree(e->plo2);
+        e->plo = e->plo2 = NULL;
+    }
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return RSD_OK;
+}
+
+extern "C" int rsd_download_pos_lo(rsd_engine *e, float *lo) {
+    if (!e || !lo) return RSD_ERR_ARG;
+    if (!e->plo) return fail(e, RSD_ERR_STATE, "positions are not hi/lo");
+    int st = flush_pending(e);
+    if (st) return st;
+    HIPCHK(e, hipSetDevice(e->device));
+    const size_t b = e->n * sizeof(float4);
+    if (!e->view) HIPCHK(e, hipMalloc((void **)&e->view, 6 * b));
+    unsigned nb = (unsigned)((e->n + 255) / 256);
+    hipLaunchKernelGGL(k_lo_view, dim3(nb), dim3(256), 0, e->stream, (const float4 *)e->plo,
+                       (const uint32_t *)(e->permuted ? e->id : NULL), (unsigned long long)e->n_rain,
+                       (unsigned long long)e->n, e->view);
+    HIPCHK(e, hipGetLastError());
+    HIPCHK(e, hipMemcpyAsync(lo, e->view, b, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return RSD_OK;
+}
 
 extern "C" int rsd_set_iq_ring(rsd_engine *e, int pulses, int range_count) {
     if (!e) return RSD_ERR_ARG;
@@ -1788,6 +1905,7 @@ extern "C" int rsd_upload_state(rsd_engine *e, const float *pos, const float *ve
     if (st) return st;
     const size_t b = e->n * sizeof(float4);
     if (pos) HIPCHK(e, hipMemcpyAsync(e->pos, pos, b, hipMemcpyHostToDevice, e->stream));
+    if (pos && e->plo) HIPCHK(e, hipMemsetAsync(e->plo, 0, b, e->stream));
     if (vel) HIPCHK(e, hipMemcpyAsync(e->vel, vel, b, hipMemcpyHostToDevice, e->stream));
     if (ori) HIPCHK(e, hipMemcpyAsync(e->ori, ori, b, hipMemcpyHostToDevice, e->stream));
     if (tum) HIPCHK(e, hipMemcpyAsync(e->tum, tum, b, hipMemcpyHostToDevice, e->stream));

@@ -28,7 +28,7 @@ def mp_dsd(rain_rate_mm_h=10.0, d_mm=None, n0=8000.0):
 
 class Config:
     def __init__(self, n_rain=131072, debris=(), concept=DBU, gate=15.0, scan="P:3.0,-12:12:0.01", seed=SEED,
-                 prt=1e-3, phase=PHASE_F64, rain_rate=10.0, box=None, name="custom"):
+                 prt=1e-3, phase=PHASE_F64, rain_rate=10.0, box=None, name="custom", range_cap=None):
         self.n_rain = n_rain
         self.debris = list(debris)
         self.concept = concept
@@ -39,6 +39,7 @@ class Config:
         self.phase = phase
         self.rain_rate = rain_rate
         self.box = box          # None: RS_suggest_scan_domain; else 8 floats (origin r,a,e,w, size r,a,e,w)
+        self.range_cap = range_cap  # None: the reference's 10 km cap on the padded extent
         self.name = name
 
     @property
@@ -64,6 +65,15 @@ def c2(scan="P:3.0,-12:12:0.01"):
     return Config(n_rain=1997824, debris=[(OBJ_LEAF, 1024)], scan=scan, name="C2")
 
 
+C3_BOX = (8500.0, -8.6, 0.0, 0.0, 3000.0, 17.2, 5.7, 0.0)   # 3 km x 3 km x 1 km at 10 km
+
+
+def c3(n_rain=8388608, scan="P:3.0,-1:1:0.01"):
+    """Precision study: rain in a 3 km x 3 km x 1 km box at 10 km range (202
+    gates of 15 m); needs the 10 km extent cap lifted."""
+    return Config(n_rain=n_rain, scan=scan, box=C3_BOX, range_cap=20000.0, name="C3")
+
+
 def c5(scan="P:3.0,-12:12:0.01"):
     return Config(n_rain=2097152, debris=[(OBJ_LEAF, 262144), (OBJ_METAL_SHEET, 262144), (OBJ_BRICK, 262144)],
                   scan=scan, name="C5")
@@ -84,6 +94,8 @@ def configure(r, cfg):
     r.set_vel_data_to_config("suctvort")
     for kind, count in cfg.debris:
         r.add_debris(kind, count)
+    if cfg.range_cap:
+        r.lib.RS_set_range_cap(r.H, float(cfg.range_cap))
     box = r.suggest_scan_domain() if cfg.box is None else np.asarray(cfg.box, np.float32)
     r.set_scan_box(box)
     r.set_population(cfg.n_rain)
