@@ -194,6 +194,31 @@ typedef struct {
 int rsd_set_iq_ring(rsd_engine *e, int pulses, int range_count);
 int rsd_moments(rsd_engine *e, int pulses, const rsd_moment_params *params, float *out);
 
+/* Batches (no counterpart in the reference, which runs one domain per handle):
+ * many engines on one GPU, each an independent rain domain of its own size,
+ * tables and pulse parameters, advanced and pulsed by ONE persistent kernel
+ * launch plus one finalize per pulse round.  While an engine belongs to a
+ * batch, rsd_make_pulse only records the pulse (and the held-back advance);
+ * rsd_batch_launch launches every recorded pulse (rsd_read_pulse,
+ * downloads and rsd_sync launch it too).  Rain-only engines with binary32
+ * positions share the persistent launch; others are launched on their own
+ * streams.  The launch is captured once as a HIP graph (descriptor upload,
+ * persistent kernel, finalize) and replayed every round; the chunk size and
+ * blocks per compute unit are picked by an online autotuner over the first
+ * rounds (0 = tune) -- results differ only by reassociation of the gate sums.
+ * Destroy the batch before closing its engines. */
+typedef struct rsd_batch rsd_batch;
+int  rsd_batch_create(rsd_engine *const *engines, int count, rsd_batch **out);
+void rsd_batch_destroy(rsd_batch *b);
+int  rsd_batch_launch(rsd_batch *b);
+int  rsd_batch_sync(rsd_batch *b);
+int  rsd_batch_set_launch(rsd_batch *b, int chunk, int blocks_per_cu);
+int  rsd_batch_get_launch(rsd_batch *b, int *chunk, int *blocks_per_cu, int *tuning);
+int  rsd_batch_set_graph(rsd_batch *b, int on);
+int  rsd_batch_timer_start(rsd_batch *b);
+int  rsd_batch_timer_stop(rsd_batch *b, double *ms);
+const char *rsd_batch_last_error(const rsd_batch *b);
+
 /* Self-test: evaluate one of the device's binary32 functions on n inputs
  * (fn: 0 log1p(a), 1 log(a), 2 sin(a), 3 cos(a), 4 atan2(a, b), 5 acos(a),
  * 6 sqrt(a), 7 a / b, 8 sin(2 pi a), 9 cos(2 pi a), 10 the pulse path's fast acos(a)).

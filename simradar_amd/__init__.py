@@ -81,7 +81,9 @@ EXPORTED_RSD = [
     "rsd_set_range_weight", "rsd_set_angular_weight", "rsd_advance", "rsd_make_pulse", "rsd_read_pulse",
     "rsd_sync", "rsd_timer_start", "rsd_timer_stop", "rsd_set_launch", "rsd_math_probe", "rsd_profile",
     "rsd_profile_read", "rsd_io_test", "rsd_set_sort", "rsd_set_fold", "rsd_set_iq_ring", "rsd_moments",
-    "rsd_set_position_precision", "rsd_download_pos_lo",
+    "rsd_set_position_precision", "rsd_download_pos_lo", "rsd_batch_create", "rsd_batch_destroy",
+    "rsd_batch_launch", "rsd_batch_sync", "rsd_batch_set_launch", "rsd_batch_get_launch", "rsd_batch_set_graph",
+    "rsd_batch_timer_start", "rsd_batch_timer_stop", "rsd_batch_last_error",
 ]
 
 
@@ -239,6 +241,16 @@ def _declare(lib):
         "rsd_moments": (I, [P, I, P, P]),
         "rsd_set_position_precision": (I, [P, I]),
         "rsd_download_pos_lo": (I, [P, P]),
+        "rsd_batch_create": (I, [P, I, P]),
+        "rsd_batch_destroy": (None, [P]),
+        "rsd_batch_launch": (I, [P]),
+        "rsd_batch_sync": (I, [P]),
+        "rsd_batch_set_launch": (I, [P, I, I]),
+        "rsd_batch_get_launch": (I, [P, P, P, P]),
+        "rsd_batch_set_graph": (I, [P, I]),
+        "rsd_batch_timer_start": (I, [P]),
+        "rsd_batch_timer_stop": (I, [P, ctypes.POINTER(ctypes.c_double)]),
+        "rsd_batch_last_error": (ctypes.c_char_p, [P]),
         "rsd_math_probe": (I, [I, I, P, P, P, S]),
         "rsd_profile": (I, [P, I]),
         "rsd_io_test": (I, [P, S, I, ctypes.POINTER(ctypes.c_double)]),
@@ -504,3 +516,63 @@ class Radar:
         inf = self.info()
         if inf.last_error:
             raise RuntimeError("engine call failed (%d): %s" % (inf.last_error, self.lib.rsd_last_error(self.engine()).decode()))
+
+
+class Batch:
+    """Independent rain domains (Radar objects on one GPU) pulsed by one
+    persistent launch per round (rsd_batch_*).  Per round: make_pulse() on every
+    radar records its pulse, launch() runs them all, then each radar's
+    advance_time() / advance_beam() as usual."""
+
+    def __init__(self, radars):
+        self.radars = list(radars)
+        self.lib = self.radars[0].lib
+        arr = (ctypes.c_void_p * len(self.radars))(*[r.engine() for r in self.radars])
+        b = ctypes.c_void_p()
+        rc = self.lib.rsd_batch_create(arr, len(self.radars), ctypes.byref(b))
+        if rc:
+            raise RuntimeError("rsd_batch_create failed (%d)" % rc)
+        self.b = b
+
+    def _chk(self, rc):
+        if rc:
+            raise RuntimeError("batch: %s" % self.lib.rsd_batch_last_error(self.b).decode())
+
+    def launch(self):
+        self._chk(self.lib.rsd_batch_launch(self.b))
+
+    def sync(self):
+        self._chk(self.lib.rsd_batch_sync(self.b))
+
+    def set_launch(self, chunk, blocks_per_cu):
+        self._chk(self.lib.rsd_batch_set_launch(self.b, chunk, blocks_per_cu))
+
+    def launch_shape(self):
+        c, g, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._chk(self.lib.rsd_batch_get_launch(self.b, ctypes.byref(c), ctypes.byref(g), ctypes.byref(t)))
+        return c.value, g.value, bool(t.value)
+
+    def set_graph(self, on):
+        self._chk(self.lib.rsd_batch_set_graph(self.b, 1 if on else 0))
+
+    def timer_start(self):
+        self._chk(self.lib.rsd_batch_timer_start(self.b))
+
+    def timer_stop(self):
+        ms = ctypes.c_double()
+        self._chk(self.lib.rsd_batch_timer_stop(self.b, ctypes.byref(ms)))
+        return ms.value
+
+    def round(self):
+        """One pulse of every domain: make_pulse, one launch, advance_time, advance_beam."""
+        for r in self.radars:
+            r.make_pulse()
+        self.launch()
+        for r in self.radars:
+            r.advance_time()
+            r.advance_beam()
+
+    def free(self):
+        if self.b:
+            self.lib.rsd_batch_destroy(self.b)
+            self.b = None

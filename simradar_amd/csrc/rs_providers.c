@@ -84,6 +84,7 @@ double rs_gauss(uint64_t *s) {
 typedef struct {
     uint32_t nx, ny, nz, frames;
     float ax, rx, az, rz, core, vmax, sigma, tp, tr;
+    float cx, cy;           /* vortex centre offset from the grid centre (m) */
     uint64_t seed;
     LESTable **tables;
 } LESMem;
@@ -111,6 +112,21 @@ LESHandle LES_init_with_config_path(const LESConfig config, const char *path) {
     (void)path;
     if (config && (!strcmp(config, LESConfigSuctionVortices) || !strcmp(config, LESConfigSuctionVorticesLarge))) {
         return LES_init_synthetic(176, 176, 80, 2.0f, 1.0212f, 2.0f, 1.05f, 50.0f, 40.0f, 2.0f, 20250101ull, 10);
+    }
+    /* "suctvort:cx,cy,seed": the same grid with the vortex centre moved by
+     * (cx, cy) metres and its own noise seed (independent domains of a batch) */
+    if (config && !strncmp(config, "suctvort:", 9)) {
+        float cx = 0.0f, cy = 0.0f;
+        unsigned long long seed = 20250101ull;
+        if (sscanf(config + 9, "%f,%f,%llu", &cx, &cy, &seed) >= 2) {
+            LESMem *h = (LESMem *)LES_init_synthetic(176, 176, 80, 2.0f, 1.0212f, 2.0f, 1.05f, 50.0f, 40.0f, 2.0f,
+                                                     (uint64_t)seed, 10);
+            if (h) {
+                h->cx = cx;
+                h->cy = cy;
+            }
+            return (LESHandle)h;
+        }
     }
     rsprint("ERROR: LES configuration '%s' needs external data; only '%s' is generated in memory.",
             config ? config : "(null)", LESConfigSuctionVortices);
@@ -168,10 +184,11 @@ static LESTable *les_make_frame(LESMem *h, uint32_t f) {
             double y = texel_position((int)j, offy, h->ax, h->rx);
             for (uint32_t i = 0; i < h->nx; i++, idx++) {
                 double x = texel_position((int)i, offx, h->ax, h->rx);
-                double rho = sqrt(x * x + y * y);
+                double xc = x - h->cx, yc = y - h->cy;
+                double rho = sqrt(xc * xc + yc * yc);
                 double vt = rho <= h->core ? h->vmax * rho / h->core : h->vmax * h->core / rho;
-                double u = rho > 0.0 ? -vt * y / rho : 0.0;
-                double v = rho > 0.0 ? vt * x / rho : 0.0;
+                double u = rho > 0.0 ? -vt * yc / rho : 0.0;
+                double v = rho > 0.0 ? vt * xc / rho : 0.0;
                 t->uvwt[idx][0] = (float)(u + h->sigma * rs_gauss(&s));
                 t->uvwt[idx][1] = (float)(v + h->sigma * rs_gauss(&s));
                 t->uvwt[idx][2] = (float)(h->sigma * rs_gauss(&s));

@@ -553,10 +553,14 @@ __device__ __forceinline__ void acc_clear(double *acc, int n) {
     __syncthreads();
 }
 
-__device__ __forceinline__ void acc_flush(const double *acc, const PulseK &k) {
+__device__ __forceinline__ void acc_flush_row(const double *acc, const PulseK &k, int row) {
     __syncthreads();
-    double *row = k.partials + (size_t)(k.row0 + blockIdx.x) * 4 * k.R;
-    for (int j = threadIdx.x; j < 4 * k.R; j += blockDim.x) row[j] = acc[j];
+    double *dst = k.partials + (size_t)row * 4 * k.R;
+    for (int j = threadIdx.x; j < 4 * k.R; j += blockDim.x) dst[j] = acc[j];
+}
+
+__device__ __forceinline__ void acc_flush(const double *acc, const PulseK &k) {
+    acc_flush_row(acc, k, k.row0 + blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -683,21 +687,11 @@ __device__ __forceinline__ void logtab_load(float *dst, const float *src) {
     for (int j = threadIdx.x; j < 3 * RF_LOG_TABLE; j += blockDim.x) dst[j] = src[j];
 }
 
+// Scatterers [lo, hi) of one rain population: advance and/or pulse terms into
+// the block's LDS gate sums acc (cleared by the caller).
 template <bool ADV, bool PULSE, bool HILO>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_rain(RainK a) {
-    extern __shared__ double acc[];
-    __shared__ float logtab[3 * RF_LOG_TABLE];
-    if (ADV) logtab_load(logtab, a.logtab);
-    if (PULSE) acc_clear(acc, 4 * a.pk.R);
-    else __syncthreads();
-    size_t lo, hi;
-    block_range(0, a.n, lo, hi);
-    const bool use_win = PULSE && a.pk.fold == RSD_FOLD_WINDOW;
-    float *wl = (float *)(acc + 4 * a.pk.R);
-    if (use_win) {
-        for (int j = threadIdx.x; j < a.pk.rangew_n; j += blockDim.x) wl[j] = a.pk.rangew[j];
-        __syncthreads();
-    }
+__device__ __forceinline__ void rain_range(const RainK &a, size_t lo, size_t hi, double *acc, const float *logtab,
+                                           const float *wl, bool use_win) {
     GateWin win;
     win_reset(win, RSD_WIN_EMPTY);
     for (size_t base = lo; base < hi; base += blockDim.x) {
@@ -741,7 +735,86 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         }
     }
     if (use_win) win_flush(win, a.pk, acc);
+}
+
+template <bool ADV, bool PULSE, bool HILO>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_rain(RainK a) {
+    extern __shared__ double acc[];
+    __shared__ float logtab[3 * RF_LOG_TABLE];
+    if (ADV) logtab_load(logtab, a.logtab);
+    if (PULSE) acc_clear(acc, 4 * a.pk.R);
+    else __syncthreads();
+    size_t lo, hi;
+    block_range(0, a.n, lo, hi);
+    const bool use_win = PULSE && a.pk.fold == RSD_FOLD_WINDOW;
+    float *wl = (float *)(acc + 4 * a.pk.R);
+    if (use_win) {
+        for (int j = threadIdx.x; j < a.pk.rangew_n; j += blockDim.x) wl[j] = a.pk.rangew[j];
+        __syncthreads();
+    }
+    rain_range<ADV, PULSE, HILO>(a, lo, hi, acc, logtab, wl, use_win);
     if (PULSE) acc_flush(acc, a.pk);
+}
+
+// Many independent rain domains in one persistent launch.  items[t] = (domain,
+// chunk): chunk c of domain d covers scatterers [c chunk, (c + 1) chunk) and owns
+// partial row c of that domain.  Block b takes the items of logical block
+// L = (b mod 8)(G / 8) + b / 8, L + G, L + 2G, ...: each XCD walks contiguous
+// items, so one domain's wind table stays in one L2.
+template <bool ADV, bool PULSE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_rain_batch(
+    const RainK *doms, const int2 *items, int n_items, int chunk, const float *logtab_g) {
+    extern __shared__ double acc[];
+    __shared__ float logtab[3 * RF_LOG_TABLE];
+    if (ADV) logtab_load(logtab, logtab_g);
+    __syncthreads();
+    const unsigned G = gridDim.x, b = blockIdx.x;
+    const unsigned L = (G & 7u) == 0u ? (b & 7u) * (G >> 3) + (b >> 3) : b;
+    for (int t = (int)L; t < n_items; t += (int)G) {
+        const int2 it = items[t];
+        const RainK &a = doms[it.x];
+        if (PULSE) acc_clear(acc, 4 * a.pk.R);
+        const bool use_win = PULSE && a.pk.fold == RSD_FOLD_WINDOW;
+        float *wl = (float *)(acc + 4 * a.pk.R);
+        if (use_win) {
+            for (int j = threadIdx.x; j < a.pk.rangew_n; j += blockDim.x) wl[j] = a.pk.rangew[j];
+            __syncthreads();
+        }
+        const size_t lo = (size_t)it.y * (size_t)chunk;
+        const size_t hi = lo + (size_t)chunk < a.n ? lo + (size_t)chunk : a.n;
+        rain_range<ADV, PULSE, false>(a, lo, hi, acc, logtab, wl, use_win);
+        if (PULSE) acc_flush_row(acc, a.pk, it.y);
+        __syncthreads();
+    }
+}
+
+// Finalize of a batch: block j belongs to the domain d with first[d] <= j <
+// first[d + 1] and sums that domain's partial rows for output j - first[d].
+struct FinK {
+    const double *partials;
+    double *out, *out2;
+    int rows, n4r;
+};
+
+__global__ __launch_bounds__(256) void k_finalize_batch(const FinK *f, const int *first, int nd) {
+    __shared__ double red[256];
+    int d = 0;
+    while (d + 1 < nd && first[d + 1] <= (int)blockIdx.x) d++;
+    const FinK a = f[d];
+    const int j = (int)blockIdx.x - first[d];
+    const int R = a.n4r / 4;
+    double s = 0.0;
+    for (int r = threadIdx.x; r < a.rows; r += blockDim.x) s += a.partials[(size_t)r * a.n4r + j];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        a.out[4 * (j % R) + j / R] = red[0];
+        if (a.out2) a.out2[4 * (j % R) + j / R] = red[0];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1202,6 +1275,11 @@ struct rsd_engine {
     int sort_bits;          // key bits of the current sort
     float4 *plo, *plo2;     // hi/lo positions: low parts (and ping-pong copy)
     int pos_hilo;           // 1: positions carried as hi/lo pairs
+    // batch membership: pulses are held back and launched by the batch
+    struct rsd_batch *batch;
+    int have_pending_pulse;
+    rsd_pulse pending_pulse;
+    int batch_wait;         // the engine stream must wait for the batch's last launch
     void *sort_tmp;
     size_t sort_tmp_bytes;
     int sort_interval;      // pulses between sorts (0 = never)
@@ -1251,6 +1329,42 @@ struct rsd_engine {
 };
 
 #define RSD_PROF_CAP 16384
+
+#define RSD_TUNE_REPS 3
+
+struct rsd_batch {
+    int device, cus;
+    hipStream_t stream;
+    int nd;
+    rsd_engine **eng;
+    // descriptors: two pinned host buffers (filled alternately) and one device copy
+    char *desc_h[2];
+    char *desc_d;
+    size_t desc_cap;
+    int buf;
+    hipEvent_t ev_buf[2];
+    hipEvent_t ev_launch;
+    // work items
+    int2 *items_d;
+    int items_cap, n_items, chunk, grid;
+    size_t items_sig;
+    int chunk_req, grid_req;       // 0 = autotune
+    // graphs (one per descriptor buffer)
+    int use_graph;
+    hipGraphExec_t gexec[2];
+    size_t gsig[2];
+    // autotuner
+    int tuning, tune_idx, tune_rep, tune_pending;
+    double tune_sum, tune_best;
+    int best_chunk, best_grid;
+    hipEvent_t tev0, tev1;
+    hipEvent_t t0, t1;
+    char err[256];
+};
+
+static int flush_pending(rsd_engine *e);
+static void join_batch(rsd_engine *e);
+extern "C" int rsd_batch_launch(struct rsd_batch *b);
 
 static int fail(rsd_engine *e, int code, const char *fmt, ...) {
     if (e) {
@@ -1358,6 +1472,17 @@ static void free_state(rsd_engine *e) {
 extern "C" void rsd_close(rsd_engine *e) {
     if (!e) return;
     (void)hipSetDevice(e->device);
+    if (e->batch) {   // leave the batch: launch what is recorded, then drop this engine
+        rsd_batch *b = e->batch;
+        if (e->have_pending_pulse) (void)rsd_batch_launch(b);
+        (void)hipStreamSynchronize(b->stream);
+        int k = 0;
+        for (int i = 0; i < b->nd; i++)
+            if (b->eng[i] != e) b->eng[k++] = b->eng[i];
+        b->nd = k;
+        b->items_sig = 0;
+        e->batch = NULL;
+    }
     (void)hipStreamSynchronize(e->stream);
     free_state(e);
     for (int s = 0; s < 2; s++) dfree(e->wind[s]);
@@ -1600,6 +1725,7 @@ static int fill_debris(rsd_engine *e, const rsd_step *s, const rsd_pulse *p, int
 // Launch the kernels of one step: advance (s != NULL) and/or pulse (p != NULL).
 static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p) {
     HIPCHK(e, hipSetDevice(e->device));
+    join_batch(e);
     const int R = p ? (int)p->range_count : 0;
     if (p && (R < 1 || R > RSD_MAX_GATES)) return fail(e, RSD_ERR_ARG, "range_count %d out of range", R);
     // Rows of the partial-sum buffer: one per block of every launch.
@@ -1789,8 +1915,6 @@ static int unsort_rain(rsd_engine *e) {
     return RSD_OK;
 }
 
-static int flush_pending(rsd_engine *e);
-
 extern "C" int rsd_set_position_precision(rsd_engine *e, int hilo) {
     if (!e || (hilo != 0 && hilo != 1)) return fail(e, RSD_ERR_ARG, "position precision %d unknown", hilo);
     HIPCHK(e, hipSetDevice(e->device));
@@ -1890,6 +2014,11 @@ extern "C" int rsd_set_sort(rsd_engine *e, int interval) {
 }
 
 static int flush_pending(rsd_engine *e) {
+    if (e->batch && e->have_pending_pulse) {
+        int st = rsd_batch_launch(e->batch);
+        if (st) return fail(e, st, "batch launch failed");
+    }
+    join_batch(e);
     if (!e->have_pending) return RSD_OK;
     e->have_pending = 0;
     return launch_step(e, &e->pending, NULL);
@@ -1901,6 +2030,7 @@ extern "C" int rsd_upload_state(rsd_engine *e, const float *pos, const float *ve
     int st = flush_pending(e);
     if (st) return st;
     HIPCHK(e, hipSetDevice(e->device));
+    join_batch(e);
     st = unsort_rain(e);
     if (st) return st;
     const size_t b = e->n * sizeof(float4);
@@ -2085,6 +2215,17 @@ extern "C" int rsd_advance(rsd_engine *e, const rsd_step *step) {
 extern "C" int rsd_make_pulse(rsd_engine *e, const rsd_pulse *p) {
     if (!e || !p) return fail(e, RSD_ERR_ARG, "null pulse");
     if (!e->pos) return fail(e, RSD_ERR_STATE, "state not allocated");
+    if (e->batch) {
+        if (e->have_pending_pulse) {
+            int st = rsd_batch_launch(e->batch);
+            if (st) return st;
+        }
+        if (p->range_count < 1 || p->range_count > RSD_MAX_GATES) return fail(e, RSD_ERR_ARG, "range_count out of range");
+        e->pending_pulse = *p;
+        e->have_pending_pulse = 1;
+        return RSD_OK;
+    }
+    join_batch(e);
     if (e->sort_interval > 0 && e->n_rain > 1 && (!e->permuted || e->since_sort >= e->sort_interval)) {
         HIPCHK(e, hipSetDevice(e->device));
         int st = sort_rain(e, e->have_pending ? e->pending.sim_desc : p->sim_desc, p->range_delta);
@@ -2100,6 +2241,11 @@ extern "C" int rsd_make_pulse(rsd_engine *e, const rsd_pulse *p) {
 
 extern "C" int rsd_read_pulse(rsd_engine *e, double *out, uint32_t R) {
     if (!e || !out) return fail(e, RSD_ERR_ARG, "null output");
+    if (e->batch && e->have_pending_pulse) {
+        int st = rsd_batch_launch(e->batch);
+        if (st) return st;
+    }
+    join_batch(e);
     if (!e->have_pulse) return fail(e, RSD_ERR_STATE, "no pulse made yet");
     if (R != e->last_pulse.range_count) return fail(e, RSD_ERR_ARG, "range_count %u != %u of the last pulse", R, e->last_pulse.range_count);
     HIPCHK(e, hipSetDevice(e->device));
@@ -2212,5 +2358,410 @@ extern "C" int rsd_timer_stop(rsd_engine *e, double *ms) {
     float t = 0.0f;
     HIPCHK(e, hipEventElapsedTime(&t, e->ev0, e->ev1));
     *ms = (double)t;
+    return RSD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batches: many engines (independent rain domains) advanced and pulsed by one
+// persistent launch (k_rain_batch) and one finalize, optionally replayed from
+// a captured HIP graph, with launch shape chosen by an online autotuner.
+// ---------------------------------------------------------------------------
+
+static const int TUNE_CHUNK[] = {1024, 2048, 4096};
+static const int TUNE_GRID[] = {2, 4, 8};     // blocks per compute unit
+#define RSD_TUNE_N 9
+
+static void join_batch(rsd_engine *e) {
+    if (e->batch && e->batch_wait) {
+        (void)hipStreamWaitEvent(e->stream, e->batch->ev_launch, 0);
+        e->batch_wait = 0;
+    }
+}
+
+static int bfail(rsd_batch *b, int code, const char *msg) {
+    snprintf(b->err, sizeof(b->err), "%s", msg);
+    return code;
+}
+
+extern "C" const char *rsd_batch_last_error(const rsd_batch *b) { return b ? b->err : "no batch"; }
+
+extern "C" int rsd_batch_create(rsd_engine *const *engines, int count, rsd_batch **out) {
+    if (!engines || count < 1 || !out) return RSD_ERR_ARG;
+    *out = NULL;
+    const int dev = engines[0]->device;
+    for (int i = 0; i < count; i++) {
+        if (!engines[i] || engines[i]->device != dev || engines[i]->batch) return RSD_ERR_ARG;
+        for (int j = 0; j < i; j++)
+            if (engines[j] == engines[i]) return RSD_ERR_ARG;
+    }
+    rsd_batch *b = (rsd_batch *)calloc(1, sizeof(rsd_batch));
+    if (!b) return RSD_ERR_ALLOC;
+    b->eng = (rsd_engine **)calloc((size_t)count, sizeof(rsd_engine *));
+    if (!b->eng) {
+        free(b);
+        return RSD_ERR_ALLOC;
+    }
+    memcpy(b->eng, engines, (size_t)count * sizeof(rsd_engine *));
+    b->nd = count;
+    b->device = dev;
+    b->use_graph = 1;
+    b->tuning = 1;
+    b->tune_best = 1e30;
+    hipDeviceProp_t prop;
+    if (hipSetDevice(dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->ev_buf[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->ev_buf[1], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->ev_launch, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreate(&b->tev0) != hipSuccess || hipEventCreate(&b->tev1) != hipSuccess ||
+        hipEventCreate(&b->t0) != hipSuccess || hipEventCreate(&b->t1) != hipSuccess) {
+        free(b->eng);
+        free(b);
+        return RSD_ERR_HIP;
+    }
+    b->cus = prop.multiProcessorCount;
+    (void)hipEventRecord(b->ev_buf[0], b->stream);
+    (void)hipEventRecord(b->ev_buf[1], b->stream);
+    (void)hipEventRecord(b->ev_launch, b->stream);
+    for (int i = 0; i < count; i++) engines[i]->batch = b;
+    *out = b;
+    return RSD_OK;
+}
+
+extern "C" void rsd_batch_destroy(rsd_batch *b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    for (int i = 0; i < b->nd; i++) {
+        rsd_engine *e = b->eng[i];
+        if (e->have_pending_pulse) (void)rsd_batch_launch(b);
+    }
+    (void)hipStreamSynchronize(b->stream);
+    for (int i = 0; i < b->nd; i++) {
+        b->eng[i]->batch = NULL;
+        b->eng[i]->batch_wait = 0;
+    }
+    for (int k = 0; k < 2; k++) {
+        if (b->gexec[k]) (void)hipGraphExecDestroy(b->gexec[k]);
+        if (b->desc_h[k]) (void)hipHostFree(b->desc_h[k]);
+        (void)hipEventDestroy(b->ev_buf[k]);
+    }
+    dfree(b->desc_d);
+    dfree(b->items_d);
+    (void)hipEventDestroy(b->ev_launch);
+    (void)hipEventDestroy(b->tev0);
+    (void)hipEventDestroy(b->tev1);
+    (void)hipEventDestroy(b->t0);
+    (void)hipEventDestroy(b->t1);
+    (void)hipStreamDestroy(b->stream);
+    free(b->eng);
+    free(b);
+}
+
+extern "C" int rsd_batch_set_launch(rsd_batch *b, int chunk, int blocks_per_cu) {
+    if (!b) return RSD_ERR_ARG;
+    if (chunk != 0 && (chunk < 256 || chunk % 256 != 0 || chunk > (1 << 20))) return bfail(b, RSD_ERR_ARG, "chunk must be a multiple of 256");
+    if (blocks_per_cu < 0 || blocks_per_cu > 64) return bfail(b, RSD_ERR_ARG, "blocks per CU out of range");
+    b->chunk_req = chunk;
+    b->grid_req = blocks_per_cu;
+    b->tuning = (chunk == 0 || blocks_per_cu == 0);
+    b->tune_idx = b->tune_rep = b->tune_pending = 0;
+    b->tune_sum = 0.0;
+    b->tune_best = 1e30;
+    return RSD_OK;
+}
+
+extern "C" int rsd_batch_set_graph(rsd_batch *b, int on) {
+    if (!b) return RSD_ERR_ARG;
+    b->use_graph = on ? 1 : 0;
+    return RSD_OK;
+}
+
+extern "C" int rsd_batch_get_launch(rsd_batch *b, int *chunk, int *blocks_per_cu, int *tuning) {
+    if (!b) return RSD_ERR_ARG;
+    if (chunk) *chunk = b->tuning ? 0 : b->best_chunk;
+    if (blocks_per_cu) *blocks_per_cu = b->tuning ? 0 : b->best_grid;
+    if (tuning) *tuning = b->tuning;
+    return RSD_OK;
+}
+
+// Launch shape of this launch (and autotuner bookkeeping of the previous one).
+static void batch_shape(rsd_batch *b, int *chunk, int *bpc) {
+    if (!b->tuning) {
+        *chunk = b->chunk_req ? b->chunk_req : b->best_chunk;
+        *bpc = b->grid_req ? b->grid_req : b->best_grid;
+        return;
+    }
+    if (b->tune_pending) {   // time of the previous tuning launch
+        float ms = 0.0f;
+        (void)hipEventSynchronize(b->tev1);
+        (void)hipEventElapsedTime(&ms, b->tev0, b->tev1);
+        if (b->tune_rep > 0) b->tune_sum += ms;   // the first launch of a shape warms it up
+        b->tune_rep++;
+        b->tune_pending = 0;
+        if (b->tune_rep == RSD_TUNE_REPS) {
+            const double avg = b->tune_sum / (RSD_TUNE_REPS - 1);
+            const int c = b->chunk_req ? b->chunk_req : TUNE_CHUNK[b->tune_idx / 3];
+            const int g = b->grid_req ? b->grid_req : TUNE_GRID[b->tune_idx % 3];
+            if (avg < b->tune_best) {
+                b->tune_best = avg;
+                b->best_chunk = c;
+                b->best_grid = g;
+            }
+            b->tune_idx++;
+            b->tune_rep = 0;
+            b->tune_sum = 0.0;
+            if (b->tune_idx == RSD_TUNE_N || (b->chunk_req && b->tune_idx % 3 == 0 && b->tune_idx >= 3) ||
+                (b->grid_req && b->tune_idx >= 3)) {
+                b->tuning = 0;
+                *chunk = b->chunk_req ? b->chunk_req : b->best_chunk;
+                *bpc = b->grid_req ? b->grid_req : b->best_grid;
+                return;
+            }
+        }
+    }
+    int idx = b->tune_idx;
+    if (b->grid_req) idx *= 3;            // only the chunk varies
+    *chunk = b->chunk_req ? b->chunk_req : TUNE_CHUNK[(idx / 3) % 3];
+    *bpc = b->grid_req ? b->grid_req : TUNE_GRID[idx % 3];
+}
+
+extern "C" int rsd_batch_launch(rsd_batch *b) {
+    if (!b) return RSD_ERR_ARG;
+    if (hipSetDevice(b->device) != hipSuccess) return bfail(b, RSD_ERR_HIP, "hipSetDevice");
+    // Engines with a pending pulse: rain-only, binary32 positions go through the
+    // batched launch; anything else is launched on its own stream.
+    rsd_engine *sel[1024];
+    int ns = 0, any_adv = -1, mixed = 0;
+    for (int i = 0; i < b->nd; i++) {
+        rsd_engine *e = b->eng[i];
+        if (!e->have_pending_pulse) continue;
+        const int batched = (e->n == e->n_rain) && !e->plo && e->n_rain > 0 && ns < 1024;
+        int adv = e->have_pending ? 1 : 0;
+        if (batched && (any_adv < 0 || any_adv == adv)) {
+            any_adv = adv;
+            sel[ns++] = e;
+        } else {
+            mixed = 1;
+            e->have_pending_pulse = 0;
+            int st;
+            if (e->have_pending) {
+                e->have_pending = 0;
+                st = launch_step(e, &e->pending, &e->pending_pulse);
+            } else {
+                st = launch_step(e, NULL, &e->pending_pulse);
+            }
+            if (st) return bfail(b, st, e->err);
+        }
+    }
+    (void)mixed;
+    if (ns == 0) return RSD_OK;
+    // range reorder of each domain that is due, on its own stream
+    for (int k = 0; k < ns; k++) {
+        rsd_engine *e = sel[k];
+        join_batch(e);
+        if (e->sort_interval > 0 && e->n_rain > 1 && (!e->permuted || e->since_sort >= e->sort_interval)) {
+            int st = sort_rain(e, e->have_pending ? e->pending.sim_desc : e->pending_pulse.sim_desc,
+                               e->pending_pulse.range_delta);
+            if (st) return bfail(b, st, e->err);
+        }
+        e->since_sort++;
+        if (hipEventRecord(e->ev_fork, e->stream) != hipSuccess ||
+            hipStreamWaitEvent(b->stream, e->ev_fork, 0) != hipSuccess)
+            return bfail(b, RSD_ERR_HIP, "stream join");
+    }
+    int chunk, bpc;
+    batch_shape(b, &chunk, &bpc);
+    // work items and partial buffers
+    size_t sig = (size_t)chunk * 1315423911u;
+    int n_items = 0;
+    for (int k = 0; k < ns; k++) {
+        const int rows = (int)((sel[k]->n_rain + chunk - 1) / chunk);
+        n_items += rows;
+        sig = sig * 31u + (size_t)sel[k]->n_rain + (size_t)(uintptr_t)sel[k];
+    }
+    if (sig != b->items_sig || n_items != b->n_items) {
+        if (n_items > b->items_cap) {
+            dfree(b->items_d);
+            b->items_d = NULL;
+            if (hipMalloc((void **)&b->items_d, (size_t)n_items * sizeof(int2)) != hipSuccess) return bfail(b, RSD_ERR_ALLOC, "items");
+            b->items_cap = n_items;
+        }
+        int2 *h = (int2 *)malloc((size_t)n_items * sizeof(int2));
+        if (!h) return bfail(b, RSD_ERR_ALLOC, "items");
+        int t = 0;
+        for (int k = 0; k < ns; k++) {
+            const int rows = (int)((sel[k]->n_rain + chunk - 1) / chunk);
+            for (int c = 0; c < rows; c++) h[t++] = make_int2(k, c);
+        }
+        hipError_t r = hipMemcpy(b->items_d, h, (size_t)n_items * sizeof(int2), hipMemcpyHostToDevice);
+        free(h);
+        if (r != hipSuccess) return bfail(b, RSD_ERR_HIP, "items upload");
+        b->items_sig = sig;
+        b->n_items = n_items;
+    }
+    int grid = bpc * b->cus;
+    if (grid > n_items) grid = n_items;
+    if (grid >= 8) grid &= ~7;
+    // descriptors
+    const size_t rk_bytes = (size_t)ns * sizeof(RainK), fk_bytes = (size_t)ns * sizeof(FinK);
+    const size_t desc_bytes = rk_bytes + fk_bytes + (size_t)(ns + 1) * sizeof(int);
+    if (desc_bytes > b->desc_cap) {
+        (void)hipStreamSynchronize(b->stream);
+        for (int k2 = 0; k2 < 2; k2++) {
+            if (b->desc_h[k2]) (void)hipHostFree(b->desc_h[k2]);
+            b->desc_h[k2] = NULL;
+            if (b->gexec[k2]) (void)hipGraphExecDestroy(b->gexec[k2]);
+            b->gexec[k2] = NULL;
+            b->gsig[k2] = 0;
+        }
+        dfree(b->desc_d);
+        b->desc_d = NULL;
+        if (hipHostMalloc((void **)&b->desc_h[0], desc_bytes, hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void **)&b->desc_h[1], desc_bytes, hipHostMallocDefault) != hipSuccess ||
+            hipMalloc((void **)&b->desc_d, desc_bytes) != hipSuccess)
+            return bfail(b, RSD_ERR_ALLOC, "descriptors");
+        b->desc_cap = desc_bytes;
+    }
+    const int buf = b->buf;
+    if (hipEventSynchronize(b->ev_buf[buf]) != hipSuccess) return bfail(b, RSD_ERR_HIP, "descriptor buffer wait");
+    RainK *rk = (RainK *)b->desc_h[buf];
+    FinK *fk = (FinK *)(b->desc_h[buf] + rk_bytes);
+    int *first = (int *)(b->desc_h[buf] + rk_bytes + fk_bytes);
+    size_t lds = 0;
+    int total4r = 0;
+    for (int k = 0; k < ns; k++) {
+        rsd_engine *e = sel[k];
+        const rsd_pulse *p = &e->pending_pulse;
+        const int R = (int)p->range_count;
+        const int rows = (int)((e->n_rain + chunk - 1) / chunk);
+        int st = ensure_partials(e, (size_t)rows, R);
+        if (st) return bfail(b, st, e->err);
+        RainK a;
+        memset(&a, 0, sizeof(a));
+        st = fill_rain(e, any_adv ? &e->pending : NULL, a);
+        if (st) return bfail(b, st, e->err);
+        st = fill_pulse(e, p, a.pk);
+        if (st) return bfail(b, st, e->err);
+        a.pk.row0 = 0;
+        a.plo = NULL;
+        rk[k] = a;
+        FinK f;
+        f.partials = e->partials;
+        f.rows = rows;
+        f.n4r = 4 * R;
+        f.out = e->pulse_d;
+        f.out2 = NULL;
+        if (e->ring && e->ring_R == R) {
+            f.out2 = e->ring + (size_t)e->ring_next * 4 * R;
+            e->ring_next = (e->ring_next + 1) % e->ring_m;
+            if (e->ring_count < e->ring_m) e->ring_count++;
+        }
+        fk[k] = f;
+        first[k] = total4r;
+        total4r += 4 * R;
+        size_t l = (size_t)4 * R * sizeof(double) + (a.pk.fold == RSD_FOLD_WINDOW ? (size_t)e->rangew_n * sizeof(float) : 0);
+        if (l > lds) lds = l;
+    }
+    first[ns] = total4r;
+    const float *logtab = sel[0]->logtab;
+    const RainK *rk_d = (const RainK *)b->desc_d;
+    const FinK *fk_d = (const FinK *)(b->desc_d + rk_bytes);
+    const int *first_d = (const int *)(b->desc_d + rk_bytes + fk_bytes);
+    const bool timed = b->tuning;
+    // everything the graph depends on besides descriptor contents
+    const size_t gsig = (((((size_t)n_items * 131u + (size_t)grid) * 131u + lds) * 131u + (size_t)chunk) * 131u +
+                         (size_t)total4r) * 4u + (size_t)any_adv * 2u + 1u + (desc_bytes << 20) +
+                        (size_t)(uintptr_t)logtab * 7u;
+    auto enqueue = [&](hipStream_t st) -> hipError_t {
+        hipError_t r = hipMemcpyAsync(b->desc_d, b->desc_h[buf], desc_bytes, hipMemcpyHostToDevice, st);
+        if (r != hipSuccess) return r;
+        if (any_adv)
+            hipLaunchKernelGGL((k_rain_batch<true, true>), dim3(grid), dim3(256), lds, st, rk_d,
+                               (const int2 *)b->items_d, n_items, chunk, logtab);
+        else
+            hipLaunchKernelGGL((k_rain_batch<false, true>), dim3(grid), dim3(256), lds, st, rk_d,
+                               (const int2 *)b->items_d, n_items, chunk, logtab);
+        r = hipGetLastError();
+        if (r != hipSuccess) return r;
+        hipLaunchKernelGGL(k_finalize_batch, dim3(total4r), dim3(256), 0, st, fk_d, first_d, ns);
+        return hipGetLastError();
+    };
+    if (timed) (void)hipEventRecord(b->tev0, b->stream);
+    if (b->use_graph && !timed) {
+        if (!b->gexec[buf] || b->gsig[buf] != gsig) {
+            if (b->gexec[buf]) (void)hipGraphExecDestroy(b->gexec[buf]);
+            b->gexec[buf] = NULL;
+            hipGraph_t g = NULL;
+            if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+                return bfail(b, RSD_ERR_HIP, "begin capture");
+            hipError_t r = enqueue(b->stream);
+            hipError_t r2 = hipStreamEndCapture(b->stream, &g);
+            if (r != hipSuccess || r2 != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return bfail(b, RSD_ERR_HIP, "capture");
+            }
+            r = hipGraphInstantiate(&b->gexec[buf], g, NULL, NULL, 0);
+            (void)hipGraphDestroy(g);
+            if (r != hipSuccess) return bfail(b, RSD_ERR_HIP, "graph instantiate");
+            b->gsig[buf] = gsig;
+        }
+        if (hipGraphLaunch(b->gexec[buf], b->stream) != hipSuccess) return bfail(b, RSD_ERR_HIP, "graph launch");
+    } else {
+        if (enqueue(b->stream) != hipSuccess) return bfail(b, RSD_ERR_HIP, "batch launch");
+    }
+    if (timed) {
+        (void)hipEventRecord(b->tev1, b->stream);
+        b->tune_pending = 1;
+    }
+    if (hipEventRecord(b->ev_buf[buf], b->stream) != hipSuccess || hipEventRecord(b->ev_launch, b->stream) != hipSuccess)
+        return bfail(b, RSD_ERR_HIP, "event");
+    b->buf ^= 1;
+    for (int k = 0; k < ns; k++) {
+        rsd_engine *e = sel[k];
+        e->have_pending = 0;
+        e->have_pending_pulse = 0;
+        e->last_pulse = e->pending_pulse;
+        e->have_pulse = 1;
+        e->batch_wait = 1;
+    }
+    return RSD_OK;
+}
+
+extern "C" int rsd_batch_sync(rsd_batch *b) {
+    if (!b) return RSD_ERR_ARG;
+    for (int i = 0; i < b->nd; i++)
+        if (b->eng[i]->have_pending_pulse) {
+            int st = rsd_batch_launch(b);
+            if (st) return st;
+            break;
+        }
+    if (hipSetDevice(b->device) != hipSuccess || hipStreamSynchronize(b->stream) != hipSuccess)
+        return bfail(b, RSD_ERR_HIP, "sync");
+    for (int i = 0; i < b->nd; i++) {
+        int st = rsd_sync(b->eng[i]);
+        if (st) return bfail(b, st, b->eng[i]->err);
+    }
+    return RSD_OK;
+}
+
+extern "C" int rsd_batch_timer_start(rsd_batch *b) {
+    if (!b) return RSD_ERR_ARG;
+    (void)hipSetDevice(b->device);
+    return hipEventRecord(b->t0, b->stream) == hipSuccess ? RSD_OK : RSD_ERR_HIP;
+}
+
+extern "C" int rsd_batch_timer_stop(rsd_batch *b, double *ms) {
+    if (!b || !ms) return RSD_ERR_ARG;
+    for (int i = 0; i < b->nd; i++)
+        if (b->eng[i]->have_pending_pulse) {
+            int st = rsd_batch_launch(b);
+            if (st) return st;
+            break;
+        }
+    float f = 0.0f;
+    if (hipEventRecord(b->t1, b->stream) != hipSuccess || hipEventSynchronize(b->t1) != hipSuccess ||
+        hipEventElapsedTime(&f, b->t0, b->t1) != hipSuccess)
+        return RSD_ERR_HIP;
+    *ms = f;
     return RSD_OK;
 }

@@ -28,7 +28,8 @@ def mp_dsd(rain_rate_mm_h=10.0, d_mm=None, n0=8000.0):
 
 class Config:
     def __init__(self, n_rain=131072, debris=(), concept=DBU, gate=15.0, scan="P:3.0,-12:12:0.01", seed=SEED,
-                 prt=1e-3, phase=PHASE_F64, rain_rate=10.0, box=None, name="custom", range_cap=None):
+                 prt=1e-3, phase=PHASE_F64, rain_rate=10.0, box=None, name="custom", range_cap=None,
+                 les="suctvort"):
         self.n_rain = n_rain
         self.debris = list(debris)
         self.concept = concept
@@ -40,6 +41,7 @@ class Config:
         self.rain_rate = rain_rate
         self.box = box          # None: RS_suggest_scan_domain; else 8 floats (origin r,a,e,w, size r,a,e,w)
         self.range_cap = range_cap  # None: the reference's 10 km cap on the padded extent
+        self.les = les          # LES provider config ("suctvort" or "suctvort:cx,cy,seed")
         self.name = name
 
     @property
@@ -74,6 +76,23 @@ def c3(n_rain=8388608, scan="P:3.0,-1:1:0.01"):
     return Config(n_rain=n_rain, scan=scan, box=C3_BOX, range_cap=20000.0, name="C3")
 
 
+def c4(n_domains=64, seed=4242, scale=1.0):
+    """Batching: independent domains with rain counts log-uniform in
+    [2^16, 2^21] (times `scale`, for reduced test sizes), rain rates log-uniform
+    in [1, 100] mm/h and vortex centres moved up to 200 m from the grid centre,
+    each with its own turbulence seed (seeded, reproducible)."""
+    g = np.random.default_rng(seed)
+    out = []
+    for d in range(n_domains):
+        n = int(round(2.0 ** g.uniform(16.0, 21.0) * scale))
+        n = max(256, n)
+        rate = float(10.0 ** g.uniform(0.0, 2.0))
+        cx, cy = g.uniform(-200.0, 200.0, 2)
+        out.append(Config(n_rain=n, rain_rate=rate, seed=SEED + d, name="C4-%02d" % d,
+                          les="suctvort:%.1f,%.1f,%d" % (cx, cy, 1000 + d)))
+    return out
+
+
 def c5(scan="P:3.0,-12:12:0.01"):
     return Config(n_rain=2097152, debris=[(OBJ_LEAF, 262144), (OBJ_METAL_SHEET, 262144), (OBJ_BRICK, 262144)],
                   scan=scan, name="C5")
@@ -91,7 +110,7 @@ def configure(r, cfg):
     r.set_dsd(nd, d, 1)
     r.set_random_seed(cfg.seed)
     r.set_scan_pattern(cfg.scan)
-    r.set_vel_data_to_config("suctvort")
+    r.set_vel_data_to_config(cfg.les)
     for kind, count in cfg.debris:
         r.add_debris(kind, count)
     if cfg.range_cap:
