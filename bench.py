@@ -9,6 +9,11 @@ complex H/V sums (RS_advance_time + RS_advance_beam + RS_make_pulse), with the
 state resident in HBM.  Prints one JSON line.
 
     python bench.py [--gpus 1] [--steps 1000] [--warmup 50] [--no-cpu]
+
+Other workloads (one JSON line each, not the headline): --workload c3 (8,388,608
+rain in a 3 km x 3 km x 1 km box at 10 km range, hi/lo positions, 202 gates),
+--workload c4 (64 independent domains of 2^16..2^21 rain, one persistent launch
+per round with HIP-graph replay, against the same domains launched one by one).
 """
 import argparse
 import json
@@ -50,11 +55,16 @@ def main():
     ap.add_argument("--e2e-steps", type=int, default=200)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--max-blocks", type=int, default=0)
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4"])
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build drives a single MI355X per process "
               "(multi-GPU sharding is a design note only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "c3":
+        return bench_c3(args)
+    if args.workload == "c4":
+        return bench_c4(args)
 
     import simradar_amd as sa
     from simradar_amd import workloads as wl
@@ -152,6 +162,109 @@ def main():
     }
     print(json.dumps(line))
     r.free()
+
+
+def _line(metric, value, unit, steps, warmup, ms, dtype, config, extra):
+    d = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+         "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+         "data": "synthetic (seeded LES / DSD tables)", "config": config}
+    d.update(extra)
+    print(json.dumps(d))
+
+
+def bench_c3(args):
+    """8,388,608 rain at 10 km range with hi/lo positions and binary64 phase."""
+    import simradar_amd as sa
+    from simradar_amd import workloads as wl
+    cfg = wl.c3()
+    print("bench: populating C3 (%d rain)" % cfg.n_rain, file=sys.stderr, flush=True)
+    r = wl.make(cfg)
+    r.set_position_precision(1)
+    R = int(r.params().range_count)
+    steps = min(args.steps, 2000) if args.steps != 1000 else 2000
+    warm = min(args.warmup, 50)
+
+    def step():
+        r.make_pulse()
+        r.advance_time()
+        r.advance_beam()
+    for _ in range(warm):
+        step()
+    r.sync()
+    r.profile(True)
+    r.timer_start()
+    for _ in range(steps):
+        step()
+    r.make_pulse()
+    ms = r.timer_stop() / (steps + 1)
+    k_ms, k_n = r.profile_read(3)
+    r.profile(False)
+    r.check()
+    kern = k_ms / max(1, k_n)
+    byts = 96 * cfg.n_rain            # pos + vel + lo read and written
+    ach = byts / (kern * 1e-3) / 1e9
+    _line("pulses/s & scatterer-updates/s (C3, hi/lo positions)", 1000.0 / ms, "pulses/s", steps + 1, warm, ms,
+          "f32 state, f32 hi/lo positions, f64 range-to-phase and gate sums",
+          {"workload": "C3: 8,388,608 rain, 3 km x 3 km x 1 km box at 10 km range, %d gates x 15 m" % R,
+           "scatterers": cfg.n_rain, "range_gates": R},
+          {"scatterer_updates_per_s": 1000.0 / ms * cfg.n_rain,
+           "roofline": {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": ach / PEAK_HBM_GBS, "traffic": None, "kernel": "k_rain<advance,pulse,hilo>",
+                        "kernel_avg_ms": kern, "bytes_per_launch": byts}})
+    r.free()
+
+
+def bench_c4(args):
+    """64 domains: one persistent launch per round (HIP graph) against the same
+    domains launched one by one, both timed over the same rounds (wall clock
+    around the rounds plus a final synchronisation)."""
+    import simradar_amd as sa
+    from simradar_amd import workloads as wl
+    cfgs = wl.c4(64)
+    total = sum(c.n_rain for c in cfgs)
+    print("bench: populating C4 (64 domains, %d rain)" % total, file=sys.stderr, flush=True)
+    rs = [wl.make(c) for c in cfgs]
+    rounds = 200 if args.steps == 1000 else args.steps
+
+    def plain_round():
+        for r in rs:
+            r.make_pulse()
+        for r in rs:
+            r.advance_time()
+            r.advance_beam()
+
+    def sync_all():
+        for r in rs:
+            r.sync()
+    for _ in range(10):
+        plain_round()
+    sync_all()
+    t0 = time.perf_counter()
+    for _ in range(rounds):
+        plain_round()
+    sync_all()
+    plain_s = (time.perf_counter() - t0) / rounds
+    b = sa.Batch(rs)
+    for _ in range(40):        # the autotuner's sweep (9 shapes x 3 rounds) and warm-up
+        b.round()
+    b.sync()
+    shape = b.launch_shape()
+    t0 = time.perf_counter()
+    for _ in range(rounds):
+        b.round()
+    b.sync()
+    batch_s = (time.perf_counter() - t0) / rounds
+    b.free()
+    for r in rs:
+        r.free()
+    _line("pulses/s & scatterer-updates/s (C4, 64 batched domains)", 64.0 / batch_s, "pulses/s", rounds, 50,
+          batch_s * 1000.0, "f32 state, f64 phase and gate sums",
+          {"workload": "C4: 64 domains, rain log-uniform 2^16..2^21 (%d in total), rain rates 1..100 mm/h, "
+                       "distinct vortex centres; one persistent launch + HIP graph per round" % total,
+           "domains": 64, "scatterers": total, "chunk": shape[0], "blocks_per_cu": shape[1]},
+          {"scatterer_updates_per_s": total / batch_s,
+           "plain_pulses_per_s": 64.0 / plain_s, "plain_ms_per_round": plain_s * 1000.0,
+           "batched_over_plain": plain_s / batch_s})
 
 
 if __name__ == "__main__":

@@ -322,6 +322,10 @@ int RS_get_pulse_f64(RSHandle *H, double *out);
 /* addition: radar moments (Z, V, ZDR, rho_hv) over the last `pulses` pulses,
  * computed on the device from its IQ history (no counterpart in the reference) */
 void RS_set_moment_pulses(RSHandle *H, int pulses);
+/* addition: one pulse round of many handles whose engines form the batch b
+ * (include/rsd.h rsd_batch_create): make_pulse of all, one device launch,
+ * advance_time + advance_beam of all; returns RSD_OK or the launch error */
+int RS_batch_round(RSHandle *const *handles, int count, rsd_batch *b);
 int RS_get_moments(RSHandle *H, int pulses, float *out);
 /* addition: descriptor accessors (which: 0 sim, 1 les, 2 range weight,
  * 3 angular weight, 4 drop rcs, 16+t adm table t, 32+t rcs table t);

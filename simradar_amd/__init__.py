@@ -71,7 +71,7 @@ EXPORTED_RS = [
     "RS_set_rcs_data_to_RCS_table", "RS_set_rcs_data_to_unity", "RS_clear_rcs_data",
     "RS_set_rcs_ellipsoid_table", "RS_revise_population", "RS_update_origins_offsets", "RS_merge_pulse_tmp",
     "RS_init_host_only", "RS_set_population", "RS_set_phase_precision", "RS_get_pulse_f64",
-    "RS_set_moment_pulses", "RS_get_moments", "RS_set_range_cap",
+    "RS_set_moment_pulses", "RS_get_moments", "RS_set_range_cap", "RS_batch_round",
     "RS_get_descriptor", "RS_radar_gain", "RS_engine", "RS_get_info", "RS_get_host_stream", "RS_get_params",
     "RS_get_dsd", "RS_suggest_scan_domain_values", "RS_set_scan_box_values", "RS_get_les_frame", "RS_get_debris_table",
 ]
@@ -202,6 +202,7 @@ def _declare(lib):
         "RS_get_pulse_f64": (I, [P, P]),
         "RS_set_moment_pulses": (None, [P, I]),
         "RS_set_range_cap": (None, [P, F]),
+        "RS_batch_round": (I, [P, I, P]),
         "RS_get_moments": (I, [P, I, P]),
         "RS_get_descriptor": (I, [P, I, P]),
         "RS_radar_gain": (F, [P]),
@@ -533,6 +534,7 @@ class Batch:
         if rc:
             raise RuntimeError("rsd_batch_create failed (%d)" % rc)
         self.b = b
+        self._handles = (ctypes.c_void_p * len(self.radars))(*[r.H for r in self.radars])
 
     def _chk(self, rc):
         if rc:
@@ -564,13 +566,9 @@ class Batch:
         return ms.value
 
     def round(self):
-        """One pulse of every domain: make_pulse, one launch, advance_time, advance_beam."""
-        for r in self.radars:
-            r.make_pulse()
-        self.launch()
-        for r in self.radars:
-            r.advance_time()
-            r.advance_beam()
+        """One pulse of every domain (RS_batch_round): make_pulse, one launch,
+        advance_time, advance_beam."""
+        self._chk(self.lib.RS_batch_round(self._handles, len(self.radars), self.b))
 
     def free(self):
         if self.b:

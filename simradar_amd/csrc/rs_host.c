@@ -1379,6 +1379,26 @@ void RS_make_pulse(RSHandle *H) {
     H->status &= ~RSStatusScattererSignalNeedsUpdate;
 }
 
+/* addition: one pulse round of a batch of handles whose engines form the
+ * batch b (rsd_batch_create): RS_make_pulse of every handle (recorded by its
+ * engine), one rsd_batch_launch, then RS_advance_time and RS_advance_beam of
+ * every handle -- the per-pulse loop of simradar.c:1011-1074 over many domains
+ * with one device launch. */
+int RS_batch_round(RSHandle *const *handles, int count, rsd_batch *b) {
+    for (int i = 0; i < count; i++) RS_make_pulse(handles[i]);
+    int st = rsd_batch_launch(b);
+    if (st != RSD_OK) {
+        rsprint("ERROR: batch launch failed (%d): %s", st, rsd_batch_last_error(b));
+        for (int i = 0; i < count; i++) handles[i]->last_error = st;
+        return st;
+    }
+    for (int i = 0; i < count; i++) {
+        RS_advance_time(handles[i]);
+        RS_advance_beam(handles[i]);
+    }
+    return RSD_OK;
+}
+
 int RS_get_pulse_f64(RSHandle *H, double *out) {
     if (!has_engine(H)) return 0;
     const uint32_t R = H->params.range_count;

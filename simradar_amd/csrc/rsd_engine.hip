@@ -1920,6 +1920,7 @@ extern "C" int rsd_set_position_precision(rsd_engine *e, int hilo) {
     HIPCHK(e, hipSetDevice(e->device));
     int st = flush_pending(e);
     if (st) return st;
+    join_batch(e);
     HIPCHK(e, hipStreamSynchronize(e->stream));
     e->pos_hilo = hilo;
     if (!e->pos) return RSD_OK;
@@ -1942,6 +1943,7 @@ extern "C" int rsd_download_pos_lo(rsd_engine *e, float *lo) {
     int st = flush_pending(e);
     if (st) return st;
     HIPCHK(e, hipSetDevice(e->device));
+    join_batch(e);
     const size_t b = e->n * sizeof(float4);
     if (!e->view) HIPCHK(e, hipMalloc((void **)&e->view, 6 * b));
     unsigned nb = (unsigned)((e->n + 255) / 256);
@@ -1958,6 +1960,9 @@ extern "C" int rsd_set_iq_ring(rsd_engine *e, int pulses, int range_count) {
     if (!e) return RSD_ERR_ARG;
     if (pulses < 0 || range_count < 0 || range_count > RSD_MAX_GATES) return fail(e, RSD_ERR_ARG, "bad IQ ring size");
     HIPCHK(e, hipSetDevice(e->device));
+    int st0 = flush_pending(e);
+    if (st0) return st0;
+    join_batch(e);
     HIPCHK(e, hipStreamSynchronize(e->stream));
     dfree(e->ring);
     dfree(e->mom_d);
@@ -1981,6 +1986,7 @@ extern "C" int rsd_moments(rsd_engine *e, int pulses, const rsd_moment_params *m
     if (pulses < 2 || pulses > e->ring_m) return fail(e, RSD_ERR_ARG, "moments over %d pulses: ring holds %d", pulses, e->ring_m);
     if (e->ring_count < pulses) return fail(e, RSD_ERR_STATE, "only %d pulses recorded", e->ring_count);
     HIPCHK(e, hipSetDevice(e->device));
+    join_batch(e);
     MomK a;
     a.ring = e->ring;
     a.ring_m = e->ring_m;
@@ -2018,7 +2024,6 @@ static int flush_pending(rsd_engine *e) {
         int st = rsd_batch_launch(e->batch);
         if (st) return fail(e, st, "batch launch failed");
     }
-    join_batch(e);
     if (!e->have_pending) return RSD_OK;
     e->have_pending = 0;
     return launch_step(e, &e->pending, NULL);
@@ -2056,6 +2061,7 @@ extern "C" int rsd_download_state(rsd_engine *e, float *pos, float *vel, float *
     if (!e || !e->pos) return fail(e, RSD_ERR_STATE, "state not allocated");
     int st = flush_pending(e);
     if (st) return st;
+    join_batch(e);
     HIPCHK(e, hipSetDevice(e->device));
     const size_t b = e->n * sizeof(float4);
     const bool perm = e->permuted && (vel || rnd);
@@ -2260,6 +2266,7 @@ extern "C" int rsd_sync(rsd_engine *e) {
     HIPCHK(e, hipSetDevice(e->device));
     int st = flush_pending(e);
     if (st) return st;
+    join_batch(e);
     HIPCHK(e, hipStreamSynchronize(e->stream));
     return RSD_OK;
 }
@@ -2423,7 +2430,15 @@ extern "C" int rsd_batch_create(rsd_engine *const *engines, int count, rsd_batch
     (void)hipEventRecord(b->ev_buf[0], b->stream);
     (void)hipEventRecord(b->ev_buf[1], b->stream);
     (void)hipEventRecord(b->ev_launch, b->stream);
-    for (int i = 0; i < count; i++) engines[i]->batch = b;
+    for (int i = 0; i < count; i++) {
+        rsd_engine *e = engines[i];
+        // wait for whatever the engine's stream still holds, then stagger the
+        // domains' range reorders over the sort interval
+        (void)hipEventRecord(e->ev_fork, e->stream);
+        (void)hipStreamWaitEvent(b->stream, e->ev_fork, 0);
+        if (e->sort_interval > 0 && e->permuted) e->since_sort = (int)((long)i * e->sort_interval / count);
+        e->batch = b;
+    }
     *out = b;
     return RSD_OK;
 }
@@ -2555,19 +2570,21 @@ extern "C" int rsd_batch_launch(rsd_batch *b) {
     }
     (void)mixed;
     if (ns == 0) return RSD_OK;
-    // range reorder of each domain that is due, on its own stream
+    // Range reorder of each domain that is due, on its own stream; only those
+    // streams have new work the batch must wait for (engine transfers are
+    // synchronous and fallback launches are not in the batch).
     for (int k = 0; k < ns; k++) {
         rsd_engine *e = sel[k];
-        join_batch(e);
         if (e->sort_interval > 0 && e->n_rain > 1 && (!e->permuted || e->since_sort >= e->sort_interval)) {
+            join_batch(e);
             int st = sort_rain(e, e->have_pending ? e->pending.sim_desc : e->pending_pulse.sim_desc,
                                e->pending_pulse.range_delta);
             if (st) return bfail(b, st, e->err);
+            if (hipEventRecord(e->ev_fork, e->stream) != hipSuccess ||
+                hipStreamWaitEvent(b->stream, e->ev_fork, 0) != hipSuccess)
+                return bfail(b, RSD_ERR_HIP, "stream join");
         }
         e->since_sort++;
-        if (hipEventRecord(e->ev_fork, e->stream) != hipSuccess ||
-            hipStreamWaitEvent(b->stream, e->ev_fork, 0) != hipSuccess)
-            return bfail(b, RSD_ERR_HIP, "stream join");
     }
     int chunk, bpc;
     batch_shape(b, &chunk, &bpc);
