@@ -219,6 +219,32 @@ int  rsd_batch_timer_start(rsd_batch *b);
 int  rsd_batch_timer_stop(rsd_batch *b, double *ms);
 const char *rsd_batch_last_error(const rsd_batch *b);
 
+/* Debris surrogate network (no counterpart in the reference; it replaces the
+ * ADM + RCS table lookups of rs.cl:195-359).  7 inputs (radar-relative
+ * orientation quaternion R = ori (x) o_conj(el, az), body-frame unit air-flow
+ * direction u) -> 4 hidden layers of 64 (tanh) -> 12 outputs (cd xyz, cm xyz,
+ * gamma-rotated hh, vv, hv as re, im), each output scaled and shifted.  BF16
+ * weights and activations, FP32 accumulation, on MFMA; FP32 master weights.
+ * Parameters, flat (RSD_MLP_PARAMS floats): W0[64][7], b0[64], W1[64][64], b1,
+ * W2, b2, W3, b3, W4[12][64], b4[12] (row-major [out][in]).
+ * rsd_mlp_gradient: loss = mean over the n x 12 outputs of (y - t)^2 (outputs
+ * before scaling, t normalised), gradient in the same flat layout; n a
+ * multiple of 256.  rsd_mlp_set_data uploads a training set (x n x 7, t n x 12
+ * normalised); rsd_mlp_train runs Adam (cosine-decayed lr) over it. */
+#define RSD_MLP_PARAMS 13772
+typedef struct rsd_mlp rsd_mlp;
+int  rsd_mlp_create(int device_index, uint64_t seed, rsd_mlp **out);
+void rsd_mlp_destroy(rsd_mlp *m);
+int  rsd_mlp_set_params(rsd_mlp *m, const float *params);
+int  rsd_mlp_get_params(rsd_mlp *m, float *params);
+int  rsd_mlp_set_output_scaling(rsd_mlp *m, const float *scale12, const float *shift12);
+int  rsd_mlp_forward(rsd_mlp *m, const float *x, float *y, size_t n);
+int  rsd_mlp_gradient(rsd_mlp *m, const float *x, const float *t, size_t n, float *grad, double *loss);
+int  rsd_mlp_set_data(rsd_mlp *m, const float *x, const float *t, size_t n);
+int  rsd_mlp_train(rsd_mlp *m, int epochs, int batch, float lr, double *last_loss);
+int  rsd_mlp_sync(rsd_mlp *m);
+const char *rsd_mlp_last_error(const rsd_mlp *m);
+
 /* Self-test: evaluate one of the device's binary32 functions on n inputs
  * (fn: 0 log1p(a), 1 log(a), 2 sin(a), 3 cos(a), 4 atan2(a, b), 5 acos(a),
  * 6 sqrt(a), 7 a / b, 8 sin(2 pi a), 9 cos(2 pi a), 10 the pulse path's fast acos(a)).

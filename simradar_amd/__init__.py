@@ -84,6 +84,8 @@ EXPORTED_RSD = [
     "rsd_set_position_precision", "rsd_download_pos_lo", "rsd_batch_create", "rsd_batch_destroy",
     "rsd_batch_launch", "rsd_batch_sync", "rsd_batch_set_launch", "rsd_batch_get_launch", "rsd_batch_set_graph",
     "rsd_batch_timer_start", "rsd_batch_timer_stop", "rsd_batch_last_error",
+    "rsd_mlp_create", "rsd_mlp_destroy", "rsd_mlp_set_params", "rsd_mlp_get_params", "rsd_mlp_set_output_scaling",
+    "rsd_mlp_forward", "rsd_mlp_gradient", "rsd_mlp_set_data", "rsd_mlp_train", "rsd_mlp_sync", "rsd_mlp_last_error",
 ]
 
 
@@ -252,6 +254,17 @@ def _declare(lib):
         "rsd_batch_timer_start": (I, [P]),
         "rsd_batch_timer_stop": (I, [P, ctypes.POINTER(ctypes.c_double)]),
         "rsd_batch_last_error": (ctypes.c_char_p, [P]),
+        "rsd_mlp_create": (I, [I, ctypes.c_uint64, P]),
+        "rsd_mlp_destroy": (None, [P]),
+        "rsd_mlp_set_params": (I, [P, P]),
+        "rsd_mlp_get_params": (I, [P, P]),
+        "rsd_mlp_set_output_scaling": (I, [P, P, P]),
+        "rsd_mlp_forward": (I, [P, P, P, S]),
+        "rsd_mlp_gradient": (I, [P, P, P, S, P, ctypes.POINTER(ctypes.c_double)]),
+        "rsd_mlp_set_data": (I, [P, P, P, S]),
+        "rsd_mlp_train": (I, [P, I, I, F, ctypes.POINTER(ctypes.c_double)]),
+        "rsd_mlp_sync": (I, [P]),
+        "rsd_mlp_last_error": (ctypes.c_char_p, [P]),
         "rsd_math_probe": (I, [I, I, P, P, P, S]),
         "rsd_profile": (I, [P, I]),
         "rsd_io_test": (I, [P, S, I, ctypes.POINTER(ctypes.c_double)]),

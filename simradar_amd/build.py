@@ -31,7 +31,7 @@ HIP_FLAGS = [
 C_FLAGS = ["-O2", "-std=gnu11", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
            "-Wall", "-Wextra", "-Wno-unused-parameter", "-I" + INCLUDE]
 
-HIP_SOURCES = ["rsd_engine.hip"]
+HIP_SOURCES = ["rsd_engine.hip", "rsd_mlp.hip"]
 C_SOURCES = ["rs_host.c", "rs_providers.c"]
 
 

@@ -1,0 +1,174 @@
+// rsd_mlp.h -- debris surrogate network: shapes, parameter images and the
+// wave-level BF16 MFMA forward tile shared by the engine's debris kernel and the
+// training kernels.
+//
+// Network: 7 inputs -> 4 hidden layers of 64 (tanh) -> 12 outputs (linear),
+// i.e. 7 -> 64 -> 64 -> 64 -> 64 -> 12.  Inputs: the radar-relative orientation
+// quaternion R = ori (x) o_conj(el, az) (4) and the body-frame unit air-flow
+// direction u = rot(normalize(v_bg - v), conj(ori)) (3).  Outputs (before the
+// per-output scale and shift): drag coefficients cd (3) and moment
+// coefficients cm (3) of the ADM table at u, and the gamma-rotated scattering
+// amplitudes hh, vv, hv (re, im) of the RCS table at R -- the quantities the
+// table path reads (rs.cl:207-243, 287-356) after its lookups.
+//
+// Device layout (the "transposed" GEMM form): a layer computes Z = W X for 16
+// samples at a time with v_mfma_f32_16x16x16_bf16, A = a 16x16 tile of W
+// (lane l holds W[16 m + l % 16][16 k + 4 (l / 16) + 0..3]), B = the layer input
+// (lane l holds features 16 k + 4 (l / 16) + 0..3 of sample l % 16) and D in
+// the same lane layout as B (features 16 m + 4 (l / 16) + 0..3 of sample
+// l % 16).  The output of one layer is therefore, tile for tile, the B operand
+// of the next: activations never leave the registers.
+#ifndef RSD_MLP_H
+#define RSD_MLP_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define MLP_IN 7
+#define MLP_OUT 12
+#define MLP_H 64
+#define MLP_K0 16                   // padded input width
+#define MLP_N4 16                   // padded output width
+
+// bf16 weight image (row-major [out][in]): W0 64x16, W1..W3 64x64, W4 16x64
+#define MLP_W0 0
+#define MLP_W1 (MLP_W0 + MLP_H * MLP_K0)
+#define MLP_W2 (MLP_W1 + MLP_H * MLP_H)
+#define MLP_W3 (MLP_W2 + MLP_H * MLP_H)
+#define MLP_W4 (MLP_W3 + MLP_H * MLP_H)
+#define MLP_WN (MLP_W4 + MLP_N4 * MLP_H)            // 14336 bf16
+// transposed image ([in][out]) of W1..W4 for the backward pass
+#define MLP_T1 0
+#define MLP_T2 (MLP_T1 + MLP_H * MLP_H)
+#define MLP_T3 (MLP_T2 + MLP_H * MLP_H)
+#define MLP_T4 (MLP_T3 + MLP_H * MLP_H)
+#define MLP_TN (MLP_T4 + MLP_H * MLP_N4)            // 13312 bf16
+// float biases: b0..b3 (64 each), b4 (16), then output scale (16) and shift (16)
+#define MLP_B0 0
+#define MLP_B4 (4 * MLP_H)
+#define MLP_SCALE (MLP_B4 + MLP_N4)
+#define MLP_SHIFT (MLP_SCALE + MLP_N4)
+#define MLP_BN (MLP_SHIFT + MLP_N4)                 // 304 floats
+
+typedef short mlp_s4 __attribute__((ext_vector_type(4)));
+typedef float mlp_f4 __attribute__((ext_vector_type(4)));
+
+// Device-side view of a network.
+struct MlpDev {
+    const uint16_t *w;      // MLP_WN bf16
+    const uint16_t *wt;     // MLP_TN bf16
+    const float *b;         // MLP_BN floats
+};
+
+__device__ __forceinline__ uint16_t mlp_bf16(float f) {   // round to nearest even
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ float mlp_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ __forceinline__ mlp_s4 mlp_pack(mlp_f4 v) {
+    mlp_s4 r;
+    r.x = (short)mlp_bf16(v.x);
+    r.y = (short)mlp_bf16(v.y);
+    r.z = (short)mlp_bf16(v.z);
+    r.w = (short)mlp_bf16(v.w);
+    return r;
+}
+
+__device__ __forceinline__ mlp_f4 mlp_mfma(mlp_s4 a, mlp_s4 b, mlp_f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ float mlp_tanh(float x) {
+    const float e = __expf(2.0f * x);
+    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+}
+
+__device__ __forceinline__ mlp_s4 mlp_ld4(const uint16_t *p) {  // 4 consecutive bf16 (8-byte aligned)
+    return *(const mlp_s4 *)p;
+}
+
+// Forward of one 16-sample tile.  x: layer-0 B operand (lanes with l / 16 >= 2
+// hold zeros); w, b: weight and bias images (LDS or global).  Returns the
+// output tile in D layout (unscaled); h[4][4] receives the hidden activations
+// (bf16, B layout, per layer and K step) when non-null.
+__device__ __forceinline__ mlp_f4 mlp_tile_forward(const uint16_t *w, const float *b, mlp_s4 x, mlp_s4 (*h)[4]) {
+    const int l = threadIdx.x & 63, row = l & 15, grp = l >> 4;
+    mlp_s4 cur[4], nxt[4];
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+        mlp_f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        acc = mlp_mfma(mlp_ld4(w + MLP_W0 + (16 * mt + row) * MLP_K0 + 4 * grp), x, acc);
+#pragma unroll
+        for (int r = 0; r < 4; r++) acc[r] = mlp_tanh(acc[r] + b[MLP_B0 + 16 * mt + 4 * grp + r]);
+        cur[mt] = mlp_pack(acc);
+    }
+    if (h) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) h[0][k] = cur[k];
+    }
+#pragma unroll
+    for (int L = 1; L < 4; L++) {
+        const uint16_t *W = w + MLP_W1 + (L - 1) * MLP_H * MLP_H;
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++) {
+            mlp_f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int kt = 0; kt < 4; kt++) acc = mlp_mfma(mlp_ld4(W + (16 * mt + row) * MLP_H + 16 * kt + 4 * grp), cur[kt], acc);
+#pragma unroll
+            for (int r = 0; r < 4; r++) acc[r] = mlp_tanh(acc[r] + b[MLP_H * L + 16 * mt + 4 * grp + r]);
+            nxt[mt] = mlp_pack(acc);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) cur[k] = nxt[k];
+        if (h) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) h[L][k] = cur[k];
+        }
+    }
+    mlp_f4 y = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kt = 0; kt < 4; kt++) y = mlp_mfma(mlp_ld4(w + MLP_W4 + row * MLP_H + 16 * kt + 4 * grp), cur[kt], y);
+#pragma unroll
+    for (int r = 0; r < 4; r++) y[r] += b[MLP_B4 + 4 * grp + r];
+    return y;
+}
+
+// Layer-0 B operand of tile t (samples 16 t .. 16 t + 15 = lanes of the wave)
+// from every lane's own 8 input features f[0..7] (f[7] = 0).
+__device__ __forceinline__ mlp_s4 mlp_gather_input(const float f[8], int t) {
+    const int l = threadIdx.x & 63, grp = l >> 4;
+    const int src = 16 * t + (l & 15);
+    float v[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) v[k] = __shfl(f[k], src, 64);
+    mlp_f4 a;
+    a.x = grp == 0 ? v[0] : (grp == 1 ? v[4] : 0.0f);
+    a.y = grp == 0 ? v[1] : (grp == 1 ? v[5] : 0.0f);
+    a.z = grp == 0 ? v[2] : (grp == 1 ? v[6] : 0.0f);
+    a.w = grp == 0 ? v[3] : (grp == 1 ? v[7] : 0.0f);
+    return mlp_pack(a);
+}
+
+// Network outputs of every lane's own sample: the four tiles of the wave are
+// evaluated and their D tiles exchanged through scratch (64 x 16 floats of LDS
+// per wave); out[0..11] = scale * y + shift.
+__device__ __forceinline__ void mlp_wave_eval(const uint16_t *w, const float *b, const float f[8], float *scratch,
+                                              float out[MLP_OUT]) {
+    const int l = threadIdx.x & 63, row = l & 15, grp = l >> 4;
+#pragma unroll 1
+    for (int t = 0; t < 4; t++) {
+        const mlp_s4 x = mlp_gather_input(f, t);
+        const mlp_f4 y = mlp_tile_forward(w, b, x, nullptr);
+        *(mlp_f4 *)(scratch + (16 * t + row) * MLP_N4 + 4 * grp) = y;
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+#pragma unroll
+    for (int k = 0; k < MLP_OUT; k++) out[k] = scratch[l * MLP_N4 + k] * b[MLP_SCALE + k] + b[MLP_SHIFT + k];
+    __builtin_amdgcn_wave_barrier();
+}
+
+#endif
