@@ -222,7 +222,7 @@ const char *rsd_batch_last_error(const rsd_batch *b);
 /* Debris surrogate network (no counterpart in the reference; it replaces the
  * ADM + RCS table lookups of rs.cl:195-359).  7 inputs (radar-relative
  * orientation quaternion R = ori (x) o_conj(el, az), body-frame unit air-flow
- * direction u) -> 4 hidden layers of 64 (tanh) -> 12 outputs (cd xyz, cm xyz,
+ * direction u) -> 4 hidden layers of 64 (ReLU) -> 12 outputs (cd xyz, cm xyz,
  * gamma-rotated hh, vv, hv as re, im), each output scaled and shifted.  BF16
  * weights and activations, FP32 accumulation, on MFMA; FP32 master weights.
  * Parameters, flat (RSD_MLP_PARAMS floats): W0[64][7], b0[64], W1[64][64], b1,
@@ -244,6 +244,16 @@ int  rsd_mlp_set_data(rsd_mlp *m, const float *x, const float *t, size_t n);
 int  rsd_mlp_train(rsd_mlp *m, int epochs, int batch, float lr, double *last_loss);
 int  rsd_mlp_sync(rsd_mlp *m);
 const char *rsd_mlp_last_error(const rsd_mlp *m);
+
+/* Training samples from the table path: n random orientations R (uniform on
+ * SO(3), R.w >= 0) and flow directions u (uniform on the sphere) with the
+ * network's 12 targets read from ADM table adm_table and RCS table rcs_table
+ * exactly as the debris kernels read them; x[n][7] = (R, u), t[n][12]. */
+int rsd_debris_samples(rsd_engine *e, int adm_table, int rcs_table, size_t n, uint64_t seed, float *x, float *t);
+/* Use network m (NULL: the tables) for debris type `type` (>= 1) of engine e:
+ * the debris kernel evaluates it once per scatterer and pulse in place of the
+ * ADM and RCS lookups.  m must stay alive while it is in use. */
+int rsd_set_debris_surrogate(rsd_engine *e, int type, const rsd_mlp *m);
 
 /* Self-test: evaluate one of the device's binary32 functions on n inputs
  * (fn: 0 log1p(a), 1 log(a), 2 sin(a), 3 cos(a), 4 atan2(a, b), 5 acos(a),

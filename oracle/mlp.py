@@ -2,7 +2,7 @@
 the smoke check and bench.py's CPU leg use it).
 
 The network of include/rsd.h (rsd_mlp_*): 7 -> 64 -> 64 -> 64 -> 64 -> 12,
-tanh hidden layers, linear output; loss = mean over n x 12 of (y - t)^2.
+ReLU hidden layers, linear output; loss = mean over n x 12 of (y - t)^2.
 Forward and backward in binary64 with numpy.  With bf16=True the values the
 device stores in bfloat16 are rounded at the same points (inputs, weights,
 hidden activations, the output error and every dZ), so the device's result
@@ -44,7 +44,7 @@ def forward(p, x, bf=False):
     xs = [a]
     for L in range(4):
         z = a @ r(W[2 * L]).T + W[2 * L + 1]
-        a = r(np.tanh(z))
+        a = r(np.maximum(z, 0.0))
         xs.append(a)
     y = a @ r(W[8]).T + W[9]
     return y, xs
@@ -64,5 +64,5 @@ def loss_grad(p, x, t, bf=False):
         grads[2 * L + 1] = dz.sum(0)
         if L > 0:
             dh = dz @ r(W[2 * L])
-            dz = r(dh * (1.0 - xs[L] * xs[L]))
+            dz = r(dh * (xs[L] > 0.0))
     return loss, flatten(grads)

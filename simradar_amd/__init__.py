@@ -86,6 +86,7 @@ EXPORTED_RSD = [
     "rsd_batch_timer_start", "rsd_batch_timer_stop", "rsd_batch_last_error",
     "rsd_mlp_create", "rsd_mlp_destroy", "rsd_mlp_set_params", "rsd_mlp_get_params", "rsd_mlp_set_output_scaling",
     "rsd_mlp_forward", "rsd_mlp_gradient", "rsd_mlp_set_data", "rsd_mlp_train", "rsd_mlp_sync", "rsd_mlp_last_error",
+    "rsd_debris_samples", "rsd_set_debris_surrogate",
 ]
 
 
@@ -265,6 +266,8 @@ def _declare(lib):
         "rsd_mlp_train": (I, [P, I, I, F, ctypes.POINTER(ctypes.c_double)]),
         "rsd_mlp_sync": (I, [P]),
         "rsd_mlp_last_error": (ctypes.c_char_p, [P]),
+        "rsd_debris_samples": (I, [P, I, I, S, ctypes.c_uint64, P, P]),
+        "rsd_set_debris_surrogate": (I, [P, I, P]),
         "rsd_math_probe": (I, [I, I, P, P, P, S]),
         "rsd_profile": (I, [P, I]),
         "rsd_io_test": (I, [P, S, I, ctypes.POINTER(ctypes.c_double)]),
@@ -493,6 +496,12 @@ class Radar:
         if self.lib.rsd_download_pos_lo(e, _ptr(out)):
             raise RuntimeError(self.lib.rsd_last_error(e).decode())
         return out
+
+    def set_debris_surrogate(self, debris_type, net):
+        """Use a surrogate network (surrogate.Mlp, or None for the tables) for a debris type."""
+        e = self.engine()
+        if self.lib.rsd_set_debris_surrogate(e, debris_type, net.m if net is not None else None):
+            raise RuntimeError(self.lib.rsd_last_error(e).decode())
 
     def set_moment_pulses(self, pulses):
         """Keep the last `pulses` pulses on the device for the moment estimator."""

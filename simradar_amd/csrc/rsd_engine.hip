@@ -44,6 +44,8 @@
 
 #include "rsd.h"
 #include "rsd_fmath.h"
+#include "rsd_mlp.h"
+#include "rsd_mlp_host.h"
 
 #define RSD_MIN_HEIGHT 2.0f
 #define RSD_PI_F 0x1.921fb6p+1f
@@ -207,6 +209,73 @@ __device__ float4 debris_rcs(float4 pos, float4 ori, const Tex2 &rr, const Tex2 
         ss.y += vh_i;
         ss.z += hv_r;
         ss.w += hv_i;
+    }
+    return ss;
+}
+
+// Radar-relative orientation R = ori (x) o_conj(el, az) of rs.cl:287-315, with
+// the sign fixed so that R.w >= 0 (R and -R are the same rotation).
+__device__ __forceinline__ float4 radar_rel_quat(float4 pos, float4 ori) {
+    float el = rf_atan2f(pos.z, rf_len2(pos.x, pos.y));
+    float az = rf_atan2f(pos.x, pos.y);
+    float se, ce, sa, ca;
+    rf_sincosf(0.5f * (el + RSD_PI_2_F), &se, &ce);
+    rf_sincosf(0.5f * az, &sa, &ca);
+    float4 oc = make_float4((-se * ca + ce * sa) * RSD_SQRT1_2_F, (ce * ca + se * sa) * RSD_SQRT1_2_F,
+                            (se * ca + ce * sa) * RSD_SQRT1_2_F, (ce * ca + (-se) * sa) * RSD_SQRT1_2_F);
+    float4 R = qmul(ori, oc);
+    if (R.w < 0.0f) R = make_float4(-R.x, -R.y, -R.z, -R.w);
+    return R;
+}
+
+// The quantities the surrogate network learns, from the tables (rs.cl:207-243,
+// 316-347): cd, cm at the body-frame flow direction u, and the gamma-rotated
+// amplitudes hh, vv, hv (re, im) at the radar-relative orientation R.
+__device__ __forceinline__ void table_targets(float4 R, float4 uh, const Tex2 &cdt, const Tex2 &cmt, const F16 &ad,
+                                              const Tex2 &rr, const Tex2 &ri, const F16 &d, float t[12]) {
+    float beta = rf_acosf(uh.x);
+    float alpha = rf_atan2f(uh.z, uh.y);
+    if (alpha < 0.0f) {
+        alpha = RSD_PI_F + alpha;
+        beta = -beta;
+    }
+    float4 cd = sample2d(cdt, rf_fma(beta, ad.s[0], ad.s[4]), rf_fma(alpha, ad.s[1], ad.s[5]));
+    float4 cm = sample2d(cmt, rf_fma(beta, ad.s[0], ad.s[4]), rf_fma(alpha, ad.s[1], ad.s[5]));
+    float4 q = make_float4(R.x, R.z, -R.y, R.w);
+    float a, b, g;
+    float beta_arg = ((q.w * q.w + q.z * q.z) - q.y * q.y) - q.x * q.x;
+    if (beta_arg > 0.999847f || beta_arg < -0.999847f) {
+        a = 0.0f;
+        b = 0.0f;
+        g = (rf_sign(q.z) * rf_acosf(q.w)) * 2.0f;
+    } else {
+        a = rf_atan2f(q.y * q.z - q.w * q.x, q.x * q.z + q.w * q.y);
+        b = rf_acosf(beta_arg);
+        g = rf_atan2f(q.y * q.z + q.w * q.x, q.w * q.y - q.x * q.z);
+    }
+    float4 re = sample2d(rr, rf_fma(a, d.s[0], d.s[4]), rf_fma(b, d.s[1], d.s[5]));
+    float4 im = sample2d(ri, rf_fma(a, d.s[0], d.s[4]), rf_fma(b, d.s[1], d.s[5]));
+    float sg, cg;
+    rf_sincosf(g, &sg, &cg);
+    t[0] = cd.x; t[1] = cd.y; t[2] = cd.z;
+    t[3] = cm.x; t[4] = cm.y; t[5] = cm.z;
+    t[6] = cg * (cg * re.x - re.z * sg) - sg * (cg * re.z - re.y * sg);
+    t[7] = cg * (cg * im.x - im.z * sg) - sg * (cg * im.z - im.y * sg);
+    t[8] = cg * (cg * re.y + re.z * sg) + sg * (cg * re.z + re.x * sg);
+    t[9] = cg * (cg * im.y + im.z * sg) + sg * (cg * im.z + im.x * sg);
+    t[10] = cg * (cg * re.z + re.x * sg) - sg * (cg * re.y + re.z * sg);
+    t[11] = cg * (cg * im.z + im.x * sg) - sg * (cg * im.y + im.z * sg);
+}
+
+// Debris RCS from the network's amplitudes (rs.cl:340-356 after the lookups;
+// the cross-polar term vh equals hv for the symmetric scattering matrix).
+__device__ __forceinline__ float4 rcs_from_amps(const float *t, uint32_t concept) {
+    float4 ss = make_float4(t[6], t[7], t[8], t[9]);
+    if (concept & 1u) {
+        ss.x += t[10];
+        ss.y += t[11];
+        ss.z += t[10];
+        ss.w += t[11];
     }
     return ss;
 }
@@ -838,13 +907,16 @@ struct DebrisK {
     uint32_t pulse_concept;
     uint32_t reuse_rcs;     // advance and refresh use the same table and concept
     const float *logtab;
+    MlpDev net;             // surrogate network of this type (k_debris<.., true>)
     PulseK pk;
 };
 
-__device__ bool debris_advance(const DebrisK &a, size_t i, float4 &pos, float4 &ori, float4 &vel, float4 &tum, float4 &rcs, const float *logtab) {
+// Position / orientation update and respawn of db_atts (rs.cl:775-873);
+// returns true on respawn (state reset, rcs 0).
+__device__ __forceinline__ bool debris_move(const DebrisK &a, size_t i, float4 &pos, float4 &ori, float4 &vel, float4 &tum,
+                                            float4 &rcs) {
     const F16 &sd = a.sd;
     const float dt = sd.s[11];
-    const uint32_t concept = __float_as_uint(sd.s[5]);
     ori = norm4(qmul(ori, tum));
     pos.x = pos.x + vel.x * dt;
     pos.y = pos.y + vel.y * dt;
@@ -884,21 +956,17 @@ __device__ bool debris_advance(const DebrisK &a, size_t i, float4 &pos, float4 &
         a.rnd[i] = seed;
         return true;
     }
-    float4 c = wind_index(pos, a.les, sd, logtab);
-    float4 bg = sample3d(a.wind, c.x, c.y, c.z);
-    float4 ur = make_float4(bg.x - vel.x, bg.y - vel.y, bg.z - vel.z, bg.w - vel.w);
-    float4 uh = qrot(norm4(ur), qconj(ori));
-    float beta = rf_acosf(uh.x);
-    float alpha = rf_atan2f(uh.z, uh.y);
-    if (alpha < 0.0f) {
-        alpha = RSD_PI_F + alpha;
-        beta = -beta;
-    }
+    return false;
+}
+
+// Velocity and tumble update of db_atts (rs.cl:236-261, 876-897) from the
+// drag / moment coefficients cd (body frame) and cm.
+__device__ __forceinline__ void debris_forces(const DebrisK &a, float4 ori, float4 &vel, float4 &tum, float4 bg, float4 ur,
+                                              float4 cd, float4 cm) {
+    const F16 &sd = a.sd;
+    const float dt = sd.s[11];
+    const uint32_t concept = __float_as_uint(sd.s[5]);
     const F16 &ad = a.adm_desc;
-    float cu = rf_fma(beta, ad.s[0], ad.s[4]);
-    float cv = rf_fma(alpha, ad.s[1], ad.s[5]);
-    float4 cd = sample2d(a.adm_cd, cu, cv);
-    float4 cm = sample2d(a.adm_cm, cu, cv);
     const float Ta = ad.s[15];
     cd = qrot(cd, ori);
     float urn = (ur.x * ur.x + ur.y * ur.y) + ur.z * ur.z;
@@ -926,14 +994,45 @@ __device__ bool debris_advance(const DebrisK &a, size_t i, float4 &pos, float4 &
     float4 tq = make_float4((cx * sy) * sz + (sx * cy) * cz, (cx * sy) * cz - (sx * cy) * sz,
                             (cx * cy) * sz + (sx * sy) * cz, (cx * cy) * cz - (sx * sy) * sz);
     tum = norm4(tq);
+}
+
+__device__ bool debris_advance(const DebrisK &a, size_t i, float4 &pos, float4 &ori, float4 &vel, float4 &tum, float4 &rcs, const float *logtab) {
+    const F16 &sd = a.sd;
+    const uint32_t concept = __float_as_uint(sd.s[5]);
+    if (debris_move(a, i, pos, ori, vel, tum, rcs)) return true;
+    float4 c = wind_index(pos, a.les, sd, logtab);
+    float4 bg = sample3d(a.wind, c.x, c.y, c.z);
+    float4 ur = make_float4(bg.x - vel.x, bg.y - vel.y, bg.z - vel.z, bg.w - vel.w);
+    float4 uh = qrot(norm4(ur), qconj(ori));
+    float beta = rf_acosf(uh.x);
+    float alpha = rf_atan2f(uh.z, uh.y);
+    if (alpha < 0.0f) {
+        alpha = RSD_PI_F + alpha;
+        beta = -beta;
+    }
+    const F16 &ad = a.adm_desc;
+    float cu = rf_fma(beta, ad.s[0], ad.s[4]);
+    float cv = rf_fma(alpha, ad.s[1], ad.s[5]);
+    float4 cd = sample2d(a.adm_cd, cu, cv);
+    float4 cm = sample2d(a.adm_cm, cu, cv);
+    debris_forces(a, ori, vel, tum, bg, ur, cd, cm);
     rcs = debris_rcs(pos, ori, a.rcs_re, a.rcs_im, a.rcs_desc, concept);
     return false;
 }
 
-template <bool ADV, bool PULSE>
+// SURR: the surrogate network replaces the ADM and RCS table lookups; every
+// wave evaluates it for its 64 lanes at once (BF16 MFMA, weights in LDS).
+template <bool ADV, bool PULSE, bool SURR>
 __global__ __launch_bounds__(256) void k_debris(DebrisK a) {
     extern __shared__ double acc[];
     __shared__ float logtab[3 * RF_LOG_TABLE];
+    __shared__ uint16_t nw[SURR ? MLP_WN : 2];
+    __shared__ float nb[SURR ? MLP_BN : 2];
+    __shared__ float nscr[SURR ? 4 * 64 * MLP_N4 : 2];
+    if (SURR) {
+        for (int j = threadIdx.x; j < MLP_WN; j += blockDim.x) nw[j] = a.net.w[j];
+        for (int j = threadIdx.x; j < MLP_BN; j += blockDim.x) nb[j] = a.net.b[j];
+    }
     if (ADV) logtab_load(logtab, a.logtab);
     if (PULSE) acc_clear(acc, 4 * a.pk.R);
     else __syncthreads();
@@ -957,6 +1056,55 @@ __global__ __launch_bounds__(256) void k_debris(DebrisK a) {
         }
         float4 rcs;
         bool rcs_known = false, respawned = false;
+        if (SURR) {
+            float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f), t = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+            float4 bg = v, ur = v;
+            if (ADV && live) {
+                v = a.vel[i];
+                t = a.tum[i];
+                respawned = debris_move(a, i, p, o, v, t, rcs);
+                if (!respawned) {
+                    float4 c = wind_index(p, a.les, a.sd, logtab);
+                    bg = sample3d(a.wind, c.x, c.y, c.z);
+                    ur = make_float4(bg.x - v.x, bg.y - v.y, bg.z - v.z, bg.w - v.w);
+                }
+            }
+            const bool need_adv = ADV && live && !respawned;
+            const bool need_ref = PULSE && live && a.refresh && !(need_adv && a.reuse_rcs);
+            float f[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            if (need_adv || need_ref) {
+                float4 R = radar_rel_quat(p, o);
+                float4 uh = need_adv ? qrot(norm4(ur), qconj(o)) : make_float4(1.0f, 0.0f, 0.0f, 0.0f);
+                f[0] = R.x; f[1] = R.y; f[2] = R.z; f[3] = R.w;
+                f[4] = uh.x; f[5] = uh.y; f[6] = uh.z;
+                // still air relative to the body (u_r = 0) has no flow direction:
+                // the table path then reads texel 0, the network gets u = 0
+#pragma unroll
+                for (int k = 4; k < 7; k++) f[k] = f[k] == f[k] ? f[k] : 0.0f;
+            }
+            float out[MLP_OUT];
+            mlp_wave_eval(nw, nb, f, nscr + (threadIdx.x >> 6) * 64 * MLP_N4, out);
+            if (ADV && live) {
+                if (need_adv) {
+                    debris_forces(a, o, v, t, bg, ur, make_float4(out[0], out[1], out[2], 0.0f),
+                                  make_float4(out[3], out[4], out[5], 0.0f));
+                    rcs = rcs_from_amps(out, __float_as_uint(a.sd.s[5]));
+                }
+                a.pos[i] = p;
+                a.ori[i] = o;
+                a.vel[i] = v;
+                a.tum[i] = t;
+                rcs_known = true;
+            }
+            if (PULSE && live) {
+                if (need_ref) {
+                    rcs = rcs_from_amps(out, a.pulse_concept);
+                    rcs_known = true;
+                } else if (!rcs_known) {
+                    rcs = a.rcs[i];
+                }
+            }
+        } else {
         if (ADV && live) {
             float4 v = a.vel[i], t = a.tum[i];
             respawned = debris_advance(a, i, p, o, v, t, rcs, logtab);
@@ -975,6 +1123,7 @@ __global__ __launch_bounds__(256) void k_debris(DebrisK a) {
             } else if (!rcs_known) {
                 rcs = a.rcs[i];
             }
+        }
         }
         if (rcs_known && live) a.rcs[i] = rcs;
         if (PULSE) {
@@ -1275,6 +1424,9 @@ struct rsd_engine {
     int sort_bits;          // key bits of the current sort
     float4 *plo, *plo2;     // hi/lo positions: low parts (and ping-pong copy)
     int pos_hilo;           // 1: positions carried as hi/lo pairs
+    // surrogate networks per debris type (k_debris<.., true>)
+    MlpDev net[RSD_MAX_TYPES];
+    int surr[RSD_MAX_TYPES];
     // batch membership: pulses are held back and launched by the batch
     struct rsd_batch *batch;
     int have_pending_pulse;
@@ -1762,9 +1914,16 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p) {
             if (st) return st;
             a.pk = pk;
             a.pk.row0 = row;
-            if (s && p) hipLaunchKernelGGL((k_debris<true, true>), dim3(deb_blocks[k]), dim3(e->block), lds, e->side, a);
-            else if (s) hipLaunchKernelGGL((k_debris<true, false>), dim3(deb_blocks[k]), dim3(e->block), 0, e->side, a);
-            else hipLaunchKernelGGL((k_debris<false, true>), dim3(deb_blocks[k]), dim3(e->block), lds, e->side, a);
+            if (e->surr[k]) {
+                a.net = e->net[k];
+                if (s && p) hipLaunchKernelGGL((k_debris<true, true, true>), dim3(deb_blocks[k]), dim3(256), lds, e->side, a);
+                else if (s) hipLaunchKernelGGL((k_debris<true, false, true>), dim3(deb_blocks[k]), dim3(256), 0, e->side, a);
+                else hipLaunchKernelGGL((k_debris<false, true, true>), dim3(deb_blocks[k]), dim3(256), lds, e->side, a);
+            } else {
+                if (s && p) hipLaunchKernelGGL((k_debris<true, true, false>), dim3(deb_blocks[k]), dim3(e->block), lds, e->side, a);
+                else if (s) hipLaunchKernelGGL((k_debris<true, false, false>), dim3(deb_blocks[k]), dim3(e->block), 0, e->side, a);
+                else hipLaunchKernelGGL((k_debris<false, true, false>), dim3(deb_blocks[k]), dim3(e->block), lds, e->side, a);
+            }
             HIPCHK(e, hipGetLastError());
             row += deb_blocks[k];
         }
@@ -2780,5 +2939,96 @@ extern "C" int rsd_batch_timer_stop(rsd_batch *b, double *ms) {
         hipEventElapsedTime(&f, b->t0, b->t1) != hipSuccess)
         return RSD_ERR_HIP;
     *ms = f;
+    return RSD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Training samples of the surrogate network from the table path.
+// ---------------------------------------------------------------------------
+struct SampK {
+    Tex2 cd, cm, rr, ri;
+    F16 ad, rd;
+    unsigned long long n, seed;
+    float *x, *t;           // [n][7], [n][12]
+};
+
+__device__ __forceinline__ float samp_uniform(uint64_t &s) {
+    s += 0x9E3779B97F4A7C15ull;
+    uint64_t z = s;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return ((float)(z >> 40) + 0.5f) * 0x1.0p-24f;
+}
+
+__global__ void k_debris_samples(SampK a) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    uint64_t s = a.seed * 0xD1B54A32D192ED03ull + (uint64_t)i * 0x9E3779B97F4A7C15ull;
+    float g[8];
+    for (int k = 0; k < 8; k += 2) {     // Box-Muller pairs
+        const float u1 = samp_uniform(s), u2 = samp_uniform(s);
+        const float rr = __builtin_sqrtf(-2.0f * __logf(u1));
+        float sn, cs;
+        __sincosf(6.2831853f * u2, &sn, &cs);
+        g[k] = rr * cs;
+        g[k + 1] = rr * sn;
+    }
+    float qn = __builtin_sqrtf(g[0] * g[0] + g[1] * g[1] + g[2] * g[2] + g[3] * g[3]);
+    float4 R = make_float4(g[0] / qn, g[1] / qn, g[2] / qn, g[3] / qn);
+    if (R.w < 0.0f) R = make_float4(-R.x, -R.y, -R.z, -R.w);
+    float un = __builtin_sqrtf(g[4] * g[4] + g[5] * g[5] + g[6] * g[6]);
+    float4 uh = make_float4(g[4] / un, g[5] / un, g[6] / un, 0.0f);
+    float t[12];
+    table_targets(R, uh, a.cd, a.cm, a.ad, a.rr, a.ri, a.rd, t);
+    float *x = a.x + i * MLP_IN;
+    x[0] = R.x; x[1] = R.y; x[2] = R.z; x[3] = R.w;
+    x[4] = uh.x; x[5] = uh.y; x[6] = uh.z;
+    for (int k = 0; k < 12; k++) a.t[i * 12 + k] = t[k];
+}
+
+extern "C" int rsd_debris_samples(rsd_engine *e, int adm_table, int rcs_table, size_t n, uint64_t seed, float *x,
+                                  float *t) {
+    if (!e || !x || !t || n == 0) return RSD_ERR_ARG;
+    if (adm_table < 0 || adm_table >= RSD_MAX_TABLES || !e->adm_cd[adm_table]) return fail(e, RSD_ERR_STATE, "ADM table %d not set", adm_table);
+    if (rcs_table < 0 || rcs_table >= RSD_MAX_TABLES || !e->rcs_re[rcs_table]) return fail(e, RSD_ERR_STATE, "RCS table %d not set", rcs_table);
+    HIPCHK(e, hipSetDevice(e->device));
+    SampK a;
+    a.cd.d = e->adm_cd[adm_table]; a.cm.d = e->adm_cm[adm_table];
+    a.cd.nx = a.cm.nx = e->adm_nx[adm_table]; a.cd.ny = a.cm.ny = e->adm_ny[adm_table];
+    a.ad = e->adm_desc[adm_table];
+    a.rr.d = e->rcs_re[rcs_table]; a.ri.d = e->rcs_im[rcs_table];
+    a.rr.nx = a.ri.nx = e->rcs_nx[rcs_table]; a.rr.ny = a.ri.ny = e->rcs_ny[rcs_table];
+    a.rd = e->rcs_desc[rcs_table];
+    a.n = n;
+    a.seed = seed;
+    HIPCHK(e, hipMalloc((void **)&a.x, n * MLP_IN * sizeof(float)));
+    hipError_t r = hipMalloc((void **)&a.t, n * 12 * sizeof(float));
+    if (r == hipSuccess) {
+        hipLaunchKernelGGL(k_debris_samples, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, e->stream, a);
+        r = hipGetLastError();
+    }
+    if (r == hipSuccess) r = hipMemcpyAsync(x, a.x, n * MLP_IN * sizeof(float), hipMemcpyDeviceToHost, e->stream);
+    if (r == hipSuccess) r = hipMemcpyAsync(t, a.t, n * 12 * sizeof(float), hipMemcpyDeviceToHost, e->stream);
+    if (r == hipSuccess) r = hipStreamSynchronize(e->stream);
+    dfree(a.x);
+    dfree(a.t);
+    HIPCHK(e, r);
+    return RSD_OK;
+}
+
+extern "C" int rsd_set_debris_surrogate(rsd_engine *e, int type, const rsd_mlp *m) {
+    if (!e || type < 1 || type >= RSD_MAX_TYPES) return fail(e, RSD_ERR_ARG, "debris type %d out of range", type);
+    int st = flush_pending(e);
+    if (st) return st;
+    if (!m) {
+        e->surr[type] = 0;
+        return RSD_OK;
+    }
+    int dev = -1;
+    MlpDev v;
+    if (rsd_mlp_device_view(m, &v, &dev) != RSD_OK || dev != e->device) return fail(e, RSD_ERR_ARG, "network on another device");
+    e->net[type] = v;
+    e->surr[type] = 1;
     return RSD_OK;
 }

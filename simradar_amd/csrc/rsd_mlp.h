@@ -2,7 +2,7 @@
 // wave-level BF16 MFMA forward tile shared by the engine's debris kernel and the
 // training kernels.
 //
-// Network: 7 inputs -> 4 hidden layers of 64 (tanh) -> 12 outputs (linear),
+// Network: 7 inputs -> 4 hidden layers of 64 (ReLU) -> 12 outputs (linear),
 // i.e. 7 -> 64 -> 64 -> 64 -> 64 -> 12.  Inputs: the radar-relative orientation
 // quaternion R = ori (x) o_conj(el, az) (4) and the body-frame unit air-flow
 // direction u = rot(normalize(v_bg - v), conj(ori)) (3).  Outputs (before the
@@ -68,23 +68,23 @@ __device__ __forceinline__ uint16_t mlp_bf16(float f) {   // round to nearest ev
 
 __device__ __forceinline__ float mlp_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
+typedef __bf16 mlp_bf4 __attribute__((ext_vector_type(4)));
+
+// Four values to bf16 with the hardware conversion (round to nearest even).
 __device__ __forceinline__ mlp_s4 mlp_pack(mlp_f4 v) {
-    mlp_s4 r;
-    r.x = (short)mlp_bf16(v.x);
-    r.y = (short)mlp_bf16(v.y);
-    r.z = (short)mlp_bf16(v.z);
-    r.w = (short)mlp_bf16(v.w);
-    return r;
+    mlp_bf4 h;
+    h.x = (__bf16)v.x;
+    h.y = (__bf16)v.y;
+    h.z = (__bf16)v.z;
+    h.w = (__bf16)v.w;
+    return __builtin_bit_cast(mlp_s4, h);
 }
 
 __device__ __forceinline__ mlp_f4 mlp_mfma(mlp_s4 a, mlp_s4 b, mlp_f4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
-__device__ __forceinline__ float mlp_tanh(float x) {
-    const float e = __expf(2.0f * x);
-    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
-}
+__device__ __forceinline__ float mlp_act(float x) { return __builtin_fmaxf(x, 0.0f); }   // ReLU
 
 __device__ __forceinline__ mlp_s4 mlp_ld4(const uint16_t *p) {  // 4 consecutive bf16 (8-byte aligned)
     return *(const mlp_s4 *)p;
@@ -102,14 +102,14 @@ __device__ __forceinline__ mlp_f4 mlp_tile_forward(const uint16_t *w, const floa
         mlp_f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
         acc = mlp_mfma(mlp_ld4(w + MLP_W0 + (16 * mt + row) * MLP_K0 + 4 * grp), x, acc);
 #pragma unroll
-        for (int r = 0; r < 4; r++) acc[r] = mlp_tanh(acc[r] + b[MLP_B0 + 16 * mt + 4 * grp + r]);
+        for (int r = 0; r < 4; r++) acc[r] = mlp_act(acc[r] + b[MLP_B0 + 16 * mt + 4 * grp + r]);
         cur[mt] = mlp_pack(acc);
     }
     if (h) {
 #pragma unroll
         for (int k = 0; k < 4; k++) h[0][k] = cur[k];
     }
-#pragma unroll
+#pragma unroll 1
     for (int L = 1; L < 4; L++) {
         const uint16_t *W = w + MLP_W1 + (L - 1) * MLP_H * MLP_H;
 #pragma unroll
@@ -118,7 +118,7 @@ __device__ __forceinline__ mlp_f4 mlp_tile_forward(const uint16_t *w, const floa
 #pragma unroll
             for (int kt = 0; kt < 4; kt++) acc = mlp_mfma(mlp_ld4(W + (16 * mt + row) * MLP_H + 16 * kt + 4 * grp), cur[kt], acc);
 #pragma unroll
-            for (int r = 0; r < 4; r++) acc[r] = mlp_tanh(acc[r] + b[MLP_H * L + 16 * mt + 4 * grp + r]);
+            for (int r = 0; r < 4; r++) acc[r] = mlp_act(acc[r] + b[MLP_H * L + 16 * mt + 4 * grp + r]);
             nxt[mt] = mlp_pack(acc);
         }
 #pragma unroll
@@ -152,18 +152,77 @@ __device__ __forceinline__ mlp_s4 mlp_gather_input(const float f[8], int t) {
     return mlp_pack(a);
 }
 
+// Forward of the wave's four 16-sample tiles together: every weight tile is
+// read once and applied to the four tiles (four independent MFMA chains).
+__device__ __forceinline__ void mlp_forward4(const uint16_t *w, const float *b, const mlp_s4 x[4], mlp_f4 y[4]) {
+    const int l = threadIdx.x & 63, row = l & 15, grp = l >> 4;
+    mlp_s4 cur[4][4], nxt[4][4];
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+        const mlp_s4 A = mlp_ld4(w + MLP_W0 + (16 * mt + row) * MLP_K0 + 4 * grp);
+        const mlp_f4 bias = *(const mlp_f4 *)(b + MLP_B0 + 16 * mt + 4 * grp);
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            mlp_f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            acc = mlp_mfma(A, x[t], acc);
+#pragma unroll
+            for (int r = 0; r < 4; r++) acc[r] = mlp_act(acc[r] + bias[r]);
+            cur[t][mt] = mlp_pack(acc);
+        }
+    }
+#pragma unroll 1
+    for (int L = 1; L < 4; L++) {
+        const uint16_t *W = w + MLP_W1 + (L - 1) * MLP_H * MLP_H;
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++) {
+            mlp_f4 acc[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) acc[t] = (mlp_f4){0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int kt = 0; kt < 4; kt++) {
+                const mlp_s4 A = mlp_ld4(W + (16 * mt + row) * MLP_H + 16 * kt + 4 * grp);
+#pragma unroll
+                for (int t = 0; t < 4; t++) acc[t] = mlp_mfma(A, cur[t][kt], acc[t]);
+            }
+            const mlp_f4 bias = *(const mlp_f4 *)(b + MLP_H * L + 16 * mt + 4 * grp);
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+#pragma unroll
+                for (int r = 0; r < 4; r++) acc[t][r] = mlp_act(acc[t][r] + bias[r]);
+                nxt[t][mt] = mlp_pack(acc[t]);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+            for (int k = 0; k < 4; k++) cur[t][k] = nxt[t][k];
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++) y[t] = (mlp_f4){0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kt = 0; kt < 4; kt++) {
+        const mlp_s4 A = mlp_ld4(w + MLP_W4 + row * MLP_H + 16 * kt + 4 * grp);
+#pragma unroll
+        for (int t = 0; t < 4; t++) y[t] = mlp_mfma(A, cur[t][kt], y[t]);
+    }
+    const mlp_f4 bias = *(const mlp_f4 *)(b + MLP_B4 + 4 * grp);
+#pragma unroll
+    for (int t = 0; t < 4; t++) y[t] += bias;
+}
+
 // Network outputs of every lane's own sample: the four tiles of the wave are
-// evaluated and their D tiles exchanged through scratch (64 x 16 floats of LDS
-// per wave); out[0..11] = scale * y + shift.
+// evaluated together and their D tiles exchanged through scratch (64 x 16
+// floats of LDS per wave); out[0..11] = scale * y + shift.
 __device__ __forceinline__ void mlp_wave_eval(const uint16_t *w, const float *b, const float f[8], float *scratch,
                                               float out[MLP_OUT]) {
     const int l = threadIdx.x & 63, row = l & 15, grp = l >> 4;
-#pragma unroll 1
-    for (int t = 0; t < 4; t++) {
-        const mlp_s4 x = mlp_gather_input(f, t);
-        const mlp_f4 y = mlp_tile_forward(w, b, x, nullptr);
-        *(mlp_f4 *)(scratch + (16 * t + row) * MLP_N4 + 4 * grp) = y;
-    }
+    mlp_s4 x[4];
+    mlp_f4 y[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) x[t] = mlp_gather_input(f, t);
+    mlp_forward4(w, b, x, y);
+#pragma unroll
+    for (int t = 0; t < 4; t++) *(mlp_f4 *)(scratch + (16 * t + row) * MLP_N4 + 4 * grp) = y[t];
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
 #pragma unroll

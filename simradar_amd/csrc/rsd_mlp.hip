@@ -4,7 +4,7 @@
 // One training step over a batch of B samples (B a multiple of 256):
 //   k_mlp_fwd_bwd  per 16-sample tile: forward (activations kept in registers),
 //                  output error dY = 2 (y - t) / (12 B), backward through the
-//                  transposed weights; stores every layer's input X_L and
+//                  transposed weights and the ReLU masks; stores every layer's input X_L and
 //                  pre-activation gradient dZ_L feature-major in bf16
 //   k_mlp_wgrad    dW_L = dZ_L X_L^T as MFMA tiles over K-slices of the batch
 //   k_mlp_bgrad    db_L = sum over the batch of dZ_L
@@ -102,21 +102,21 @@ __global__ __launch_bounds__(256) void k_mlp_fwd_bwd(TrainK a) {
     if (l == 0) a.loss[tile] = se;
     const mlp_s4 dyb = mlp_pack(dy);
     store_fm(a.dz + (size_t)4 * 64 * a.B, a.B, 4 * grp, s, dyb);
-    // backward: dZ3 = (W4^T dY) * (1 - h3^2)
+    // backward: dZ3 = (W4^T dY) * [h3 > 0]
     mlp_s4 dcur[4], dnxt[4];
 #pragma unroll
     for (int mt = 0; mt < 4; mt++) {
         mlp_f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
         acc = mlp_mfma(mlp_ld4(wt + MLP_T4 + (16 * mt + row) * MLP_N4 + 4 * grp), dyb, acc);
         const mlp_s4 hv = h[3][mt];
-        acc[0] *= 1.0f - mlp_f32((uint16_t)hv.x) * mlp_f32((uint16_t)hv.x);
-        acc[1] *= 1.0f - mlp_f32((uint16_t)hv.y) * mlp_f32((uint16_t)hv.y);
-        acc[2] *= 1.0f - mlp_f32((uint16_t)hv.z) * mlp_f32((uint16_t)hv.z);
-        acc[3] *= 1.0f - mlp_f32((uint16_t)hv.w) * mlp_f32((uint16_t)hv.w);
+        acc[0] = mlp_f32((uint16_t)hv.x) > 0.0f ? acc[0] : 0.0f;
+        acc[1] = mlp_f32((uint16_t)hv.y) > 0.0f ? acc[1] : 0.0f;
+        acc[2] = mlp_f32((uint16_t)hv.z) > 0.0f ? acc[2] : 0.0f;
+        acc[3] = mlp_f32((uint16_t)hv.w) > 0.0f ? acc[3] : 0.0f;
         dcur[mt] = mlp_pack(acc);
         store_fm(a.dz + (size_t)3 * 64 * a.B, a.B, 16 * mt + 4 * grp, s, dcur[mt]);
     }
-    // dZ(L-1) = (W_L^T dZ_L) * (1 - h(L-1)^2), L = 3, 2, 1
+    // dZ(L-1) = (W_L^T dZ_L) * [h(L-1) > 0], L = 3, 2, 1
 #pragma unroll
     for (int L = 3; L >= 1; L--) {
         const uint16_t *WT = wt + MLP_T1 + (L - 1) * MLP_H * MLP_H;
@@ -126,10 +126,10 @@ __global__ __launch_bounds__(256) void k_mlp_fwd_bwd(TrainK a) {
 #pragma unroll
             for (int kt = 0; kt < 4; kt++) acc = mlp_mfma(mlp_ld4(WT + (16 * mt + row) * MLP_H + 16 * kt + 4 * grp), dcur[kt], acc);
             const mlp_s4 hv = h[L - 1][mt];
-            acc[0] *= 1.0f - mlp_f32((uint16_t)hv.x) * mlp_f32((uint16_t)hv.x);
-            acc[1] *= 1.0f - mlp_f32((uint16_t)hv.y) * mlp_f32((uint16_t)hv.y);
-            acc[2] *= 1.0f - mlp_f32((uint16_t)hv.z) * mlp_f32((uint16_t)hv.z);
-            acc[3] *= 1.0f - mlp_f32((uint16_t)hv.w) * mlp_f32((uint16_t)hv.w);
+            acc[0] = mlp_f32((uint16_t)hv.x) > 0.0f ? acc[0] : 0.0f;
+            acc[1] = mlp_f32((uint16_t)hv.y) > 0.0f ? acc[1] : 0.0f;
+            acc[2] = mlp_f32((uint16_t)hv.z) > 0.0f ? acc[2] : 0.0f;
+            acc[3] = mlp_f32((uint16_t)hv.w) > 0.0f ? acc[3] : 0.0f;
             dnxt[mt] = mlp_pack(acc);
             store_fm(a.dz + (size_t)(L - 1) * 64 * a.B, a.B, 16 * mt + 4 * grp, s, dnxt[mt]);
         }
@@ -306,7 +306,7 @@ extern "C" int rsd_mlp_create(int device, uint64_t seed, rsd_mlp **out) {
         return RSD_ERR_ALLOC;
     }
     m->b = m->master + MLP_WN;
-    // Xavier-uniform weights from a splitmix64 stream, zero biases, unit scaling
+    // He-uniform weights (ReLU) from a splitmix64 stream, zero biases, unit scaling
     float *h = (float *)calloc(MLP_PN, sizeof(float));
     if (!h) {
         rsd_mlp_destroy(m);
@@ -326,7 +326,7 @@ extern "C" int rsd_mlp_create(int device, uint64_t seed, rsd_mlp **out) {
         z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
         z ^= z >> 31;
         const double u = ((double)(z >> 11) + 0.5) * 0x1.0p-53;
-        const double lim = sqrt(6.0 / (fan_in + fan_out));
+        const double lim = i >= MLP_W4 ? sqrt(6.0 / (fan_in + fan_out)) : sqrt(6.0 / fan_in);
         h[i] = (float)((2.0 * u - 1.0) * lim);
     }
     for (int k = 0; k < MLP_OUT; k++) h[MLP_WN + MLP_SCALE + k] = 1.0f;

@@ -37,7 +37,7 @@ def test_oracle_gradient_matches_finite_differences():
     loss, g = mlp.loss_grad(p, x, t)
     rng = np.random.default_rng(4)
     idx = rng.choice(p.size, 60, replace=False)
-    h = 1e-6
+    h = 1e-7   # small enough that no ReLU changes state
     for i in idx:
         pp, pm = p.copy(), p.copy()
         pp[i] += h
@@ -46,19 +46,19 @@ def test_oracle_gradient_matches_finite_differences():
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)) + 1e-9, (i, fd, g[i])
 
 
-def test_oracle_linear_limit():
-    """With tiny weights the hidden layers are near-linear (tanh z ~ z): the
-    network equals the product of its matrices to first order."""
-    p = _params(5, scale=1e-3)
-    W = mlp.unflatten(p)
-    for L in range(4):
-        W[2 * L + 1][:] = 0.0
-    W[9][:] = 0.0
+def test_oracle_linear_regime():
+    """With positive weights, inputs and biases every ReLU is active and the
+    network is the affine map of its matrices."""
+    g = np.random.default_rng(5)
+    W = [np.abs(g.normal(size=s)) * (0.1 if len(s) == 2 else 0.01) for s in mlp.SHAPES]
     p = mlp.flatten(W)
-    x = _data(8, 6)
+    x = np.abs(_data(8, 6))
     y, _ = mlp.forward(p, x)
-    lin = x @ W[0].T @ W[2].T @ W[4].T @ W[6].T @ W[8].T
-    assert np.abs(y - lin).max() <= 1e-5 * np.sqrt(np.mean(lin ** 2))
+    a = x
+    for L in range(4):
+        a = a @ W[2 * L].T + W[2 * L + 1]
+    lin = a @ W[8].T + W[9]
+    np.testing.assert_allclose(y, lin, rtol=1e-12)
 
 
 def test_bf16_rounding_is_nearest_even():
