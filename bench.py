@@ -13,7 +13,9 @@ state resident in HBM.  Prints one JSON line.
 Other workloads (one JSON line each, not the headline): --workload c3 (8,388,608
 rain in a 3 km x 3 km x 1 km box at 10 km range, hi/lo positions, 202 gates),
 --workload c4 (64 independent domains of 2^16..2^21 rain, one persistent launch
-per round with HIP-graph replay, against the same domains launched one by one).
+per round with HIP-graph replay, against the same domains launched one by one),
+--workload c5 (2,097,152 rain + 3 x 262,144 debris: the ADM / RCS table path
+against the trained surrogate networks, training time reported separately).
 """
 import argparse
 import json
@@ -55,7 +57,7 @@ def main():
     ap.add_argument("--e2e-steps", type=int, default=200)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--max-blocks", type=int, default=0)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"])
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build drives a single MI355X per process "
@@ -65,6 +67,8 @@ def main():
         return bench_c3(args)
     if args.workload == "c4":
         return bench_c4(args)
+    if args.workload == "c5":
+        return bench_c5(args)
 
     import simradar_amd as sa
     from simradar_amd import workloads as wl
@@ -265,6 +269,53 @@ def bench_c4(args):
           {"scatterer_updates_per_s": total / batch_s,
            "plain_pulses_per_s": 64.0 / plain_s, "plain_ms_per_round": plain_s * 1000.0,
            "batched_over_plain": plain_s / batch_s})
+
+
+def bench_c5(args):
+    """C5: table lookups against surrogate networks for the three debris types."""
+    import simradar_amd as sa
+    from simradar_amd import surrogate as sg
+    from simradar_amd import workloads as wl
+    cfg = wl.c5()
+    print("bench: populating C5 (%d scatterers)" % cfg.n_total, file=sys.stderr, flush=True)
+    r = wl.make(cfg)
+    steps = 500 if args.steps == 1000 else args.steps
+
+    def timed():
+        for _ in range(20):
+            r.make_pulse()
+            r.advance_time()
+            r.advance_beam()
+        r.sync()
+        r.timer_start()
+        for _ in range(steps):
+            r.make_pulse()
+            r.advance_time()
+            r.advance_beam()
+        r.make_pulse()
+        return r.timer_stop() / (steps + 1)
+    ms_table = timed()
+    t0 = time.perf_counter()
+    nets, errs = [], []
+    for k in (1, 2, 3):
+        m, _ = sg.fit(r, k, n_samples=1 << 22, epochs=40)
+        errs.append(sg.error(m, r, k))
+        nets.append(m)
+    train_s = time.perf_counter() - t0
+    for k, m in zip((1, 2, 3), nets):
+        r.set_debris_surrogate(k, m)
+    ms_surr = timed()
+    _line("pulses/s & scatterer-updates/s (C5, debris surrogate)", 1000.0 / ms_surr, "pulses/s", steps + 1, 20,
+          ms_surr, "f32 physics, bf16 network (fp32 accumulate), f64 phase and gate sums",
+          {"workload": "C5: 2,097,152 rain + leaf, plate, brick at 262,144 each; surrogate networks 7-4x64-12",
+           "scatterers": cfg.n_total},
+          {"scatterer_updates_per_s": 1000.0 / ms_surr * cfg.n_total,
+           "table_pulses_per_s": 1000.0 / ms_table, "surrogate_over_table": ms_table / ms_surr,
+           "training_s": train_s, "training_samples_per_type": 1 << 22,
+           "heldout_rel_err_cd_cm_amp": errs})
+    for m in nets:
+        m.free()
+    r.free()
 
 
 if __name__ == "__main__":
