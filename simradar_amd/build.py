@@ -16,6 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsimradar_amd.so")
+CLI = os.path.join(HERE, "bin", "simradar")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CC = os.environ.get("CC", "gcc")
@@ -58,6 +59,11 @@ def build(verbose=True):
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-lm"])
     os.replace(tmp, LIB)
+    # command-line driver (simradar.c equivalent), linked against the library
+    os.makedirs(os.path.dirname(CLI), exist_ok=True)
+    _run([CC] + C_FLAGS + [os.path.join(CSRC, "simradar_cli.c"), "-o", CLI + ".tmp", "-L" + HERE, "-lsimradar_amd",
+                           "-Wl,-rpath,$ORIGIN/..", "-lm"])
+    os.replace(CLI + ".tmp", CLI)
     if verbose:
         print("built", LIB)
     return LIB
