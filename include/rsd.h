@@ -150,10 +150,13 @@ int rsd_set_sort(rsd_engine *e, int interval);
  *   RSD_FOLD_WAVE64  where a wave's scatterers fall into at most 16 gates, the
  *   RSD_FOLD_WAVE32  wave sums each gate's terms (binary64 / binary32 partials
  *                    of 64 terms) and one lane adds them;
- *   RSD_FOLD_WINDOW  (default) each lane sums its scatterers' terms into a
- *                    register window of 8 consecutive gates (binary32) and the
- *                    wave adds the windows once per launch; taken when the range
- *                    weight spans under 6 gate spacings, else WAVE32.
+ *   RSD_FOLD_WINDOW  (default) terms are summed in registers (binary32) over a
+ *                    window of 8 consecutive gates and added to LDS once per
+ *                    launch: for rain the window is shared by the 64 lanes of a
+ *                    wave (streams in range order) and reduced in one transposed
+ *                    wave pass; for debris each lane keeps its own window.
+ *                    Taken when the range weight spans under 6 gate spacings
+ *                    and its table ends in zeros, else WAVE32.
  * No counterpart in the reference (one work-item per scatterer, dense gate loop,
  * rs.cl:1182-1217). */
 #define RSD_FOLD_LANE    0

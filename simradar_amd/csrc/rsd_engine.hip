@@ -530,20 +530,40 @@ __device__ __forceinline__ void win_lane_flush(const GateWin &w, const PulseK &k
     }
 }
 
-// Range weight of offset x from the table in LDS (same clamp / fract / mix as
-// table_lerp; the upper texel is min(ip + 1, n - 1)).  Only used where the
-// table ends in zeros, so offsets outside the support weigh exactly 0.
-__device__ __forceinline__ float lds_weight(float x, const float *W, const F4 &d, int n) {
+// Range weight of offset x from the table in LDS, held as pairs (W[i],
+// W[min(i + 1, n - 1)] - W[i]): the same clamp / fract / mix as table_lerp,
+// W[i] + (W[i + 1] - W[i]) dec with the difference rounded once when the table
+// is staged.  Only used where the table ends in zeros, so offsets outside the
+// support weigh exactly 0.
+__device__ __forceinline__ float lds_weight(float x, const float2 *W, const F4 &d) {
     float x0 = rf_clamp(rf_fma(x, d.s[0], d.s[1]), 0.0f, d.s[2]);
     float ip = __builtin_floorf(x0);
     float dec = __builtin_fminf(x0 - ip, 0x1.fffffep-1f);
-    int i0 = (int)ip;
-    int i1 = min(i0 + 1, n - 1);
-    return rf_mix(W[i0], W[i1], dec);
+    const float2 p = W[(int)ip];
+    return p.x + p.y * dec;
+}
+
+// LDS layout of the window folds: gate sums acc[4 R] (binary64), then the range
+// weight table as the pairs lds_weight reads, then the R gate ranges.
+__device__ __forceinline__ float2 *win_weights(double *acc, const PulseK &k) { return (float2 *)(acc + 4 * k.R); }
+__device__ __forceinline__ float *win_gates(double *acc, const PulseK &k) {
+    return (float *)(win_weights(acc, k) + k.rangew_n);
+}
+
+__device__ __forceinline__ void stage_window(double *acc, const PulseK &k) {
+    float2 *wl = win_weights(acc, k);
+    float *gr = win_gates(acc, k);
+    const int n = k.rangew_n;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        const float w0 = k.rangew[j], w1 = k.rangew[min(j + 1, n - 1)];
+        wl[j] = make_float2(w0, w1 - w0);
+    }
+    for (int j = threadIdx.x; j < k.R; j += blockDim.x) gr[j] = k.gate_r[j];
+    __syncthreads();
 }
 
 __device__ __forceinline__ void win_fold(GateWin &w, bool on, float r, float4 sa, const PulseK &k, double *acc,
-                                         const float *wl) {
+                                         const float2 *wl, const float *grl) {
     int klo = 0x3fffffff, khi = -1;
     if (on) gate_support(r, k, klo, khi);
     on = klo <= khi;
@@ -559,7 +579,7 @@ __device__ __forceinline__ void win_fold(GateWin &w, bool on, float r, float4 sa
     for (int s = 0; s < RSD_WIN; s++) {
         if (s < smin || s > smax) continue;
         const int g = min(max(w.kb + s, 0), gmax);
-        float wr = lds_weight(r - k.gate_r[g], wl, k.rangew_desc, k.rangew_n);
+        float wr = lds_weight(r - grl[g], wl, k.rangew_desc);
         if (!on || w.kb + s != g) wr = 0.0f;
         w.a[s][0] = __builtin_fmaf(wr, sa.x, w.a[s][0]);
         w.a[s][1] = __builtin_fmaf(wr, sa.y, w.a[s][1]);
@@ -598,6 +618,125 @@ __device__ __forceinline__ void win_flush(GateWin &w, const PulseK &k, double *a
                 }
             }
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Wave-shared window (rain streams in range order): the 64 lanes of a wave
+// share one window of RSD_WIN consecutive gates [kb, kb + RSD_WIN), kb
+// wave-uniform.  Each lane keeps binary32 partial sums of its scatterers' terms
+// for the 32 (gate, component) slots; the wave reduces them in one transposed
+// pass when the window moves or the launch ends.  Scatterers whose gates fall
+// outside the window (respawned ones, until the next reorder) add their terms
+// to LDS directly.
+// ---------------------------------------------------------------------------
+template <int CTRL>
+__device__ __forceinline__ float dpp_perm(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+
+// Transposed sum over the wave of 32 values per lane: afterwards lanes 2j and
+// 2j + 1 hold the sum over all 64 lanes of a[j].  Each step pairs every lane
+// with one partner, keeps half of its values and adds the partner's copy of
+// that half: l ^ 32 (permlane32 swap), l ^ 16 (permlane16 swap), l ^ 15 (DPP
+// row_mirror), l ^ 7 (row_half_mirror), l ^ 2 and l ^ 1 (quad_perm).
+// 70 instructions for the 32 sums instead of 32 separate wave reductions.
+__device__ __forceinline__ float wave_transpose_sum(const float (&a)[32]) {
+    float b[16], c[8], d[4], e[2];
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        const auto s = __builtin_amdgcn_permlane32_swap(__float_as_uint(a[j]), __float_as_uint(a[j + 16]), false, false);
+        b[j] = __uint_as_float(s[0]) + __uint_as_float(s[1]);      // value j + 16 bit5(l)
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const auto s = __builtin_amdgcn_permlane16_swap(__float_as_uint(b[j]), __float_as_uint(b[j + 8]), false, false);
+        c[j] = __uint_as_float(s[0]) + __uint_as_float(s[1]);      // value j + 8 bit4 + 16 bit5
+    }
+    const unsigned lane = __lane_id();
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const float t0 = c[j] + dpp_perm<0x140>(c[j]);
+        const float t1 = c[j + 4] + dpp_perm<0x140>(c[j + 4]);
+        d[j] = (lane & 8u) ? t1 : t0;                               // value j + 4 bit3 + ...
+    }
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const float t0 = d[j] + dpp_perm<0x141>(d[j]);
+        const float t1 = d[j + 2] + dpp_perm<0x141>(d[j + 2]);
+        e[j] = (lane & 4u) ? t1 : t0;                               // value j + 2 bit2 + ...
+    }
+    const float t0 = e[0] + dpp_perm<0x4e>(e[0]);
+    const float t1 = e[1] + dpp_perm<0x4e>(e[1]);
+    const float f = (lane & 2u) ? t1 : t0;                          // value l >> 1
+    return f + dpp_perm<0xb1>(f);
+}
+
+struct WaveWin {
+    float a[4 * RSD_WIN];   // slot 4 s + c: gate kb + s, component c
+    int kb;                 // wave-uniform first gate; RSD_WIN_EMPTY: no window
+};
+
+__device__ __forceinline__ void wwin_clear(WaveWin &w) {
+#pragma unroll
+    for (int j = 0; j < 4 * RSD_WIN; j++) w.a[j] = 0.0f;
+}
+
+// Terms of one scatterer straight into the LDS gate sums.
+__device__ __forceinline__ void lane_fold(float r, float4 sa, int klo, int khi, const PulseK &k, double *acc,
+                                          const float2 *wl, const float *grl) {
+    const int R = k.R;
+    for (int g = klo; g <= khi; g++) {
+        const float wr = lds_weight(r - grl[g], wl, k.rangew_desc);
+        if (wr == 0.0f) continue;
+        atomicAdd(&acc[g], (double)(wr * sa.x));
+        atomicAdd(&acc[R + g], (double)(wr * sa.y));
+        atomicAdd(&acc[2 * R + g], (double)(wr * sa.z));
+        atomicAdd(&acc[3 * R + g], (double)(wr * sa.w));
+    }
+}
+
+__device__ __forceinline__ void wwin_flush(const WaveWin &w, const PulseK &k, double *acc) {
+    if (w.kb == RSD_WIN_EMPTY) return;
+    const float v = wave_transpose_sum(w.a);
+    const unsigned lane = __lane_id();
+    const int g = w.kb + (int)(lane >> 3), c = (int)((lane >> 1) & 3u);
+    if (!(lane & 1u) && g >= 0 && g < k.R && v != 0.0f) atomicAdd(&acc[c * k.R + g], (double)v);
+}
+
+// One scatterer per lane (on: it has a contribution).  The slots a lane
+// touches need no mask: outside a scatterer's support the table weight is
+// exactly 0 (the window is only used for tables that end in zeros).
+__device__ __forceinline__ void wwin_fold(WaveWin &w, bool on, float r, float4 sa, const PulseK &k, double *acc,
+                                          const float2 *wl, const float *grl) {
+    int klo = 0x3fffffff, khi = -1;
+    if (on) gate_support(r, k, klo, khi);
+    on = klo <= khi;
+    const unsigned long long on_m = __ballot(on);
+    if (on_m == 0) return;
+    bool fit = on && klo >= w.kb && khi < w.kb + RSD_WIN;
+    const unsigned long long fit_m = __ballot(fit);
+    if (2 * __popcll(fit_m) < __popcll(on_m)) {
+        // Most of this step's scatterers lie outside: move the window to the
+        // first of them, one gate of room below.
+        wwin_flush(w, k, acc);
+        wwin_clear(w);
+        const int lead = __builtin_ctzll(on_m & ~fit_m);
+        w.kb = __builtin_amdgcn_readlane(klo, lead) - 1;
+        fit = on && klo >= w.kb && khi < w.kb + RSD_WIN;
+    }
+    if (on && !fit) lane_fold(r, sa, klo, khi, k, acc, wl, grl);
+    if (!fit) sa = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+    for (int s = 0; s < RSD_WIN; s++) {
+        const int g = w.kb + s;
+        if (g < 0 || g >= k.R) continue;
+        if (__ballot(fit && klo <= g && g <= khi) == 0) continue;
+        const float wr = lds_weight(r - grl[g], wl, k.rangew_desc);
+        w.a[4 * s + 0] = __builtin_fmaf(wr, sa.x, w.a[4 * s + 0]);
+        w.a[4 * s + 1] = __builtin_fmaf(wr, sa.y, w.a[4 * s + 1]);
+        w.a[4 * s + 2] = __builtin_fmaf(wr, sa.z, w.a[4 * s + 2]);
+        w.a[4 * s + 3] = __builtin_fmaf(wr, sa.w, w.a[4 * s + 3]);
     }
 }
 
@@ -760,9 +899,12 @@ __device__ __forceinline__ void logtab_load(float *dst, const float *src) {
 // the block's LDS gate sums acc (cleared by the caller).
 template <bool ADV, bool PULSE, bool HILO>
 __device__ __forceinline__ void rain_range(const RainK &a, size_t lo, size_t hi, double *acc, const float *logtab,
-                                           const float *wl, bool use_win) {
-    GateWin win;
-    win_reset(win, RSD_WIN_EMPTY);
+                                           bool use_win) {
+    const float2 *wl = win_weights(acc, a.pk);
+    const float *grl = win_gates(acc, a.pk);
+    WaveWin win;
+    win.kb = RSD_WIN_EMPTY;
+    wwin_clear(win);
     for (size_t base = lo; base < hi; base += blockDim.x) {
         const size_t i = base + threadIdx.x;
         const bool live = i < hi;
@@ -799,11 +941,11 @@ __device__ __forceinline__ void rain_range(const RainK &a, size_t lo, size_t hi,
                 if (!have_rcs) rcs = stale ? a.rcs[i] : ellipsoid_rcs(p, a.drop, a.drop_desc);
                 on = scat_signal<HILO>(p, rcs, a.pk, sa, r, plo);
             }
-            if (use_win) win_fold(win, on, r, sa, a.pk, acc, wl);
+            if (use_win) wwin_fold(win, on, r, sa, a.pk, acc, wl, grl);
             else fold_gates(on, r, sa, a.pk, acc);
         }
     }
-    if (use_win) win_flush(win, a.pk, acc);
+    if (use_win) wwin_flush(win, a.pk, acc);
 }
 
 template <bool ADV, bool PULSE, bool HILO>
@@ -816,12 +958,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     size_t lo, hi;
     block_range(0, a.n, lo, hi);
     const bool use_win = PULSE && a.pk.fold == RSD_FOLD_WINDOW;
-    float *wl = (float *)(acc + 4 * a.pk.R);
-    if (use_win) {
-        for (int j = threadIdx.x; j < a.pk.rangew_n; j += blockDim.x) wl[j] = a.pk.rangew[j];
-        __syncthreads();
-    }
-    rain_range<ADV, PULSE, HILO>(a, lo, hi, acc, logtab, wl, use_win);
+    if (use_win) stage_window(acc, a.pk);
+    rain_range<ADV, PULSE, HILO>(a, lo, hi, acc, logtab, use_win);
     if (PULSE) acc_flush(acc, a.pk);
 }
 
@@ -844,14 +982,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         const RainK &a = doms[it.x];
         if (PULSE) acc_clear(acc, 4 * a.pk.R);
         const bool use_win = PULSE && a.pk.fold == RSD_FOLD_WINDOW;
-        float *wl = (float *)(acc + 4 * a.pk.R);
-        if (use_win) {
-            for (int j = threadIdx.x; j < a.pk.rangew_n; j += blockDim.x) wl[j] = a.pk.rangew[j];
-            __syncthreads();
-        }
+        if (use_win) stage_window(acc, a.pk);
         const size_t lo = (size_t)it.y * (size_t)chunk;
         const size_t hi = lo + (size_t)chunk < a.n ? lo + (size_t)chunk : a.n;
-        rain_range<ADV, PULSE, false>(a, lo, hi, acc, logtab, wl, use_win);
+        rain_range<ADV, PULSE, false>(a, lo, hi, acc, logtab, use_win);
         if (PULSE) acc_flush_row(acc, a.pk, it.y);
         __syncthreads();
     }
@@ -1039,11 +1173,9 @@ __global__ __launch_bounds__(256) void k_debris(DebrisK a) {
     size_t lo, hi;
     block_range(a.begin, a.end, lo, hi);
     const bool use_win = PULSE && a.pk.fold == RSD_FOLD_WINDOW;
-    float *wl = (float *)(acc + 4 * a.pk.R);
-    if (use_win) {
-        for (int j = threadIdx.x; j < a.pk.rangew_n; j += blockDim.x) wl[j] = a.pk.rangew[j];
-        __syncthreads();
-    }
+    if (use_win) stage_window(acc, a.pk);
+    const float2 *wl = win_weights(acc, a.pk);
+    const float *grl = win_gates(acc, a.pk);
     GateWin win;
     win_reset(win, RSD_WIN_EMPTY);
     for (size_t base = lo; base < hi; base += blockDim.x) {
@@ -1130,7 +1262,7 @@ __global__ __launch_bounds__(256) void k_debris(DebrisK a) {
             float4 sa = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             float r = 0.0f;
             bool on = live && scat_signal(p, rcs, a.pk, sa, r);
-            if (use_win) win_fold(win, on, r, sa, a.pk, acc, wl);
+            if (use_win) win_fold(win, on, r, sa, a.pk, acc, wl, grl);
             else fold_gates(on, r, sa, a.pk, acc);
         }
     }
@@ -1232,6 +1364,13 @@ __global__ void k_math_probe(int fn, const float *a, const float *b, float *out,
         case 7: r = rf_div(x, y); break;
         case 8: rf_sincos_turns(x, &s, &c); r = s; break;
         case 9: rf_sincos_turns(x, &s, &c); r = c; break;
+        case 11: {  // transposed wave sum of a[j] = x + j y (n a multiple of 64)
+            float v[32];
+#pragma unroll
+            for (int j = 0; j < 32; j++) v[j] = rf_fma((float)j, y, x);
+            r = wave_transpose_sum(v);
+            break;
+        }
         default: break;
     }
     out[i] = r;
@@ -1811,8 +1950,11 @@ static int fill_pulse(rsd_engine *e, const rsd_pulse *p, PulseK &k) {
     k.fold = e->fold;
     // The window needs every scatterer's gates inside RSD_WIN consecutive gates
     // (with room to move); wider supports take the wave sums.
+    // Its LDS tables (weight pairs and gate ranges) must leave room for the
+    // debris kernels' static LDS next to the 4 R binary64 gate sums.
     if (k.fold == RSD_FOLD_WINDOW &&
-        (k.dense || (k.sup_hi - k.sup_lo) * k.gate_inv_delta >= (float)(RSD_WIN - 2)))
+        (k.dense || (k.sup_hi - k.sup_lo) * k.gate_inv_delta >= (float)(RSD_WIN - 2) ||
+         (size_t)k.R * (4 * sizeof(double) + sizeof(float)) + (size_t)k.rangew_n * sizeof(float2) > 96 * 1024))
         k.fold = RSD_FOLD_WAVE32;
     k.partials = e->partials;
     k.row0 = 0;
@@ -1898,7 +2040,7 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p) {
         st = fill_pulse(e, p, pk);
         if (st) return st;
     }
-    const size_t lds = p ? (size_t)4 * R * sizeof(double) + (pk.fold == RSD_FOLD_WINDOW ? (size_t)e->rangew_n * sizeof(float) : 0) : 0;
+    const size_t lds = p ? (size_t)4 * R * sizeof(double) + (pk.fold == RSD_FOLD_WINDOW ? (size_t)e->rangew_n * sizeof(float2) + (size_t)R * sizeof(float) : 0) : 0;
     // Debris types run on the side stream, concurrently with the rain kernel;
     // partial rows: rain blocks first, then each debris type.
     const bool any_debris = rows > rain_blocks;
@@ -2432,6 +2574,7 @@ extern "C" int rsd_sync(rsd_engine *e) {
 
 extern "C" int rsd_math_probe(int device, int fn, const float *a, const float *b, float *out, size_t n) {
     if (!a || !b || !out || n == 0) return RSD_ERR_ARG;
+    if (fn == 11 && n % 64 != 0) return RSD_ERR_ARG;   // whole waves only
     if (device < 0 || device >= rsd_device_count()) return RSD_ERR_NO_DEVICE;
     if (hipSetDevice(device) != hipSuccess) return RSD_ERR_HIP;
     float *d = NULL, *lt = NULL;
@@ -2835,7 +2978,7 @@ extern "C" int rsd_batch_launch(rsd_batch *b) {
         fk[k] = f;
         first[k] = total4r;
         total4r += 4 * R;
-        size_t l = (size_t)4 * R * sizeof(double) + (a.pk.fold == RSD_FOLD_WINDOW ? (size_t)e->rangew_n * sizeof(float) : 0);
+        size_t l = (size_t)4 * R * sizeof(double) + (a.pk.fold == RSD_FOLD_WINDOW ? (size_t)e->rangew_n * sizeof(float2) + (size_t)R * sizeof(float) : 0);
         if (l > lds) lds = l;
     }
     first[ns] = total4r;

@@ -177,3 +177,23 @@ def test_device_fast_acos_accuracy(gpu_ready):
     c = np.concatenate([g.uniform(-1, 1, 100000), 1 - np.exp(g.uniform(-20, -1, 50000))]).astype(np.float32)
     err = np.abs(sa.math_probe(10, c).astype(np.float64) - np.arccos(c.astype(np.float64)))
     assert err.max() < 3e-7, err.max()
+
+
+def test_wave_transpose_sum_layout(gpu_ready):
+    """The gate window's wave reduction: lanes 2j and 2j + 1 of every wave end
+    with the sum over the wave's 64 lanes of value j = x + j y (integers, so the
+    sums are exact whatever the order)."""
+    g = np.random.default_rng(5)
+    n = 256
+    x = g.integers(-1000, 1000, n).astype(np.float32)
+    y = g.integers(-50, 50, n).astype(np.float32)
+    out = sa.math_probe(11, x, y)
+    lane = np.arange(n) % 64
+    wave = np.arange(n) // 64
+    j = lane >> 1
+    xs = x.astype(np.float64).reshape(-1, 64).sum(1)
+    ys = y.astype(np.float64).reshape(-1, 64).sum(1)
+    want = xs[wave] + j * ys[wave]
+    assert np.array_equal(out.astype(np.float64), want), (out[:16], want[:16])
+    with pytest.raises(RuntimeError):
+        sa.math_probe(11, x[:100], y[:100])
