@@ -123,9 +123,10 @@ int rsd_timer_start(rsd_engine *e);
 int rsd_timer_stop(rsd_engine *e, double *ms);
 
 /* Per-launch HIP-event timing of the background (rain) kernel on the engine's
- * stream: rsd_profile(e, 1) clears and starts recording; rsd_profile_read sums
- * the durations of the recorded launches of one kind (1 advance only, 2 pulse
- * only, 3 fused advance + pulse, -1 all). */
+ * stream: rsd_profile(e, n) clears and starts recording every n-th launch
+ * (n = 1: all; 0 stops); rsd_profile_read sums the durations of the recorded
+ * launches of one kind (1 advance only, 2 pulse only, 3 fused advance + pulse,
+ * -1 all). */
 int rsd_profile(rsd_engine *e, int enable);
 int rsd_profile_read(rsd_engine *e, int kind, double *total_ms, int *count);
 

@@ -464,9 +464,10 @@ class Radar:
             raise RuntimeError(self.lib.rsd_last_error(e).decode())
         return ms.value
 
-    def profile(self, enable=True):
+    def profile(self, enable=True, stride=1):
+        """Record HIP-event timings of every stride-th rain-kernel launch."""
         e = self.engine()
-        if self.lib.rsd_profile(e, 1 if enable else 0):
+        if self.lib.rsd_profile(e, int(stride) if enable else 0):
             raise RuntimeError(self.lib.rsd_last_error(e).decode())
 
     def profile_read(self, kind=-1):

@@ -744,8 +744,7 @@ __device__ __forceinline__ void wwin_fold(WaveWin &w, bool on, float r, float4 s
 // [L chunk, (L + 1) chunk).  Blocks are dispatched round-robin over the 8 XCDs,
 // so L = (b mod 8) (G / 8) + b / 8 gives each XCD one contiguous eighth of the
 // (spatially sorted) streams and keeps its L2 on one region of the wind table.
-__device__ __forceinline__ void block_range(size_t begin, size_t end, size_t &lo, size_t &hi) {
-    const unsigned G = gridDim.x, b = blockIdx.x;
+__device__ __forceinline__ void block_range_at(size_t begin, size_t end, unsigned b, unsigned G, size_t &lo, size_t &hi) {
     const unsigned L = (G & 7u) == 0u ? (b & 7u) * (G >> 3) + (b >> 3) : b;
     const size_t n = end - begin;
     size_t chunk = (n + G - 1) / G;
@@ -754,6 +753,10 @@ __device__ __forceinline__ void block_range(size_t begin, size_t end, size_t &lo
     hi = lo + chunk;
     if (lo > end) lo = end;
     if (hi > end) hi = end;
+}
+
+__device__ __forceinline__ void block_range(size_t begin, size_t end, size_t &lo, size_t &hi) {
+    block_range_at(begin, end, blockIdx.x, gridDim.x, lo, hi);
 }
 
 __device__ __forceinline__ void acc_clear(double *acc, int n) {
@@ -946,21 +949,6 @@ __device__ __forceinline__ void rain_range(const RainK &a, size_t lo, size_t hi,
         }
     }
     if (use_win) wwin_flush(win, a.pk, acc);
-}
-
-template <bool ADV, bool PULSE, bool HILO>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_rain(RainK a) {
-    extern __shared__ double acc[];
-    __shared__ float logtab[3 * RF_LOG_TABLE];
-    if (ADV) logtab_load(logtab, a.logtab);
-    if (PULSE) acc_clear(acc, 4 * a.pk.R);
-    else __syncthreads();
-    size_t lo, hi;
-    block_range(0, a.n, lo, hi);
-    const bool use_win = PULSE && a.pk.fold == RSD_FOLD_WINDOW;
-    if (use_win) stage_window(acc, a.pk);
-    rain_range<ADV, PULSE, HILO>(a, lo, hi, acc, logtab, use_win);
-    if (PULSE) acc_flush(acc, a.pk);
 }
 
 // Many independent rain domains in one persistent launch.  items[t] = (domain,
@@ -1156,24 +1144,13 @@ __device__ bool debris_advance(const DebrisK &a, size_t i, float4 &pos, float4 &
 
 // SURR: the surrogate network replaces the ADM and RCS table lookups; every
 // wave evaluates it for its 64 lanes at once (BF16 MFMA, weights in LDS).
+// Debris scatterers [lo, hi) of one type: advance and/or pulse terms into the
+// block's LDS gate sums (cleared, window tables staged by the caller).  SURR:
+// the surrogate network in LDS (nw, nb) with per-wave scratch nscr.
 template <bool ADV, bool PULSE, bool SURR>
-__global__ __launch_bounds__(256) void k_debris(DebrisK a) {
-    extern __shared__ double acc[];
-    __shared__ float logtab[3 * RF_LOG_TABLE];
-    __shared__ uint16_t nw[SURR ? MLP_WN : 2];
-    __shared__ float nb[SURR ? MLP_BN : 2];
-    __shared__ float nscr[SURR ? 4 * 64 * MLP_N4 : 2];
-    if (SURR) {
-        for (int j = threadIdx.x; j < MLP_WN; j += blockDim.x) nw[j] = a.net.w[j];
-        for (int j = threadIdx.x; j < MLP_BN; j += blockDim.x) nb[j] = a.net.b[j];
-    }
-    if (ADV) logtab_load(logtab, a.logtab);
-    if (PULSE) acc_clear(acc, 4 * a.pk.R);
-    else __syncthreads();
-    size_t lo, hi;
-    block_range(a.begin, a.end, lo, hi);
+__device__ __forceinline__ void debris_range(const DebrisK &a, size_t lo, size_t hi, double *acc, const float *logtab,
+                                             const uint16_t *nw, const float *nb, float *nscr) {
     const bool use_win = PULSE && a.pk.fold == RSD_FOLD_WINDOW;
-    if (use_win) stage_window(acc, a.pk);
     const float2 *wl = win_weights(acc, a.pk);
     const float *grl = win_gates(acc, a.pk);
     GateWin win;
@@ -1267,7 +1244,70 @@ __global__ __launch_bounds__(256) void k_debris(DebrisK a) {
         }
     }
     if (use_win) win_flush(win, a.pk, acc);
+}
+
+template <bool ADV, bool PULSE, bool SURR>
+__global__ __launch_bounds__(256) void k_debris(DebrisK a) {
+    extern __shared__ double acc[];
+    __shared__ float logtab[3 * RF_LOG_TABLE];
+    __shared__ uint16_t nw[SURR ? MLP_WN : 2];
+    __shared__ float nb[SURR ? MLP_BN : 2];
+    __shared__ float nscr[SURR ? 4 * 64 * MLP_N4 : 2];
+    if (SURR) {
+        for (int j = threadIdx.x; j < MLP_WN; j += blockDim.x) nw[j] = a.net.w[j];
+        for (int j = threadIdx.x; j < MLP_BN; j += blockDim.x) nb[j] = a.net.b[j];
+    }
+    if (ADV) logtab_load(logtab, a.logtab);
+    if (PULSE) acc_clear(acc, 4 * a.pk.R);
+    else __syncthreads();
+    size_t lo, hi;
+    block_range(a.begin, a.end, lo, hi);
+    if (PULSE && a.pk.fold == RSD_FOLD_WINDOW) stage_window(acc, a.pk);
+    debris_range<ADV, PULSE, SURR>(a, lo, hi, acc, logtab, nw, nb, nscr);
     if (PULSE) acc_flush(acc, a.pk);
+}
+
+
+// Rain launch of one step.  Debris types with small populations ride in the
+// same launch (no second stream, no cross-stream events per pulse): blocks
+// [first[j], first[j + 1]) take debris type j, the rain takes the blocks after
+// first[nt] (a multiple of 8, so the rain blocks keep their XCD mapping).
+#define RSD_STEP_DEBRIS 3
+struct StepDebris {
+    int nt;                             // debris types in this launch
+    int first[RSD_STEP_DEBRIS + 1];     // first block of each type; first[nt]: rain
+    DebrisK t[RSD_STEP_DEBRIS];
+};
+
+template <bool ADV, bool PULSE, bool HILO>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_rain(RainK a, StepDebris d) {
+    extern __shared__ double acc[];
+    __shared__ float logtab[3 * RF_LOG_TABLE];
+    if (ADV) logtab_load(logtab, a.logtab);
+    const unsigned nd = (unsigned)d.first[d.nt];
+    if (blockIdx.x < nd) {
+        int j = 0;
+        while (j + 1 < d.nt && (int)blockIdx.x >= d.first[j + 1]) j++;
+        const DebrisK &da = d.t[j];
+        if (PULSE) acc_clear(acc, 4 * da.pk.R);
+        else __syncthreads();
+        if (PULSE && da.pk.fold == RSD_FOLD_WINDOW) stage_window(acc, da.pk);
+        const unsigned b = blockIdx.x - (unsigned)d.first[j];
+        size_t lo, hi;
+        block_range_at(da.begin, da.end, b, (unsigned)(d.first[j + 1] - d.first[j]), lo, hi);
+        debris_range<ADV, PULSE, false>(da, lo, hi, acc, logtab, nullptr, nullptr, nullptr);
+        if (PULSE) acc_flush_row(acc, da.pk, da.pk.row0 + (int)b);
+        return;
+    }
+    if (PULSE) acc_clear(acc, 4 * a.pk.R);
+    else __syncthreads();
+    const unsigned b = blockIdx.x - nd;
+    size_t lo, hi;
+    block_range_at(0, a.n, b, gridDim.x - nd, lo, hi);
+    const bool use_win = PULSE && a.pk.fold == RSD_FOLD_WINDOW;
+    if (use_win) stage_window(acc, a.pk);
+    rain_range<ADV, PULSE, HILO>(a, lo, hi, acc, logtab, use_win);
+    if (PULSE) acc_flush_row(acc, a.pk, a.pk.row0 + (int)b);
 }
 
 // Sum the per-block partial rows of one pulse: one block per output value.
@@ -1615,6 +1655,7 @@ struct rsd_engine {
     hipEvent_t ev0, ev1;
     float *logtab;           // 3 x 128 floats of the wind-index log1p
     int prof_on, prof_n;
+    int prof_stride, prof_tick;  // time every prof_stride-th rain launch
     hipEvent_t *prof_ev;     // start/stop pairs around the rain kernel
     int *prof_kind;          // bit 0 advance, bit 1 pulse
 };
@@ -2016,21 +2057,45 @@ static int fill_debris(rsd_engine *e, const rsd_step *s, const rsd_pulse *p, int
     return RSD_OK;
 }
 
+// Debris populations small next to the rain ride in the rain launch; larger
+// ones, and types with a surrogate network, run on the side stream.
+static bool debris_rides(const rsd_engine *e, int k) {
+    const size_t c = e->counts[k];
+    return e->n_rain > 0 && !e->surr[k] && c > 0 && (c <= 65536 || c * 16 <= e->n_rain);
+}
+
 // Launch the kernels of one step: advance (s != NULL) and/or pulse (p != NULL).
 static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p) {
     HIPCHK(e, hipSetDevice(e->device));
     join_batch(e);
     const int R = p ? (int)p->range_count : 0;
     if (p && (R < 1 || R > RSD_MAX_GATES)) return fail(e, RSD_ERR_ARG, "range_count %d out of range", R);
-    // Rows of the partial-sum buffer: one per block of every launch.
+    // Rows of the partial-sum buffer: one per block of every launch, rain
+    // first, then each debris type in type order.
     int rain_blocks = e->n_rain ? blocks_for(e, e->n_rain) : 0;
     int deb_blocks[RSD_MAX_TYPES] = {0};
-    int rows = rain_blocks;
+    bool ride[RSD_MAX_TYPES] = {false};
+    int n_ride = 0, ride_blocks = 0, last_ride = -1;
     for (int k = 1; k < RSD_MAX_TYPES; k++) {
-        if (e->counts[k]) {
-            deb_blocks[k] = blocks_for(e, e->counts[k]);
-            rows += deb_blocks[k];
+        if (!e->counts[k]) continue;
+        deb_blocks[k] = blocks_for(e, e->counts[k]);
+        if (n_ride < RSD_STEP_DEBRIS && debris_rides(e, k)) {
+            ride[k] = true;
+            n_ride++;
+            ride_blocks += deb_blocks[k];
+            last_ride = k;
         }
+    }
+    if (last_ride > 0 && ride_blocks % 8) {  // the rain blocks keep their XCD mapping
+        deb_blocks[last_ride] += 8 - ride_blocks % 8;
+        ride_blocks += 8 - ride_blocks % 8;
+    }
+    int rows = rain_blocks, row_of[RSD_MAX_TYPES] = {0};
+    bool any_debris = false;       // types on the side stream
+    for (int k = 1; k < RSD_MAX_TYPES; k++) {
+        row_of[k] = rows;
+        rows += deb_blocks[k];
+        if (e->counts[k] && !ride[k]) any_debris = true;
     }
     PulseK pk;
     memset(&pk, 0, sizeof(pk));
@@ -2041,21 +2106,17 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p) {
         if (st) return st;
     }
     const size_t lds = p ? (size_t)4 * R * sizeof(double) + (pk.fold == RSD_FOLD_WINDOW ? (size_t)e->rangew_n * sizeof(float2) + (size_t)R * sizeof(float) : 0) : 0;
-    // Debris types run on the side stream, concurrently with the rain kernel;
-    // partial rows: rain blocks first, then each debris type.
-    const bool any_debris = rows > rain_blocks;
     if (any_debris) {
         HIPCHK(e, hipEventRecord(e->ev_fork, e->stream));
         HIPCHK(e, hipStreamWaitEvent(e->side, e->ev_fork, 0));
-        int row = rain_blocks;
         for (int k = 1; k < RSD_MAX_TYPES; k++) {
-            if (!e->counts[k]) continue;
+            if (!e->counts[k] || ride[k]) continue;
             DebrisK a;
             memset(&a, 0, sizeof(a));
             int st = fill_debris(e, s, p, k, a);
             if (st) return st;
             a.pk = pk;
-            a.pk.row0 = row;
+            a.pk.row0 = row_of[k];
             if (e->surr[k]) {
                 a.net = e->net[k];
                 if (s && p) hipLaunchKernelGGL((k_debris<true, true, true>), dim3(deb_blocks[k]), dim3(256), lds, e->side, a);
@@ -2067,7 +2128,6 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p) {
                 else hipLaunchKernelGGL((k_debris<false, true, false>), dim3(deb_blocks[k]), dim3(e->block), lds, e->side, a);
             }
             HIPCHK(e, hipGetLastError());
-            row += deb_blocks[k];
         }
         HIPCHK(e, hipEventRecord(e->ev_join, e->side));
     }
@@ -2078,17 +2138,30 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p) {
         if (st) return st;
         a.pk = pk;
         a.pk.row0 = 0;
-        const bool timed = e->prof_on && e->prof_n < RSD_PROF_CAP;
-        if (timed) HIPCHK(e, hipEventRecord(e->prof_ev[2 * e->prof_n], e->stream));
         a.plo = e->plo;
+        StepDebris d;
+        memset(&d, 0, sizeof(d));
+        for (int k = 1; k < RSD_MAX_TYPES; k++) {
+            if (!ride[k]) continue;
+            DebrisK &da = d.t[d.nt];
+            st = fill_debris(e, s, p, k, da);
+            if (st) return st;
+            da.pk = pk;
+            da.pk.row0 = row_of[k];
+            d.first[d.nt + 1] = d.first[d.nt] + deb_blocks[k];
+            d.nt++;
+        }
+        const dim3 grid(ride_blocks + rain_blocks);
+        const bool timed = e->prof_on && e->prof_n < RSD_PROF_CAP && (e->prof_tick++ % e->prof_stride) == 0;
+        if (timed) HIPCHK(e, hipEventRecord(e->prof_ev[2 * e->prof_n], e->stream));
         if (e->plo) {
-            if (s && p) hipLaunchKernelGGL((k_rain<true, true, true>), dim3(rain_blocks), dim3(e->block), lds, e->stream, a);
-            else if (s) hipLaunchKernelGGL((k_rain<true, false, true>), dim3(rain_blocks), dim3(e->block), 0, e->stream, a);
-            else hipLaunchKernelGGL((k_rain<false, true, true>), dim3(rain_blocks), dim3(e->block), lds, e->stream, a);
+            if (s && p) hipLaunchKernelGGL((k_rain<true, true, true>), grid, dim3(e->block), lds, e->stream, a, d);
+            else if (s) hipLaunchKernelGGL((k_rain<true, false, true>), grid, dim3(e->block), 0, e->stream, a, d);
+            else hipLaunchKernelGGL((k_rain<false, true, true>), grid, dim3(e->block), lds, e->stream, a, d);
         } else {
-            if (s && p) hipLaunchKernelGGL((k_rain<true, true, false>), dim3(rain_blocks), dim3(e->block), lds, e->stream, a);
-            else if (s) hipLaunchKernelGGL((k_rain<true, false, false>), dim3(rain_blocks), dim3(e->block), 0, e->stream, a);
-            else hipLaunchKernelGGL((k_rain<false, true, false>), dim3(rain_blocks), dim3(e->block), lds, e->stream, a);
+            if (s && p) hipLaunchKernelGGL((k_rain<true, true, false>), grid, dim3(e->block), lds, e->stream, a, d);
+            else if (s) hipLaunchKernelGGL((k_rain<true, false, false>), grid, dim3(e->block), 0, e->stream, a, d);
+            else hipLaunchKernelGGL((k_rain<false, true, false>), grid, dim3(e->block), lds, e->stream, a, d);
         }
         HIPCHK(e, hipGetLastError());
         if (timed) {
@@ -2630,7 +2703,10 @@ extern "C" int rsd_profile(rsd_engine *e, int enable) {
         if (!e->prof_ev || !e->prof_kind) return fail(e, RSD_ERR_ALLOC, "profile events");
         for (int i = 0; i < 2 * RSD_PROF_CAP; i++) HIPCHK(e, hipEventCreate(&e->prof_ev[i]));
     }
+    if (enable < 0) return fail(e, RSD_ERR_ARG, "profile stride %d", enable);
     e->prof_on = enable ? 1 : 0;
+    e->prof_stride = enable > 0 ? enable : 1;
+    e->prof_tick = 0;
     e->prof_n = 0;
     return RSD_OK;
 }
