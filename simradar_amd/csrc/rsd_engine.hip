@@ -704,9 +704,16 @@ __device__ __forceinline__ void wwin_flush(const WaveWin &w, const PulseK &k, do
     if (!(lane & 1u) && g >= 0 && g < k.R && v != 0.0f) atomicAdd(&acc[c * k.R + g], (double)v);
 }
 
-// One scatterer per lane (on: it has a contribution).  The slots a lane
-// touches need no mask: outside a scatterer's support the table weight is
-// exactly 0 (the window is only used for tables that end in zeros).
+// One scatterer per lane (on: it has a contribution).  The window is placed
+// once per launch, at the first step with contributions, one gate below the
+// median of the first, middle and last contributing lanes' first gates (a lane
+// far from the others, e.g. a respawned scatterer, cannot misplace it); a
+// launch's range-ordered scatterers then stay inside it, and those that do not
+// fold directly.  The slots a lane touches need no mask: outside a scatterer's
+// support the table weight is exactly 0 (the window is only used for tables
+// that end in zeros).
+__device__ __forceinline__ int median3(int a, int b, int c) { return max(min(a, b), min(max(a, b), c)); }
+
 __device__ __forceinline__ void wwin_fold(WaveWin &w, bool on, float r, float4 sa, const PulseK &k, double *acc,
                                           const float2 *wl, const float *grl) {
     int klo = 0x3fffffff, khi = -1;
@@ -714,17 +721,13 @@ __device__ __forceinline__ void wwin_fold(WaveWin &w, bool on, float r, float4 s
     on = klo <= khi;
     const unsigned long long on_m = __ballot(on);
     if (on_m == 0) return;
-    bool fit = on && klo >= w.kb && khi < w.kb + RSD_WIN;
-    const unsigned long long fit_m = __ballot(fit);
-    if (2 * __popcll(fit_m) < __popcll(on_m)) {
-        // Most of this step's scatterers lie outside: move the window to the
-        // first of them, one gate of room below.
-        wwin_flush(w, k, acc);
-        wwin_clear(w);
-        const int lead = __builtin_ctzll(on_m & ~fit_m);
-        w.kb = __builtin_amdgcn_readlane(klo, lead) - 1;
-        fit = on && klo >= w.kb && khi < w.kb + RSD_WIN;
+    if (w.kb == RSD_WIN_EMPTY) {
+        const int l0 = __builtin_ctzll(on_m), l2 = 63 - __builtin_clzll(on_m);
+        const int l1 = ((on_m >> 32) & 1ull) ? 32 : l0;
+        w.kb = median3(__builtin_amdgcn_readlane(klo, l0), __builtin_amdgcn_readlane(klo, l1),
+                       __builtin_amdgcn_readlane(klo, l2)) - 1;
     }
+    const bool fit = on && klo >= w.kb && khi < w.kb + RSD_WIN;
     if (on && !fit) lane_fold(r, sa, klo, khi, k, acc, wl, grl);
     if (!fit) sa = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
@@ -900,18 +903,29 @@ __device__ __forceinline__ void logtab_load(float *dst, const float *src) {
 
 // Scatterers [lo, hi) of one rain population: advance and/or pulse terms into
 // the block's LDS gate sums acc (cleared by the caller).
-template <bool ADV, bool PULSE, bool HILO>
-__device__ __forceinline__ void rain_range(const RainK &a, size_t lo, size_t hi, double *acc, const float *logtab,
-                                           bool use_win) {
+template <bool ADV, bool PULSE, bool HILO, bool WIN>
+__device__ __forceinline__ void rain_range_t(const RainK &a, size_t lo, size_t hi, double *acc, const float *logtab) {
     const float2 *wl = win_weights(acc, a.pk);
     const float *grl = win_gates(acc, a.pk);
     WaveWin win;
     win.kb = RSD_WIN_EMPTY;
     wwin_clear(win);
+    // pos / vel of the lane's next scatterer are loaded one step ahead
+    const float4 idle = make_float4(0.0f, 1.0f, 0.0f, 0.0f);
+    float4 pn = idle, vn = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (lo + threadIdx.x < hi) {
+        pn = a.pos[lo + threadIdx.x];
+        if (ADV) vn = a.vel[lo + threadIdx.x];
+    }
     for (size_t base = lo; base < hi; base += blockDim.x) {
         const size_t i = base + threadIdx.x;
         const bool live = i < hi;
-        float4 p = live ? a.pos[i] : make_float4(0.0f, 1.0f, 0.0f, 0.0f);
+        float4 p = pn;
+        const float4 v_in = vn;
+        if (i + blockDim.x < hi) {
+            pn = a.pos[i + blockDim.x];
+            if (ADV) vn = a.vel[i + blockDim.x];
+        }
         bool stale = __builtin_signbit(p.w);
         p.w = __builtin_fabsf(p.w);
         float4 rcs;
@@ -919,7 +933,7 @@ __device__ __forceinline__ void rain_range(const RainK &a, size_t lo, size_t hi,
         float4 plo = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (HILO && live) plo = a.plo[i];
         if (ADV && live) {
-            float4 v = a.vel[i];
+            float4 v = v_in;
             const float4 p_in = p;
             bool respawned = rain_advance<HILO>(a, i, p, v, logtab, plo);
             if (HILO) a.plo[i] = plo;
@@ -944,11 +958,21 @@ __device__ __forceinline__ void rain_range(const RainK &a, size_t lo, size_t hi,
                 if (!have_rcs) rcs = stale ? a.rcs[i] : ellipsoid_rcs(p, a.drop, a.drop_desc);
                 on = scat_signal<HILO>(p, rcs, a.pk, sa, r, plo);
             }
-            if (use_win) wwin_fold(win, on, r, sa, a.pk, acc, wl, grl);
+            if (WIN) wwin_fold(win, on, r, sa, a.pk, acc, wl, grl);
             else fold_gates(on, r, sa, a.pk, acc);
         }
     }
-    if (use_win) wwin_flush(win, a.pk, acc);
+    if (WIN) wwin_flush(win, a.pk, acc);
+}
+
+// The fold strategy is a template flag so that the loop carrying the window
+// registers has no second fold path (which makes the compiler copy the 32
+// window registers at every step).
+template <bool ADV, bool PULSE, bool HILO>
+__device__ __forceinline__ void rain_range(const RainK &a, size_t lo, size_t hi, double *acc, const float *logtab,
+                                           bool use_win) {
+    if (use_win) rain_range_t<ADV, PULSE, HILO, true>(a, lo, hi, acc, logtab);
+    else rain_range_t<ADV, PULSE, HILO, false>(a, lo, hi, acc, logtab);
 }
 
 // Many independent rain domains in one persistent launch.  items[t] = (domain,
