@@ -29,7 +29,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I" + INCLUDE, "-I" + CSRC,
 ]
-C_FLAGS = ["-O2", "-std=gnu11", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
+C_FLAGS = ["-O2", "-std=gnu11", "-fPIC", "-pthread", "-ffp-contract=off", "-fno-fast-math",
            "-Wall", "-Wextra", "-Wno-unused-parameter", "-I" + INCLUDE]
 
 HIP_SOURCES = ["rsd_engine.hip", "rsd_mlp.hip"]
@@ -57,12 +57,12 @@ def build(verbose=True):
         _run([CC] + C_FLAGS + ["-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-lm"])
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-lm", "-lpthread"])
     os.replace(tmp, LIB)
     # command-line driver (simradar.c equivalent), linked against the library
     os.makedirs(os.path.dirname(CLI), exist_ok=True)
     _run([CC] + C_FLAGS + [os.path.join(CSRC, "simradar_cli.c"), "-o", CLI + ".tmp", "-L" + HERE, "-lsimradar_amd",
-                           "-Wl,-rpath,$ORIGIN/..", "-lm"])
+                           "-Wl,-rpath,$ORIGIN/..", "-lm", "-lpthread"])
     os.replace(CLI + ".tmp", CLI)
     if verbose:
         print("built", LIB)

@@ -7,6 +7,7 @@
  */
 #include "rs_providers.h"
 
+#include <pthread.h>
 #include <stdarg.h>
 #include <time.h>
 
@@ -87,6 +88,12 @@ typedef struct {
     float cx, cy;           /* vortex centre offset from the grid centre (m) */
     uint64_t seed;
     LESTable **tables;
+    /* The frame after the one last handed out is generated ahead on a thread
+     * of its own, as the reference reads the next LES file ahead (les.c:268,
+     * les.c:666-692). */
+    pthread_t worker;
+    int worker_on;
+    uint32_t ahead;
 } LESMem;
 
 LESHandle LES_init_synthetic(uint32_t nx, uint32_t ny, uint32_t nz, float a_xy, float r_xy, float a_z, float r_z,
@@ -142,9 +149,17 @@ static void les_free_table(LESTable *t) {
     free(t);
 }
 
+static void les_join(LESMem *h) {
+    if (h->worker_on) {
+        pthread_join(h->worker, NULL);
+        h->worker_on = 0;
+    }
+}
+
 void LES_free(LESHandle i) {
     LESMem *h = (LESMem *)i;
     if (!h) return;
+    les_join(h);
     for (uint32_t f = 0; f < h->frames; f++) les_free_table(h->tables[f]);
     free(h->tables);
     free(h);
@@ -159,6 +174,45 @@ static double texel_position(int i, double offset, double a, double r) {
     return s * a * (pow(r, fabs(m)) - 1.0) / (r - 1.0);
 }
 
+/* One slab of z slices of a frame.  splitmix64 advances its state by a fixed
+ * increment per draw and every texel takes six draws, so a slab starts from the
+ * state the sequential walk would have there: the frame is the same for any
+ * number of slabs. */
+typedef struct {
+    const LESMem *h;
+    LESTable *t;
+    const double *xs, *ys;      /* texel positions along x and y */
+    uint64_t s0;                /* generator state before texel 0 */
+    uint32_t k0, k1;            /* z slices [k0, k1) */
+} LESSlab;
+
+static void *les_slab_run(void *arg) {
+    const LESSlab *p = (const LESSlab *)arg;
+    const LESMem *h = p->h;
+    for (uint32_t k = p->k0; k < p->k1; k++) {
+        size_t idx = (size_t)k * h->nx * h->ny;
+        uint64_t s = p->s0 + (uint64_t)(6 * idx) * 0x9E3779B97F4A7C15ull;
+        for (uint32_t j = 0; j < h->ny; j++) {
+            const double y = p->ys[j];
+            for (uint32_t i = 0; i < h->nx; i++, idx++) {
+                const double x = p->xs[i];
+                double xc = x - h->cx, yc = y - h->cy;
+                double rho = sqrt(xc * xc + yc * yc);
+                double vt = rho <= h->core ? h->vmax * rho / h->core : h->vmax * h->core / rho;
+                double u = rho > 0.0 ? -vt * yc / rho : 0.0;
+                double v = rho > 0.0 ? vt * xc / rho : 0.0;
+                p->t->uvwt[idx][0] = (float)(u + h->sigma * rs_gauss(&s));
+                p->t->uvwt[idx][1] = (float)(v + h->sigma * rs_gauss(&s));
+                p->t->uvwt[idx][2] = (float)(h->sigma * rs_gauss(&s));
+                p->t->uvwt[idx][3] = 0.0f;
+            }
+        }
+    }
+    return NULL;
+}
+
+#define LES_SLABS 8
+
 static LESTable *les_make_frame(LESMem *h, uint32_t f) {
     LESTable *t = (LESTable *)calloc(1, sizeof(LESTable));
     if (!t) return NULL;
@@ -172,37 +226,57 @@ static LESTable *les_make_frame(LESMem *h, uint32_t f) {
     t->ax = h->ax; t->ay = h->ax; t->az = h->az;
     t->rx = h->rx; t->ry = h->rx; t->rz = h->rz;
     t->uvwt = (LESFloat4 *)malloc((size_t)t->nn * sizeof(LESFloat4));
-    if (!t->uvwt) {
+    double *xs = (double *)malloc((size_t)(h->nx + h->ny) * sizeof(double));
+    if (!t->uvwt || !xs) {
+        free(t->uvwt);
+        free(xs);
         free(t);
         return NULL;
     }
+    double *ys = xs + h->nx;
     const double offx = 0.5 * (double)(h->nx - 1), offy = 0.5 * (double)(h->ny - 1);
-    uint64_t s = h->seed * 0x9E3779B97F4A7C15ull + (uint64_t)(f + 1) * 0xD1B54A32D192ED03ull;
-    size_t idx = 0;
-    for (uint32_t k = 0; k < h->nz; k++) {
-        for (uint32_t j = 0; j < h->ny; j++) {
-            double y = texel_position((int)j, offy, h->ax, h->rx);
-            for (uint32_t i = 0; i < h->nx; i++, idx++) {
-                double x = texel_position((int)i, offx, h->ax, h->rx);
-                double xc = x - h->cx, yc = y - h->cy;
-                double rho = sqrt(xc * xc + yc * yc);
-                double vt = rho <= h->core ? h->vmax * rho / h->core : h->vmax * h->core / rho;
-                double u = rho > 0.0 ? -vt * yc / rho : 0.0;
-                double v = rho > 0.0 ? vt * xc / rho : 0.0;
-                t->uvwt[idx][0] = (float)(u + h->sigma * rs_gauss(&s));
-                t->uvwt[idx][1] = (float)(v + h->sigma * rs_gauss(&s));
-                t->uvwt[idx][2] = (float)(h->sigma * rs_gauss(&s));
-                t->uvwt[idx][3] = 0.0f;
-            }
-        }
+    for (uint32_t i = 0; i < h->nx; i++) xs[i] = texel_position((int)i, offx, h->ax, h->rx);
+    for (uint32_t j = 0; j < h->ny; j++) ys[j] = texel_position((int)j, offy, h->ax, h->rx);
+    const uint64_t s0 = h->seed * 0x9E3779B97F4A7C15ull + (uint64_t)(f + 1) * 0xD1B54A32D192ED03ull;
+    const uint32_t nslab = h->nz < LES_SLABS ? h->nz : LES_SLABS;
+    LESSlab slab[LES_SLABS];
+    pthread_t tid[LES_SLABS];
+    int started[LES_SLABS] = {0};
+    for (uint32_t b = 0; b < nslab; b++) {
+        slab[b].h = h;
+        slab[b].t = t;
+        slab[b].xs = xs;
+        slab[b].ys = ys;
+        slab[b].s0 = s0;
+        slab[b].k0 = (uint32_t)((uint64_t)h->nz * b / nslab);
+        slab[b].k1 = (uint32_t)((uint64_t)h->nz * (b + 1) / nslab);
+        if (b > 0) started[b] = pthread_create(&tid[b], NULL, les_slab_run, &slab[b]) == 0;
     }
+    les_slab_run(&slab[0]);
+    for (uint32_t b = 1; b < nslab; b++) {
+        if (started[b]) pthread_join(tid[b], NULL);
+        else les_slab_run(&slab[b]);
+    }
+    free(xs);
     return t;
+}
+
+static void *les_ahead_run(void *arg) {
+    LESMem *h = (LESMem *)arg;
+    h->tables[h->ahead] = les_make_frame(h, h->ahead);
+    return NULL;
 }
 
 LESTable *LES_get_frame(const LESHandle i, const int n) {
     LESMem *h = (LESMem *)i;
     if (!h || n < 0 || (uint32_t)n >= h->frames) return NULL;
+    les_join(h);
     if (!h->tables[n]) h->tables[n] = les_make_frame(h, (uint32_t)n);
+    const uint32_t next = ((uint32_t)n + 1) % h->frames;
+    if (!h->tables[next]) {
+        h->ahead = next;
+        h->worker_on = pthread_create(&h->worker, NULL, les_ahead_run, h) == 0;
+    }
     return h->tables[n];
 }
 

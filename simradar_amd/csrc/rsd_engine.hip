@@ -8,7 +8,8 @@
 //   adm cd/cm, rcs re/im         : float4[ny][nx] per table
 //   small tables                 : drop RCS (96 float4), angular / range weights,
 //                                  binary32 gate ranges
-//   partials                     : double[rows][4][R] per-block gate sums
+//   partials                     : double[4 R][rows] per-block gate sums, one
+//                                  column per (component, gate)
 //
 // Rain keeps its radar cross-section out of memory: it is a function of the
 // drop radius and the position's elevation (rs.cl:268-280) and is recomputed in
@@ -20,16 +21,18 @@
 // |pos.w| and the reconstructed rcs, so the streams a caller sees are the
 // reference's.
 //
-// One pulse = one fused kernel per scatterer type (advance of the previous PRT,
-// if pending, then the signal of every scatterer folded into its gates) plus a
-// finalize kernel that sums the per-block gate partials in binary64.
+// One pulse = one fused kernel (advance of the previous PRT, if pending, then
+// the signal of every scatterer folded into its gates; small debris populations
+// in leading blocks of the rain launch, large ones and surrogate-driven types in
+// kernels of their own on a second stream) plus a finalize kernel that sums the
+// per-block gate partials in binary64.
 //
 // Spatial order: every sort_interval pulses the rain streams (pos, vel, rcs,
 // rnd) are reordered by range (shells of 1/8 gate) and, inside a shell, by
 // a Morton index of 16^3 cells of the domain box, with id[j] = the reference
 // index of stored scatterer j.  Each scatterer keeps its own state and random
-// stream, so the physics is unchanged; the scatterers a lane visits then fall
-// into the same few range gates (summed in registers, see GateWin) and
+// stream, so the physics is unchanged; the scatterers a wave visits then fall
+// into the same few range gates (summed in registers, see WaveWin) and
 // neighbouring lanes gather neighbouring wind texels.  Downloads scatter through
 // id and uploads first restore the reference order, so callers never see it.
 // aux (read only by downloads) stays in the reference order.
@@ -296,8 +299,9 @@ struct PulseK {
     int dense;              // 1: every gate may be weighted
     int fold;               // RSD_FOLD_* strategy of the gate sums
     uint32_t phase_mode;
-    double *partials;       // [rows][4][R]
+    double *partials;       // [4 R][rows]: column j = component j / R, gate j % R
     int row0;               // first partial row of this launch
+    int rows;               // rows of the partial buffer (column length)
 };
 
 __device__ __forceinline__ float table_lerp(float x, const float *W, const F4 &d) {
@@ -747,7 +751,9 @@ __device__ __forceinline__ void wwin_fold(WaveWin &w, bool on, float r, float4 s
 // [L chunk, (L + 1) chunk).  Blocks are dispatched round-robin over the 8 XCDs,
 // so L = (b mod 8) (G / 8) + b / 8 gives each XCD one contiguous eighth of the
 // (spatially sorted) streams and keeps its L2 on one region of the wind table.
-__device__ __forceinline__ void block_range_at(size_t begin, size_t end, unsigned b, unsigned G, size_t &lo, size_t &hi) {
+// Returns the logical block L.
+__device__ __forceinline__ unsigned block_range_at(size_t begin, size_t end, unsigned b, unsigned G, size_t &lo,
+                                                   size_t &hi) {
     const unsigned L = (G & 7u) == 0u ? (b & 7u) * (G >> 3) + (b >> 3) : b;
     const size_t n = end - begin;
     size_t chunk = (n + G - 1) / G;
@@ -756,10 +762,11 @@ __device__ __forceinline__ void block_range_at(size_t begin, size_t end, unsigne
     hi = lo + chunk;
     if (lo > end) lo = end;
     if (hi > end) hi = end;
+    return L;
 }
 
 __device__ __forceinline__ void block_range(size_t begin, size_t end, size_t &lo, size_t &hi) {
-    block_range_at(begin, end, blockIdx.x, gridDim.x, lo, hi);
+    (void)block_range_at(begin, end, blockIdx.x, gridDim.x, lo, hi);
 }
 
 __device__ __forceinline__ void acc_clear(double *acc, int n) {
@@ -767,10 +774,11 @@ __device__ __forceinline__ void acc_clear(double *acc, int n) {
     __syncthreads();
 }
 
+// Row `row` of the transposed partial buffer: one value per column.  Logical
+// blocks on one XCD own consecutive rows, so its L2 merges them into whole lines.
 __device__ __forceinline__ void acc_flush_row(const double *acc, const PulseK &k, int row) {
     __syncthreads();
-    double *dst = k.partials + (size_t)row * 4 * k.R;
-    for (int j = threadIdx.x; j < 4 * k.R; j += blockDim.x) dst[j] = acc[j];
+    for (int j = threadIdx.x; j < 4 * k.R; j += blockDim.x) k.partials[(size_t)j * k.rows + row] = acc[j];
 }
 
 __device__ __forceinline__ void acc_flush(const double *acc, const PulseK &k) {
@@ -1005,6 +1013,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 
 // Finalize of a batch: block j belongs to the domain d with first[d] <= j <
 // first[d + 1] and sums that domain's partial rows for output j - first[d].
+// Binary64 sum of column j (rows values) by one 256-thread block, in a fixed
+// order: per thread r = t, t + 256, ... (loads issued 8 at a time), then a DPP
+// wave sum, then the four wave sums in order.
+__device__ __forceinline__ double column_sum(const double *col, int rows) {
+    __shared__ double red[4];
+    double s = 0.0;
+    int r = threadIdx.x;
+    for (; r + 7 * 256 < rows; r += 8 * 256) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = col[r + u * 256];
+#pragma unroll
+        for (int u = 0; u < 8; u++) s += v[u];
+    }
+    for (; r < rows; r += 256) s += col[r];
+    s = wave_sum64(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    return ((red[0] + red[1]) + red[2]) + red[3];
+}
+
 struct FinK {
     const double *partials;
     double *out, *out2;
@@ -1012,23 +1041,15 @@ struct FinK {
 };
 
 __global__ __launch_bounds__(256) void k_finalize_batch(const FinK *f, const int *first, int nd) {
-    __shared__ double red[256];
     int d = 0;
     while (d + 1 < nd && first[d + 1] <= (int)blockIdx.x) d++;
     const FinK a = f[d];
     const int j = (int)blockIdx.x - first[d];
     const int R = a.n4r / 4;
-    double s = 0.0;
-    for (int r = threadIdx.x; r < a.rows; r += blockDim.x) s += a.partials[(size_t)r * a.n4r + j];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
+    const double s = column_sum(a.partials + (size_t)j * a.rows, a.rows);
     if (threadIdx.x == 0) {
-        a.out[4 * (j % R) + j / R] = red[0];
-        if (a.out2) a.out2[4 * (j % R) + j / R] = red[0];
+        a.out[4 * (j % R) + j / R] = s;
+        if (a.out2) a.out2[4 * (j % R) + j / R] = s;
     }
 }
 
@@ -1316,41 +1337,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         if (PULSE) acc_clear(acc, 4 * da.pk.R);
         else __syncthreads();
         if (PULSE && da.pk.fold == RSD_FOLD_WINDOW) stage_window(acc, da.pk);
-        const unsigned b = blockIdx.x - (unsigned)d.first[j];
         size_t lo, hi;
-        block_range_at(da.begin, da.end, b, (unsigned)(d.first[j + 1] - d.first[j]), lo, hi);
+        const unsigned L = block_range_at(da.begin, da.end, blockIdx.x - (unsigned)d.first[j],
+                                          (unsigned)(d.first[j + 1] - d.first[j]), lo, hi);
         debris_range<ADV, PULSE, false>(da, lo, hi, acc, logtab, nullptr, nullptr, nullptr);
-        if (PULSE) acc_flush_row(acc, da.pk, da.pk.row0 + (int)b);
+        if (PULSE) acc_flush_row(acc, da.pk, da.pk.row0 + (int)L);
         return;
     }
     if (PULSE) acc_clear(acc, 4 * a.pk.R);
     else __syncthreads();
-    const unsigned b = blockIdx.x - nd;
     size_t lo, hi;
-    block_range_at(0, a.n, b, gridDim.x - nd, lo, hi);
+    const unsigned L = block_range_at(0, a.n, blockIdx.x - nd, gridDim.x - nd, lo, hi);
     const bool use_win = PULSE && a.pk.fold == RSD_FOLD_WINDOW;
     if (use_win) stage_window(acc, a.pk);
     rain_range<ADV, PULSE, HILO>(a, lo, hi, acc, logtab, use_win);
-    if (PULSE) acc_flush_row(acc, a.pk, a.pk.row0 + (int)b);
+    if (PULSE) acc_flush_row(acc, a.pk, a.pk.row0 + (int)L);
 }
 
-// Sum the per-block partial rows of one pulse: one block per output value.
-// Rows hold [component][gate]; the pulse buffer is [gate][component].
+// Sum the per-block partial rows of one pulse: one block per output value,
+// reading its contiguous column.  Columns are [component][gate]; the pulse
+// buffer is [gate][component].
 __global__ __launch_bounds__(256) void k_finalize(const double *partials, int rows, int n4r, double *out, double *out2) {
-    __shared__ double red[256];
     const int j = blockIdx.x;
     const int R = n4r / 4;
-    double s = 0.0;
-    for (int r = threadIdx.x; r < rows; r += blockDim.x) s += partials[(size_t)r * n4r + j];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
+    const double s = column_sum(partials + (size_t)j * rows, rows);
     if (threadIdx.x == 0) {
-        out[4 * (j % R) + j / R] = red[0];
-        if (out2) out2[4 * (j % R) + j / R] = red[0];
+        out[4 * (j % R) + j / R] = s;
+        if (out2) out2[4 * (j % R) + j / R] = s;
     }
 }
 
@@ -1613,6 +1626,10 @@ struct rsd_engine {
     hipStream_t stream;
     hipStream_t side;       // debris kernels, forked from / joined into stream
     hipEvent_t ev_fork, ev_join;
+    float4 *wind_stage[2];      // pinned staging of the stream-ordered wind uploads
+    size_t wind_stage_n[2];     // texels of each staging buffer
+    int wind_ok[2];             // slot holds an uploaded table
+    hipEvent_t ev_wind[2];      // last copy out of each staging buffer
     char err[512];
     size_t n, n_rain;
     size_t origins[RSD_MAX_TYPES], counts[RSD_MAX_TYPES];
@@ -1841,7 +1858,11 @@ extern "C" void rsd_close(rsd_engine *e) {
     }
     (void)hipStreamSynchronize(e->stream);
     free_state(e);
-    for (int s = 0; s < 2; s++) dfree(e->wind[s]);
+    for (int s = 0; s < 2; s++) {
+        dfree(e->wind[s]);
+        if (e->wind_stage[s]) (void)hipHostFree(e->wind_stage[s]);
+        if (e->ev_wind[s]) (void)hipEventDestroy(e->ev_wind[s]);
+    }
     for (int t = 0; t < RSD_MAX_TABLES; t++) {
         dfree(e->adm_cd[t]); dfree(e->adm_cm[t]); dfree(e->rcs_re[t]); dfree(e->rcs_im[t]);
     }
@@ -2028,7 +2049,7 @@ static int fill_pulse(rsd_engine *e, const rsd_pulse *p, PulseK &k) {
 
 static int fill_rain(rsd_engine *e, const rsd_step *s, RainK &a) {
     int slot = s ? s->wind_slot : 0;
-    if (s && (!e->wind[slot])) return fail(e, RSD_ERR_STATE, "wind slot %d not set", slot);
+    if (s && (!e->wind_ok[slot])) return fail(e, RSD_ERR_STATE, "wind slot %d not set", slot);
     if (!e->drop) return fail(e, RSD_ERR_STATE, "drop RCS table not set");
     a.pos = e->pos; a.vel = e->vel; a.rcs = e->rcs; a.rnd = e->rnd;
     a.n = e->n_rain;
@@ -2049,7 +2070,7 @@ static int fill_debris(rsd_engine *e, const rsd_step *s, const rsd_pulse *p, int
     a.logtab = e->logtab;
     if (s) {
         int slot = s->wind_slot, ai = s->adm_index[type], ri = s->rcs_index[type];
-        if (!e->wind[slot]) return fail(e, RSD_ERR_STATE, "wind slot %d not set", slot);
+        if (!e->wind_ok[slot]) return fail(e, RSD_ERR_STATE, "wind slot %d not set", slot);
         if (ai < 0 || ai >= RSD_MAX_TABLES || !e->adm_cd[ai]) return fail(e, RSD_ERR_STATE, "ADM table %d not set", ai);
         if (ri < 0 || ri >= RSD_MAX_TABLES || !e->rcs_re[ri]) return fail(e, RSD_ERR_STATE, "RCS table %d not set", ri);
         a.wind.d = e->wind[slot]; a.wind.nx = e->wnx[slot]; a.wind.ny = e->wny[slot]; a.wind.nz = e->wnz[slot];
@@ -2128,6 +2149,7 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p) {
         if (st) return st;
         st = fill_pulse(e, p, pk);
         if (st) return st;
+        pk.rows = rows;
     }
     const size_t lds = p ? (size_t)4 * R * sizeof(double) + (pk.fold == RSD_FOLD_WINDOW ? (size_t)e->rangew_n * sizeof(float2) + (size_t)R * sizeof(float) : 0) : 0;
     if (any_debris) {
@@ -2519,13 +2541,41 @@ extern "C" int rsd_set_wind(rsd_engine *e, int slot, const float *uvwt, int nx, 
     if (st) return st;
     const size_t texels = (size_t)nx * ny * nz;
     if (e->wind[slot] && e->wnx[slot] == nx && e->wny[slot] == ny && e->wnz[slot] == nz) {
-        HIPCHK(e, hipMemcpyAsync(e->wind[slot], uvwt, texels * sizeof(float4), hipMemcpyHostToDevice, e->stream));
-        HIPCHK(e, hipStreamSynchronize(e->stream));
+        // New frame into a slot of the same shape (the LES switch of
+        // RS_advance_time): a stream-ordered copy through pinned staging, so
+        // the queue keeps running; launches already enqueued finish first.
+        join_batch(e);
+        if (!e->ev_wind[slot]) HIPCHK(e, hipEventCreateWithFlags(&e->ev_wind[slot], hipEventDisableTiming));
+        else HIPCHK(e, hipEventSynchronize(e->ev_wind[slot]));   // the staging buffer's last copy is done
+        if (e->wind_stage_n[slot] < texels) {
+            if (e->wind_stage[slot]) HIPCHK(e, hipHostFree(e->wind_stage[slot]));
+            e->wind_stage[slot] = NULL;
+            e->wind_stage_n[slot] = 0;
+            HIPCHK(e, hipHostMalloc((void **)&e->wind_stage[slot], texels * sizeof(float4), hipHostMallocDefault));
+            e->wind_stage_n[slot] = texels;
+        }
+        memcpy(e->wind_stage[slot], uvwt, texels * sizeof(float4));
+        HIPCHK(e, hipMemcpyAsync(e->wind[slot], e->wind_stage[slot], texels * sizeof(float4), hipMemcpyHostToDevice,
+                                 e->stream));
+        HIPCHK(e, hipEventRecord(e->ev_wind[slot], e->stream));
     } else {
         st = upload_tex(e, &e->wind[slot], uvwt, texels);
         if (st) return st;
+        // the other slot takes the next frame of the same shape: allocate it now
+        // so that the switch needs no allocation (and no device-wide wait)
+        const int o = 1 - slot;
+        if (!e->wind[o]) {
+            HIPCHK(e, hipMalloc((void **)&e->wind[o], texels * sizeof(float4)));
+            e->wnx[o] = nx; e->wny[o] = ny; e->wnz[o] = nz;
+            e->wind_ok[o] = 0;
+            if (!e->wind_stage[o]) {
+                HIPCHK(e, hipHostMalloc((void **)&e->wind_stage[o], texels * sizeof(float4), hipHostMallocDefault));
+                e->wind_stage_n[o] = texels;
+            }
+        }
     }
     e->wnx[slot] = nx; e->wny[slot] = ny; e->wnz[slot] = nz;
+    e->wind_ok[slot] = 1;
     memcpy(e->les[slot].s, desc, sizeof(e->les[slot].s));
     return RSD_OK;
 }
@@ -3062,6 +3112,7 @@ extern "C" int rsd_batch_launch(rsd_batch *b) {
         st = fill_pulse(e, p, a.pk);
         if (st) return bfail(b, st, e->err);
         a.pk.row0 = 0;
+        a.pk.rows = rows;
         a.plo = NULL;
         rk[k] = a;
         FinK f;

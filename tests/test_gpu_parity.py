@@ -197,3 +197,32 @@ def test_wave_transpose_sum_layout(gpu_ready):
     assert np.array_equal(out.astype(np.float64), want), (out[:16], want[:16])
     with pytest.raises(RuntimeError):
         sa.math_probe(11, x[:100], y[:100])
+
+
+def test_les_frame_switch_is_stream_ordered(gpu_ready):
+    """The LES frame switch (every 2 s of simulated time, here every 200 pulses
+    of 10 ms) copies the next frame on the engine's stream while the queue is
+    full: across two switches the state matches, bit for bit, a run that waits
+    for the device before every step."""
+    cfg = h.Config(n_rain=8192, debris=[(sa.OBJ_LEAF, 512)], scan="P:3.0,-1:1:0.01", prt=0.01)
+    runs = []
+    for drain in (False, True):
+        r = sa.Radar(verb=0)
+        h.configure(r, cfg)
+        r.populate()
+        r.check()
+        for _ in range(450):
+            r.make_pulse()
+            if drain:
+                r.sync()
+            r.advance_time()
+            r.advance_beam()
+        r.make_pulse()
+        r.download()
+        r.check()
+        runs.append({s: r.stream(s).copy() for s in ("pos", "vel", "ori", "tum")})
+        runs[-1]["iq"] = r.pulse_f64()
+        r.free()
+    for s in ("pos", "vel", "ori", "tum"):
+        assert np.array_equal(_bits(runs[0][s]), _bits(runs[1][s])), s
+    assert np.array_equal(runs[0]["iq"], runs[1]["iq"])
