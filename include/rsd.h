@@ -261,9 +261,11 @@ int rsd_set_debris_surrogate(rsd_engine *e, int type, const rsd_mlp *m);
 
 /* Self-test: evaluate one of the device's binary32 functions on n inputs
  * (fn: 0 log1p(a), 1 log(a), 2 sin(a), 3 cos(a), 4 atan2(a, b), 5 acos(a),
- * 6 sqrt(a), 7 a / b, 8 sin(2 pi a), 9 cos(2 pi a), 10 the pulse path's fast acos(a)).
- * No counterpart in the
- * reference; it lets a caller check the device arithmetic the advection uses. */
+ * 6 sqrt(a), 7 a / b, 8 sin(2 pi a), 9 cos(2 pi a), 10 the pulse path's fast
+ * acos(a), 11 the gate window's transposed wave sum of a + j b, j = 0..31 (n a
+ * multiple of 64), 12 the wind index's log1p(a) for a >= 0).  No counterpart in
+ * the reference; it lets a caller check the device arithmetic the advection
+ * uses. */
 int rsd_math_probe(int device_index, int fn, const float *a, const float *b, float *out, size_t n);
 
 #ifdef __cplusplus

@@ -121,9 +121,15 @@ __device__ __forceinline__ float4 wind_index(float4 pos, const F16 &d, const F16
         float cx = sd.s[8] + 0.5f * sd.s[12];
         float cy = sd.s[9] + 0.5f * sd.s[13];
         float rx = pos.x - cx, ry = pos.y - cy, rz = pos.z - 0.0f;
-        c.x = __builtin_copysignf(d.s[0], rx) * rf_log1pf(d.s[4] * __builtin_fabsf(rx), logtab) + d.s[8];
-        c.y = __builtin_copysignf(d.s[1], ry) * rf_log1pf(d.s[5] * __builtin_fabsf(ry), logtab) + d.s[9];
-        c.z = __builtin_copysignf(d.s[2], rz) * rf_log1pf(d.s[6] * __builtin_fabsf(rz), logtab) + d.s[10];
+        if (d.s[4] >= 0.0f && d.s[5] >= 0.0f && d.s[6] >= 0.0f) {   // growing spacing: arguments >= 0
+            c.x = __builtin_copysignf(d.s[0], rx) * rf_log1p_pos(d.s[4] * __builtin_fabsf(rx), logtab) + d.s[8];
+            c.y = __builtin_copysignf(d.s[1], ry) * rf_log1p_pos(d.s[5] * __builtin_fabsf(ry), logtab) + d.s[9];
+            c.z = __builtin_copysignf(d.s[2], rz) * rf_log1p_pos(d.s[6] * __builtin_fabsf(rz), logtab) + d.s[10];
+        } else {
+            c.x = __builtin_copysignf(d.s[0], rx) * rf_log1pf(d.s[4] * __builtin_fabsf(rx), logtab) + d.s[8];
+            c.y = __builtin_copysignf(d.s[1], ry) * rf_log1pf(d.s[5] * __builtin_fabsf(ry), logtab) + d.s[9];
+            c.z = __builtin_copysignf(d.s[2], rz) * rf_log1pf(d.s[6] * __builtin_fabsf(rz), logtab) + d.s[10];
+        }
     } else if (spacing == 0u) {
         c.x = rf_fma(pos.x, d.s[0], d.s[4]);
         c.y = rf_fma(pos.y, d.s[1], d.s[5]);
@@ -1431,6 +1437,7 @@ __global__ void k_math_probe(int fn, const float *a, const float *b, float *out,
     float r = 0.0f;
     switch (fn) {
         case 0: r = rf_log1pf(x, logtab); break;
+        case 12: r = rf_log1p_pos(x, logtab); break;
         case 10: r = rf_acos_fast(x); break;
         case 1: r = rf_logf(x); break;
         case 2: rf_sincosf(x, &s, &c); r = s; break;

@@ -111,6 +111,28 @@ __device__ __forceinline__ float rf_log1pf(float x, const float *tab) {
     return hi + (p + lo);
 }
 
+// rf_log1pf for finite x >= 0 (the stretched wind index): the same series
+// below 2^-7 and the same table form above, without the branches for the
+// cases such an argument cannot take (equal results there: the series returns
+// +0 for +0).
+__device__ __forceinline__ float rf_log1p_pos(float x, const float *tab) {
+    if (x < 0x1p-7f) return rf_fma(x * x, rf_fma(x, rf_fma(x, -0.25f, 0x1.555556p-2f), -0.5f), x);
+    float u = 1.0f + x;
+    float tail = (u < 0x1p+24f) ? x - (u - 1.0f) : 0.0f;
+    uint32_t iu = __float_as_uint(u);
+    int k = (int)((iu >> 23) & 0xffu) - 127;
+    int j = (int)((iu >> 16) & 0x7fu);
+    float m = __uint_as_float((iu & 0x007fffffu) | 0x3f800000u);
+    float g = tab[j];
+    float r = rf_fma(m, g, -1.0f);
+    float p = rf_fma(r * r, rf_fma(r, rf_fma(r, -0.25f, 0x1.555556p-2f), -0.5f), r);
+    float kf = (float)k;
+    float scale = __uint_as_float((uint32_t)(127 - k) << 23);
+    float lo = rf_fma(kf, RF_LN2_LO, tab[2 * RF_LOG_TABLE + j]) + (tail * g) * scale;
+    float hi = rf_fma(kf, RF_LN2_HI, tab[RF_LOG_TABLE + j]);
+    return hi + (p + lo);
+}
+
 // Faster forms for quantities that only enter the pulse sums (checked against
 // the oracle within the IQ tolerance, not bit for bit): hardware reciprocal
 // (about 1 ulp) and acos from asin(x) = x + x^3 P(x^2) on |x| <= 1/2 (P a
