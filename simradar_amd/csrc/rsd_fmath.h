@@ -181,12 +181,12 @@ __device__ __forceinline__ void rf_sincos_poly(float r, int n, float *s_out, flo
     float s = rf_fma(r * z, ps, r);
     float pc = rf_fma(z, rf_fma(z, rf_fma(z, rf_fma(z, -0x1.27e4fcp-22f, 0x1.a01a02p-16f), -0x1.6c16c2p-10f), 0x1.555556p-5f), -0.5f);
     float c = rf_fma(z, pc, 1.0f);
-    switch (n & 3) {
-        case 0: *s_out = s; *c_out = c; break;
-        case 1: *s_out = c; *c_out = -s; break;
-        case 2: *s_out = -s; *c_out = -c; break;
-        default: *s_out = -c; *c_out = s; break;
-    }
+    // quadrant n mod 4: (s, c), (c, -s), (-s, -c), (-c, s) by selects and sign
+    // flips (a divergent switch would run every case under masks)
+    const uint32_t q = (uint32_t)n;
+    const float a = (q & 1u) ? c : s, b = (q & 1u) ? s : c;
+    *s_out = __uint_as_float(__float_as_uint(a) ^ ((q & 2u) << 30));
+    *c_out = __uint_as_float(__float_as_uint(b) ^ (((q + 1u) & 2u) << 30));
 }
 
 __device__ __forceinline__ void rf_sincosf(float x, float *s_out, float *c_out) {
