@@ -5,9 +5,14 @@
  * obj.h:13-47, pos.h:22-66 and log.h, because RS_* signatures and the
  * RSHandle fields callers read are written in these types.  The reference's
  * providers read external data files (LES Fortran records, .adm, .rcs) that are
- * not part of the source tree; here every configuration is generated in memory
- * from documented analytic fields and a seeded generator, so a run is
- * reproducible on any machine:
+ * not part of the source tree.  These providers read the same files when they
+ * are found where the reference looks for them (the given path, or
+ * <cwd>/Contents/Resources/..., $HOME/Documents/tables, $HOME/Downloads/tables,
+ * $HOME/tables, and for LES also $HOME/{Desktop,Documents,Downloads}/les:
+ * les/<config>/fort.10_2 + LES_mean_1_6_fnum<k>.dat, adm/<name>.adm,
+ * rcs/<name>.rcs); otherwise every configuration is generated in memory from
+ * documented analytic fields and a seeded generator, so a run is reproducible
+ * on any machine:
  *
  *   LES "suctvort": 176 x 176 x 80 stretched grid (a = 2 m, r = 1.0212 in x/y,
  *     1.05 in z; les.c:159-171), Rankine vortex at the grid centre (core 50 m,
@@ -100,6 +105,13 @@ void ADM_dimension_set(ADMTable *T, const float x, const float y, const float z,
 /* nb x na grid (beta fastest) of a synthetic air-drag model; seed selects the noise. */
 ADMTable *ADM_table_synthetic(const char *name, uint32_t nb, uint32_t na, float noise, uint64_t seed);
 void ADM_table_free(ADMTable *T);
+/* One .adm file (adm.c:152-225 layout); NULL if it cannot be read. */
+ADMTable *ADM_table_read(const char *path);
+/* adm.h handles: NULL when no folder with adm/square_plate.adm is found. */
+ADMHandle ADM_init_with_path(const char *path);
+ADMHandle ADM_init(void);
+ADMTable *ADM_get_table(const ADMHandle, const ADMConfig);
+void ADM_free(ADMHandle);
 
 /* ---- RCS (rcs.h) ---- */
 #define RCSConfigLeaf      "leaf"
@@ -126,6 +138,13 @@ typedef struct _rcs_table {
 /* na x nb grid (alpha fastest) of synthetic complex hh/vv/hv amplitudes. */
 RCSTable *RCS_table_synthetic(const char *name, uint32_t na, uint32_t nb, float scale, float noise, uint64_t seed);
 void RCS_table_free(RCSTable *T);
+/* One .rcs file (rcs.c:181-246 layout); NULL if it cannot be read. */
+RCSTable *RCS_table_read(const char *path);
+/* rcs.h handles: NULL when no folder with rcs/leaf.rcs is found. */
+RCSHandle RCS_init_with_path(const char *path);
+RCSHandle RCS_init(void);
+RCSTable *RCS_get_table(const RCSHandle, const RCSConfig);
+void RCS_free(RCSHandle);
 
 /* ---- OBJ (obj.h) ---- */
 typedef int OBJConfig;
@@ -147,6 +166,9 @@ typedef struct _obj_table {
 
 typedef void *OBJHandle;
 OBJHandle OBJ_init(void);
+/* Tables of a type come from files when the ADM / RCS folders are found from
+ * path as adm.c / rcs.c search them, else from the synthetic generators. */
+OBJHandle OBJ_init_with_path(const char *path);
 void OBJ_free(OBJHandle);
 OBJTable *OBJ_get_table(const OBJHandle, OBJConfig type);
 /* The t-th table handed out by OBJ_get_table (NULL beyond the count). */

@@ -9,7 +9,10 @@
 
 #include <pthread.h>
 #include <stdarg.h>
+#include <strings.h>
+#include <sys/stat.h>
 #include <time.h>
+#include <unistd.h>
 
 /* ------------------------------------------------------------------ logging */
 
@@ -94,7 +97,57 @@ typedef struct {
     pthread_t worker;
     int worker_on;
     uint32_t ahead;
+    /* Frames read from the reference's LES files instead of generated. */
+    int file_mode;
+    char dir[2200];         /* <tables>/les/<config> */
+    float v0, p0, t0, ay, ry;
+    int stretched, p_for_cn2;
 } LESMem;
+
+/* ---------------------------------------------------------- table files */
+
+/* First directory of dirs[0..n) holding the file probe (relative to it). */
+static int find_table_dir(char dirs[][1024], int n, const char *probe, char *out) {
+    struct stat st;
+    char f[2200];
+    for (int i = 0; i < n; i++) {
+        if (!dirs[i][0]) continue;
+        snprintf(f, sizeof(f), "%s/%s", dirs[i], probe);
+        if (stat(dirs[i], &st) == 0 && S_ISDIR(st.st_mode) && stat(f, &st) == 0 && S_ISREG(st.st_mode)) {
+            snprintf(out, 1024, "%s", dirs[i]);
+            return 1;
+        }
+    }
+    return 0;
+}
+
+/* Search list of adm.c:56-79 / rcs.c:92-114 (sub "les" for ADM, "rcs" for
+ * RCS) and les.c:74-91 (les = 1): the given path (LES: path/tables) or
+ * <cwd>/Contents/Resources/<sub>, then folders under $HOME. */
+static int table_search_list(const char *path, const char *sub, int les, char dirs[8][1024]) {
+    int k = 0;
+    if (path) {
+        snprintf(dirs[k++], 1024, les ? "%s/tables" : "%s", path);
+    } else {
+        char cwd[960];
+        if (getcwd(cwd, sizeof(cwd))) snprintf(dirs[k++], 1024, "%s/Contents/Resources/%s", cwd, sub);
+    }
+    const char *home = getenv("HOME");
+    if (home) {
+        snprintf(dirs[k++], 1024, "%s/Documents/tables", home);
+        snprintf(dirs[k++], 1024, "%s/Downloads/tables", home);
+        if (les) {
+            snprintf(dirs[k++], 1024, "%s/Desktop/les", home);
+            snprintf(dirs[k++], 1024, "%s/Documents/les", home);
+            snprintf(dirs[k++], 1024, "%s/Downloads/les", home);
+        } else {
+            snprintf(dirs[k++], 1024, "%s/tables", home);
+        }
+    }
+    return k;
+}
+
+#define LES_FILE_NBLOCK 10      /* volumes per LES data file (les.c:12) */
 
 LESHandle LES_init_synthetic(uint32_t nx, uint32_t ny, uint32_t nz, float a_xy, float r_xy, float a_z, float r_z,
                              float core_m, float vmax, float sigma, uint64_t seed, uint32_t frames) {
@@ -115,8 +168,109 @@ LESHandle LES_init_synthetic(uint32_t nx, uint32_t ny, uint32_t nz, float a_xy, 
     return (LESHandle)h;
 }
 
+/* LES from the reference's files (les.c:71-291, 318-422, 474-517): the grid
+ * record <dir>/les/<config>/fort.10_2 (uint32 rev, nx, ny, nz, then the x, y and
+ * z planes, each after 8 bytes) and the data files LES_mean_1_6_fnum<k>.dat
+ * (uint32 version, then per volume a float time stamp and the u, v, w, p, t
+ * planes, each record followed by 8 bytes), scaled by v0 / p0 / t0 and given
+ * the grid spacings of the configuration as les.c:154-199 does. */
+static LESHandle les_init_files(const char *config, const char *dir) {
+    char f[2400];
+    snprintf(f, sizeof(f), "%s/les/%s/fort.10_2", dir, config);
+    FILE *g = fopen(f, "rb");
+    if (!g) return NULL;
+    uint32_t hdr[4];
+    if (fread(hdr, sizeof(uint32_t), 4, g) != 4 || hdr[1] < 2 || hdr[2] < 2 || hdr[3] < 2) {
+        fclose(g);
+        return NULL;
+    }
+    LESMem *h = (LESMem *)calloc(1, sizeof(LESMem));
+    if (!h) {
+        fclose(g);
+        return NULL;
+    }
+    h->file_mode = 1;
+    snprintf(h->dir, sizeof(h->dir), "%s/les/%s", dir, config);
+    h->nx = hdr[1]; h->ny = hdr[2]; h->nz = hdr[3];
+    const size_t nn = (size_t)h->nx * h->ny * h->nz;
+    h->tr = 50.0f;
+    h->v0 = 100.0f;
+    h->p0 = h->v0 * h->v0;
+    h->t0 = h->v0 * h->v0;
+    if (!strcasecmp(config, LESConfigSuctionVortices) || !strcasecmp(config, LESConfigSuctionVorticesLarge)) {
+        h->ax = h->ay = h->az = 2.0f;
+        h->rx = h->ry = 1.0212f;
+        h->rz = 1.05f;
+        h->tp = 2.0f;
+        h->stretched = 1;
+    } else if (!strcasecmp(config, LESConfigTwoCell)) {
+        h->v0 = 225.0f;
+        h->p0 = h->t0 = h->v0 * h->v0;
+        h->ax = h->ay = h->az = 1.0f;
+        h->rx = h->ry = h->rz = 1.0f;
+        h->tp = 5.0f;
+    } else if (!strcasecmp(config, LESConfigFlat)) {
+        /* spacings from the grid record: x[1] - x[0], y[nx] - y[0], z[nx ny] - z[0] (km) */
+        float *xyz = (float *)malloc(3 * nn * sizeof(float));
+        int ok = xyz != NULL;
+        for (int c = 0; ok && c < 3; c++)
+            ok = fseek(g, 2 * sizeof(float), SEEK_CUR) == 0 && fread(xyz + c * nn, sizeof(float), nn, g) == nn;
+        if (!ok) {
+            free(xyz);
+            fclose(g);
+            free(h);
+            return NULL;
+        }
+        h->p0 = 1.0f;
+        h->ax = h->ay = h->az = 0.0f;
+        h->rx = 1.0e3f * (xyz[1] - xyz[0]);
+        h->ry = 1.0e3f * (xyz[nn + h->nx] - xyz[nn]);
+        h->rz = 1.0e3f * (xyz[2 * nn + (size_t)h->nx * h->ny] - xyz[2 * nn]);
+        h->tp = 60.0f;
+        h->p_for_cn2 = 1;
+        free(xyz);
+    } else {
+        h->ax = h->ay = h->az = 1.0f;
+        h->rx = h->ry = h->rz = 1.0f;
+        h->tp = 5.0f;
+    }
+    fclose(g);
+    /* volumes per file from the first file's size; files numbered from 1 */
+    const size_t vol = 4 + 8 + 5 * (nn * sizeof(float) + 8);
+    size_t nvol = 0, nfiles = 0;
+    for (;;) {
+        snprintf(f, sizeof(f), "%s/LES_mean_1_6_fnum%zu.dat", h->dir, nfiles + 1);
+        struct stat st;
+        if (stat(f, &st) != 0 || !S_ISREG(st.st_mode)) break;
+        if (nfiles == 0) nvol = st.st_size > 4 ? ((size_t)st.st_size - 4) / vol : 0;
+        nfiles++;
+    }
+    if (nfiles == 0 || nvol == 0) {
+        rsprint("ERROR: LES data files missing or short in %s.", h->dir);
+        free(h);
+        return NULL;
+    }
+    if (nvol != LES_FILE_NBLOCK) rsprint("WARNING: each LES data file holds %zu volumes, expected %d.", nvol, LES_FILE_NBLOCK);
+    h->seed = nvol;                    /* volumes per file */
+    h->frames = (uint32_t)(nfiles * nvol);
+    h->tables = (LESTable **)calloc(h->frames, sizeof(LESTable *));
+    if (!h->tables) {
+        free(h);
+        return NULL;
+    }
+    return (LESHandle)h;
+}
+
 LESHandle LES_init_with_config_path(const LESConfig config, const char *path) {
-    (void)path;
+    if (config && config[0]) {
+        char dirs[8][1024], dir[1024], probe[1100];
+        const int n = table_search_list(path, "les", 1, dirs);
+        snprintf(probe, sizeof(probe), "les/%s/fort.10_2", config);
+        if (find_table_dir(dirs, n, probe, dir)) {
+            LESHandle h = les_init_files(config, dir);
+            if (h) return h;
+        }
+    }
     if (config && (!strcmp(config, LESConfigSuctionVortices) || !strcmp(config, LESConfigSuctionVorticesLarge))) {
         return LES_init_synthetic(176, 176, 80, 2.0f, 1.0212f, 2.0f, 1.05f, 50.0f, 40.0f, 2.0f, 20250101ull, 10);
     }
@@ -261,9 +415,62 @@ static LESTable *les_make_frame(LESMem *h, uint32_t f) {
     return t;
 }
 
+/* Volume f of a file-backed LES (les.c:318-384). */
+static LESTable *les_read_frame(LESMem *h, uint32_t f) {
+    const size_t nn = (size_t)h->nx * h->ny * h->nz, nvol = (size_t)h->seed;
+    char name[2300];
+    snprintf(name, sizeof(name), "%s/LES_mean_1_6_fnum%zu.dat", h->dir, f / nvol + 1);
+    FILE *fid = fopen(name, "rb");
+    if (!fid) return NULL;
+    const long off = (long)(4 + (f % nvol) * (4 + 8 + 5 * (nn * sizeof(float) + 8)));
+    float *buf = (float *)malloc(5 * nn * sizeof(float));
+    LESTable *t = (LESTable *)calloc(1, sizeof(LESTable));
+    int ok = buf && t && fseek(fid, off + 4 + 8, SEEK_SET) == 0;
+    for (int c = 0; ok && c < 5; c++) {
+        ok = fread(buf + c * nn, sizeof(float), nn, fid) == nn && fseek(fid, 8, SEEK_CUR) == 0;
+    }
+    fclose(fid);
+    if (ok) {
+        t->uvwt = (LESFloat4 *)malloc(nn * sizeof(LESFloat4));
+        t->cpxx = (LESFloat4 *)malloc(nn * sizeof(LESFloat4));
+        ok = t->uvwt && t->cpxx;
+    }
+    if (!ok) {
+        free(buf);
+        if (t) {
+            free(t->uvwt);
+            free(t->cpxx);
+            free(t);
+        }
+        return NULL;
+    }
+    t->nx = h->nx; t->ny = h->ny; t->nz = h->nz;
+    t->nn = (uint32_t)nn;
+    t->nt = (uint32_t)nvol;
+    t->nc = h->frames;
+    t->is_stretched = h->stretched != 0;
+    t->tr = h->tr;
+    t->tp = h->tp;
+    t->ax = h->ax; t->ay = h->ay; t->az = h->az;
+    t->rx = h->rx; t->ry = h->ry; t->rz = h->rz;
+    for (size_t k = 0; k < nn; k++) {
+        const float u = buf[k] * h->v0, v = buf[nn + k] * h->v0, w = buf[2 * nn + k] * h->v0;
+        const float pp = buf[3 * nn + k] * h->p0, tt = buf[4 * nn + k] * h->t0;
+        t->uvwt[k][0] = u; t->uvwt[k][1] = v; t->uvwt[k][2] = w; t->uvwt[k][3] = tt;
+        t->cpxx[k][0] = h->p_for_cn2 ? pp : 0.0f;
+        t->cpxx[k][1] = h->p_for_cn2 ? 0.0f : pp;
+        t->cpxx[k][2] = 0.0f;
+        t->cpxx[k][3] = 0.0f;
+    }
+    free(buf);
+    return t;
+}
+
+static LESTable *les_frame(LESMem *h, uint32_t f) { return h->file_mode ? les_read_frame(h, f) : les_make_frame(h, f); }
+
 static void *les_ahead_run(void *arg) {
     LESMem *h = (LESMem *)arg;
-    h->tables[h->ahead] = les_make_frame(h, h->ahead);
+    h->tables[h->ahead] = les_frame(h, h->ahead);
     return NULL;
 }
 
@@ -271,8 +478,18 @@ LESTable *LES_get_frame(const LESHandle i, const int n) {
     LESMem *h = (LESMem *)i;
     if (!h || n < 0 || (uint32_t)n >= h->frames) return NULL;
     les_join(h);
-    if (!h->tables[n]) h->tables[n] = les_make_frame(h, (uint32_t)n);
+    if (!h->tables[n]) h->tables[n] = les_frame(h, (uint32_t)n);
     const uint32_t next = ((uint32_t)n + 1) % h->frames;
+    if (h->file_mode) {
+        /* file-backed: keep the frame handed out and the next one only (the
+         * caller has copied the previous frame when it asks for a new one) */
+        for (uint32_t f = 0; f < h->frames; f++) {
+            if (f != (uint32_t)n && f != next && h->tables[f]) {
+                les_free_table(h->tables[f]);
+                h->tables[f] = NULL;
+            }
+        }
+    }
     if (!h->tables[next]) {
         h->ahead = next;
         h->worker_on = pthread_create(&h->worker, NULL, les_ahead_run, h) == 0;
@@ -404,6 +621,154 @@ void RCS_table_free(RCSTable *T) {
     free(T);
 }
 
+/* ------------------------------------------------------ ADM / RCS files */
+
+/* <path> of adm.c:152-225 / rcs.c:181-246: two uint16 grid sizes, then six
+ * float planes of nn values.  ADM: nb (beta, x) then na, planes cdx cdy cdz
+ * cmx cmy cmz, axes beta in [-180, 180] and alpha in [0, 180] degrees, body
+ * 0.002 x 0.04 x 0.04 m at 1120 kg m^-3 until ADM_dimension_set.  RCS: na
+ * (alpha, x) then nb, planes hh vv hv real then imaginary, lambda 0.1 m. */
+static int read_planes(const char *path, uint16_t dims[2], float **planes, int np) {
+    FILE *fid = fopen(path, "rb");
+    if (!fid) return 0;
+    int ok = fread(dims, sizeof(uint16_t), 2, fid) == 2 && dims[0] > 1 && dims[1] > 1;
+    const size_t nn = ok ? (size_t)dims[0] * dims[1] : 0;
+    for (int p = 0; ok && p < np; p++) {
+        planes[p] = (float *)malloc(nn * sizeof(float));
+        ok = planes[p] && fread(planes[p], sizeof(float), nn, fid) == nn;
+    }
+    fclose(fid);
+    return ok;
+}
+
+ADMTable *ADM_table_read(const char *path) {
+    ADMTable *T = (ADMTable *)calloc(1, sizeof(ADMTable));
+    if (!T) return NULL;
+    uint16_t d[2];
+    float *pl[6] = {0};
+    if (!read_planes(path, d, pl, 6)) {
+        for (int p = 0; p < 6; p++) free(pl[p]);
+        free(T);
+        rsprint("ERROR: unable to read ADM table %s.", path);
+        return NULL;
+    }
+    T->nb = d[0];
+    T->na = d[1];
+    T->nn = T->nb * T->na;
+    T->data.cdx = pl[0]; T->data.cdy = pl[1]; T->data.cdz = pl[2];
+    T->data.cmx = pl[3]; T->data.cmy = pl[4]; T->data.cmz = pl[5];
+    T->data.b = (float *)malloc(T->nb * sizeof(float));
+    T->data.a = (float *)malloc(T->na * sizeof(float));
+    if (!T->data.b || !T->data.a) {
+        ADM_table_free(T);
+        return NULL;
+    }
+    for (uint32_t i = 0; i < T->nb; i++) T->data.b[i] = (float)i / (float)(T->nb - 1) * 360.0f - 180.0f;
+    for (uint32_t i = 0; i < T->na; i++) T->data.a[i] = (float)i / (float)(T->na - 1) * 180.0f;
+    const char *base = strrchr(path, '/');
+    snprintf(T->name, sizeof(T->name), "%s", base ? base + 1 : path);
+    snprintf(T->path, sizeof(T->path), "%s", path);
+    ADM_dimension_set(T, 0.002f, 0.040f, 0.040f, 1120.0f);
+    return T;
+}
+
+RCSTable *RCS_table_read(const char *path) {
+    RCSTable *T = (RCSTable *)calloc(1, sizeof(RCSTable));
+    if (!T) return NULL;
+    uint16_t d[2];
+    float *pl[6] = {0};
+    if (!read_planes(path, d, pl, 6)) {
+        for (int p = 0; p < 6; p++) free(pl[p]);
+        free(T);
+        rsprint("ERROR: unable to read RCS table %s.", path);
+        return NULL;
+    }
+    T->na = d[0];
+    T->nb = d[1];
+    T->nn = T->na * T->nb;
+    T->lambda = 0.1f;
+    T->data.hh_real = pl[0]; T->data.vv_real = pl[1]; T->data.hv_real = pl[2];
+    T->data.hh_imag = pl[3]; T->data.vv_imag = pl[4]; T->data.hv_imag = pl[5];
+    T->data.a = (float *)malloc(T->na * sizeof(float));
+    T->data.b = (float *)malloc(T->nb * sizeof(float));
+    if (!T->data.a || !T->data.b) {
+        RCS_table_free(T);
+        return NULL;
+    }
+    for (uint32_t i = 0; i < T->na; i++) T->data.a[i] = (float)i / (float)(T->na - 1) * 360.0f - 180.0f;
+    for (uint32_t i = 0; i < T->nb; i++) T->data.b[i] = (float)i / (float)(T->nb - 1) * 180.0f;
+    const char *base = strrchr(path, '/');
+    snprintf(T->name, sizeof(T->name), "%s", base ? base + 1 : path);
+    snprintf(T->path, sizeof(T->path), "%s", path);
+    return T;
+}
+
+/* Handles of adm.h / rcs.h: a table folder found on the search list, tables
+ * read from it on request and freed with the handle. */
+#define TABLES_MAX 64
+typedef struct {
+    char dir[1024];
+    int count;
+    void *tables[TABLES_MAX];
+} TableMem;
+
+static TableMem *table_mem(const char *path, const char *sub, const char *probe) {
+    char dirs[8][1024];
+    const int n = table_search_list(path, sub, 0, dirs);
+    TableMem *h = (TableMem *)calloc(1, sizeof(TableMem));
+    if (!h) return NULL;
+    if (!find_table_dir(dirs, n, probe, h->dir)) {
+        free(h);
+        rsprint("ERROR: Unable to find the %s data folder.", sub);
+        return NULL;
+    }
+    return h;
+}
+
+ADMHandle ADM_init_with_path(const char *path) {
+    return (ADMHandle)table_mem(path, "les", "adm/" ADMConfigSquarePlate ".adm");
+}
+ADMHandle ADM_init(void) { return ADM_init_with_path(NULL); }
+
+ADMTable *ADM_get_table(const ADMHandle in, const ADMConfig config) {
+    TableMem *h = (TableMem *)in;
+    if (!h || !config || h->count >= TABLES_MAX) return NULL;
+    char f[1200];
+    snprintf(f, sizeof(f), "%s/adm/%s.adm", h->dir, config);
+    ADMTable *T = ADM_table_read(f);
+    if (T) h->tables[h->count++] = T;
+    return T;
+}
+
+void ADM_free(ADMHandle in) {
+    TableMem *h = (TableMem *)in;
+    if (!h) return;
+    for (int i = 0; i < h->count; i++) ADM_table_free((ADMTable *)h->tables[i]);
+    free(h);
+}
+
+RCSHandle RCS_init_with_path(const char *path) {
+    return (RCSHandle)table_mem(path, "rcs", "rcs/" RCSConfigLeaf ".rcs");
+}
+RCSHandle RCS_init(void) { return RCS_init_with_path(NULL); }
+
+RCSTable *RCS_get_table(const RCSHandle in, const RCSConfig config) {
+    TableMem *h = (TableMem *)in;
+    if (!h || !config || h->count >= TABLES_MAX) return NULL;
+    char f[1200];
+    snprintf(f, sizeof(f), "%s/rcs/%s.rcs", h->dir, config);
+    RCSTable *T = RCS_table_read(f);
+    if (T) h->tables[h->count++] = T;
+    return T;
+}
+
+void RCS_free(RCSHandle in) {
+    TableMem *h = (TableMem *)in;
+    if (!h) return;
+    for (int i = 0; i < h->count; i++) RCS_table_free((RCSTable *)h->tables[i]);
+    free(h);
+}
+
 /* ----------------------------------------------------------------- OBJ */
 
 #define OBJ_MAX 64
@@ -411,9 +776,25 @@ void RCS_table_free(RCSTable *T) {
 typedef struct {
     int count;
     OBJTable obj_table[OBJ_MAX];
+    char adm_dir[1024];     /* folder holding adm/<config>.adm, "" if none found */
+    char rcs_dir[1024];     /* folder holding rcs/<config>.rcs */
 } OBJMem;
 
-OBJHandle OBJ_init(void) { return (OBJHandle)calloc(1, sizeof(OBJMem)); }
+/* The ADM and RCS folders are looked up as adm.c:56-111 and rcs.c:92-145 do
+ * (probing adm/square_plate.adm and rcs/leaf.rcs); a type whose files are
+ * found is read from them, any other uses the synthetic tables. */
+OBJHandle OBJ_init_with_path(const char *path) {
+    OBJMem *O = (OBJMem *)calloc(1, sizeof(OBJMem));
+    if (!O) return NULL;
+    char dirs[8][1024];
+    int n = table_search_list(path, "les", 0, dirs);
+    if (!find_table_dir(dirs, n, "adm/" ADMConfigSquarePlate ".adm", O->adm_dir)) O->adm_dir[0] = '\0';
+    n = table_search_list(path, "rcs", 0, dirs);
+    if (!find_table_dir(dirs, n, "rcs/" RCSConfigLeaf ".rcs", O->rcs_dir)) O->rcs_dir[0] = '\0';
+    return (OBJHandle)O;
+}
+
+OBJHandle OBJ_init(void) { return OBJ_init_with_path(NULL); }
 
 void OBJ_free(OBJHandle in) {
     OBJMem *O = (OBJMem *)in;
@@ -441,9 +822,26 @@ OBJTable *OBJ_get_table(const OBJHandle in, OBJConfig type) {
         case OBJConfigBrick: rcs = RCSConfigBrick; dx = 0.065f; dy = 0.215f; dz = 0.1125f; rho = 2200.0f; scale = 0.05f; break;
         default: return NULL;
     }
-    t->adm_table = ADM_table_synthetic(adm, 361, 181, 0.01f, 1000ull + (uint64_t)type);
-    t->rcs_table = RCS_table_synthetic(rcs, 361, 181, scale, 0.02f, 2000ull + (uint64_t)type);
-    if (!t->adm_table || !t->rcs_table) return NULL;
+    char f[1200];
+    if (O->adm_dir[0]) {
+        snprintf(f, sizeof(f), "%s/adm/%s.adm", O->adm_dir, adm);
+        t->adm_table = ADM_table_read(f);
+    } else {
+        t->adm_table = ADM_table_synthetic(adm, 361, 181, 0.01f, 1000ull + (uint64_t)type);
+    }
+    if (O->rcs_dir[0]) {
+        snprintf(f, sizeof(f), "%s/rcs/%s.rcs", O->rcs_dir, rcs);
+        t->rcs_table = RCS_table_read(f);
+    } else {
+        t->rcs_table = RCS_table_synthetic(rcs, 361, 181, scale, 0.02f, 2000ull + (uint64_t)type);
+    }
+    if (!t->adm_table || !t->rcs_table) {
+        ADM_table_free(t->adm_table);
+        RCS_table_free(t->rcs_table);
+        t->adm_table = NULL;
+        t->rcs_table = NULL;
+        return NULL;
+    }
     ADM_dimension_set(t->adm_table, dx, dy, dz, rho);
     O->count++;
     return t;

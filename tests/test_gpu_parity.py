@@ -239,3 +239,30 @@ def test_wind_index_log1p_matches_general_form(gpu_ready):
     ref = orc.math("log1p", x)
     bad = int(np.sum(_bits(dev) != _bits(ref)))
     assert bad == 0, "differs at %d of %d inputs" % (bad, x.size)
+
+
+def test_file_tables_parity(gpu_ready, tmp_path, monkeypatch):
+    """Rain + leaves on tables read from reference-format files (LES records,
+    .adm, .rcs) instead of the synthetic ones: state bit-exact, IQ within
+    tolerance against the oracle on the same tables."""
+    from test_tables import _write_adm, _write_les, _write_rcs
+    g = np.random.default_rng(8)
+    home = tmp_path / "home"
+    (home / "tables" / "adm").mkdir(parents=True)
+    (home / "tables" / "rcs").mkdir(parents=True)
+    for n in ("plate", "square_plate"):
+        _write_adm(str(home / "tables" / "adm" / (n + ".adm")), 73, 37, g)
+    for n in ("leaf", "plate", "brick", "woodboard"):
+        _write_rcs(str(home / "tables" / "rcs" / (n + ".rcs")), 73, 37, g)
+    # the suctvort grid (the scan box follows the LES extent), one volume
+    _write_les(str(home / "Documents" / "tables" / "les" / "suctvort"), 176, 176, 80, files=1, nvol=1, g=g)
+    monkeypatch.setenv("HOME", str(home))
+    monkeypatch.chdir(tmp_path)
+    cfg = h.Config(n_rain=8192, debris=[(sa.OBJ_LEAF, 512)], scan="P:3.0,-1:1:0.01")
+    r, o, pg, po = _run(cfg, 6)
+    assert r.les_frame(0).shape == (80, 176, 176, 4)
+    assert r.debris_table(0, 0).shape == (37, 73, 4)
+    _check_state(r, o)
+    errs = [h.rel_err(pg[k], po[k]) for k in range(len(pg))]
+    assert max(errs) < IQ_TOL, errs
+    r.free()
