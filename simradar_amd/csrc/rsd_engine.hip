@@ -80,25 +80,38 @@ __device__ __forceinline__ float4 lerp4(float4 p, float4 q, float t) {
     return make_float4(lerpf_(p.x, q.x, t), lerpf_(p.y, q.y, t), lerpf_(p.z, q.z, t), lerpf_(p.w, q.w, t));
 }
 
-__device__ __forceinline__ float4 sample3d(const Tex3 &t, float u, float v, float w) {
+// Trilinear sample in two halves: gather3d issues the eight texel reads and
+// lerp3d combines them, so that independent work can run while the reads are
+// in flight.  Texel order t[4 z + 2 y + x].
+struct Gather3 { float4 t[8]; float a, b, c; };
+
+__device__ __forceinline__ void gather3d(const Tex3 &t, float u, float v, float w, Gather3 &g) {
     int i0, i1, j0, j1, k0, k1;
-    float a, b, c;
-    tex_axis(u, t.nx, i0, i1, a);
-    tex_axis(v, t.ny, j0, j1, b);
-    tex_axis(w, t.nz, k0, k1, c);
+    tex_axis(u, t.nx, i0, i1, g.a);
+    tex_axis(v, t.ny, j0, j1, g.b);
+    tex_axis(w, t.nz, k0, k1, g.c);
     const uint32_t nx = (uint32_t)t.nx, nxy = (uint32_t)t.nx * (uint32_t)t.ny;
     const float4 *D = t.d;
     const uint32_t r00 = k0 * nxy + j0 * nx, r10 = k0 * nxy + j1 * nx;
     const uint32_t r01 = k1 * nxy + j0 * nx, r11 = k1 * nxy + j1 * nx;
-    float4 t000 = D[r00 + i0], t100 = D[r00 + i1];
-    float4 t010 = D[r10 + i0], t110 = D[r10 + i1];
-    float4 t001 = D[r01 + i0], t101 = D[r01 + i1];
-    float4 t011 = D[r11 + i0], t111 = D[r11 + i1];
-    float4 x00 = lerp4(t000, t100, a);
-    float4 x10 = lerp4(t010, t110, a);
-    float4 x01 = lerp4(t001, t101, a);
-    float4 x11 = lerp4(t011, t111, a);
-    return lerp4(lerp4(x00, x10, b), lerp4(x01, x11, b), c);
+    g.t[0] = D[r00 + i0]; g.t[1] = D[r00 + i1];
+    g.t[2] = D[r10 + i0]; g.t[3] = D[r10 + i1];
+    g.t[4] = D[r01 + i0]; g.t[5] = D[r01 + i1];
+    g.t[6] = D[r11 + i0]; g.t[7] = D[r11 + i1];
+}
+
+__device__ __forceinline__ float4 lerp3d(const Gather3 &g) {
+    float4 x00 = lerp4(g.t[0], g.t[1], g.a);
+    float4 x10 = lerp4(g.t[2], g.t[3], g.a);
+    float4 x01 = lerp4(g.t[4], g.t[5], g.a);
+    float4 x11 = lerp4(g.t[6], g.t[7], g.a);
+    return lerp4(lerp4(x00, x10, g.b), lerp4(x01, x11, g.b), g.c);
+}
+
+__device__ __forceinline__ float4 sample3d(const Tex3 &t, float u, float v, float w) {
+    Gather3 g;
+    gather3d(t, u, v, w, g);
+    return lerp3d(g);
 }
 
 __device__ __forceinline__ float4 sample2d(const Tex2 &t, float u, float v) {
@@ -114,7 +127,7 @@ __device__ __forceinline__ float4 sample2d(const Tex2 &t, float u, float v) {
 }
 
 // rs.cl:123-176: LES table coordinates of a position.
-__device__ __forceinline__ float4 wind_index(float4 pos, const F16 &d, const F16 &sd, const float *logtab) {
+__device__ __forceinline__ float4 wind_index(float4 pos, const F16 &d, const F16 &sd, const float4 *logtab) {
     float4 c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     uint32_t spacing = __float_as_uint(d.s[7]);
     if (spacing == 7u) {
@@ -153,13 +166,21 @@ __device__ __forceinline__ bool is_outside(float4 p, const F16 &sd) {
     return (p.x <= sd.s[8]) | (p.y <= sd.s[9]) | (p.z <= sd.s[10]) | (p.x >= ex) | (p.y >= ey) | (p.z >= ez);
 }
 
-// rs.cl:268-280 with cos^2(atan2(z, rho)) = rho^2 / (rho^2 + z^2).
-__device__ __forceinline__ float4 ellipsoid_rcs(float4 p, const float4 *table, const F4 &td) {
-    float fidx = rf_clamp(rf_fma(p.w, td.s[0], td.s[1]), 0.0f, td.s[2]);
-    float4 xz = table[(uint32_t)fidx];
+// rs.cl:268-280 with cos^2(atan2(z, rho)) = rho^2 / (rho^2 + z^2): the table
+// entry (alpha_x, alpha_z) of a drop radius (truncating index), then the
+// elevation mix of the position.
+__device__ __forceinline__ uint32_t drop_index(float radius, const F4 &td) {
+    return (uint32_t)rf_clamp(rf_fma(radius, td.s[0], td.s[1]), 0.0f, td.s[2]);
+}
+
+__device__ __forceinline__ float4 drop_mix(float4 p, float4 xz) {
     float rho2 = p.x * p.x + p.y * p.y;
     float cb2 = rho2 * rf_rcp_fast(rho2 + p.z * p.z);
     return make_float4(xz.x, xz.y, xz.x + (xz.z - xz.x) * cb2, xz.y + (xz.w - xz.y) * cb2);
+}
+
+__device__ __forceinline__ float4 ellipsoid_rcs(float4 p, const float4 *table, const F4 &td) {
+    return drop_mix(p, table[drop_index(p.w, td)]);
 }
 
 // rs.cl:54-85
@@ -177,7 +198,7 @@ __device__ __forceinline__ float4 norm4(float4 q) {
 }
 
 // rs.cl:282-359
-__device__ float4 debris_rcs(float4 pos, float4 ori, const Tex2 &rr, const Tex2 &ri, const F16 &d, uint32_t concept) {
+__device__ __forceinline__ float4 debris_rcs(float4 pos, float4 ori, const Tex2 &rr, const Tex2 &ri, const F16 &d, uint32_t concept) {
     float el = rf_atan2f(pos.z, rf_len2(pos.x, pos.y));
     float az = rf_atan2f(pos.x, pos.y);
     float se, ce, sa, ca;
@@ -303,6 +324,8 @@ struct PulseK {
     float gate_start, gate_inv_delta;
     float sup_lo, sup_hi;   // range-offset interval outside which the weight is 0
     int dense;              // 1: every gate may be weighted
+    int tri;                // 1: the range weight table is the triangle (0, 1, 0)
+    float tri_s0, tri_s1;   // its desc slots 0 and 1 - 1 (see tri_weight)
     int fold;               // RSD_FOLD_* strategy of the gate sums
     uint32_t phase_mode;
     double *partials;       // [4 R][rows]: column j = component j / R, gate j % R
@@ -333,8 +356,8 @@ __device__ __forceinline__ float beam_angle(float4 p, const F16 &sd) {
 // With HILO the position is the pair hi + lo (binary32 each) and the range that
 // enters the phase is formed in binary64 from it.
 template <bool HILO = false>
-__device__ __forceinline__ bool scat_signal(float4 p, float4 rcs, const PulseK &k, float4 &sa, float &r,
-                                            float4 lo = make_float4(0.0f, 0.0f, 0.0f, 0.0f)) {
+__device__ __forceinline__ bool scat_signal(float4 p, float4 rcs, const PulseK &k, const float *angw, float4 &sa,
+                                            float &r, float4 lo = make_float4(0.0f, 0.0f, 0.0f, 0.0f)) {
     double r64 = 0.0;
     if (HILO) {
         const double x = (double)p.x + (double)lo.x, y = (double)p.y + (double)lo.y, z = (double)p.z + (double)lo.z;
@@ -344,7 +367,7 @@ __device__ __forceinline__ bool scat_signal(float4 p, float4 rcs, const PulseK &
         r = rf_len3(p.x, p.y, p.z);
     }
     const float inv_r = rf_rcp_fast(r);
-    float wa = table_lerp(beam_angle(p, k.sd), k.angw, k.angw_desc);
+    float wa = table_lerp(beam_angle(p, k.sd), angw, k.angw_desc);
     if (wa == 0.0f) return false;
     float atten = inv_r * inv_r;
     float ss, cc;
@@ -495,6 +518,8 @@ __device__ __forceinline__ void fold_gates(bool on, float r, float4 sa, const Pu
 // them to LDS once per launch.  A scatterer whose gates leave the window first
 // flushes the lane's window with per-lane LDS atomics.
 #define RSD_WIN 8
+#define RSD_LDS_DROP_MAX 256      // drop RCS entries staged in LDS by rain pulse launches
+#define RSD_LDS_ANGW_MAX 1024     // angular weight entries staged in LDS
 #define RSD_WIN_EMPTY (-(1 << 24))
 
 struct GateWin {
@@ -551,6 +576,13 @@ __device__ __forceinline__ float lds_weight(float x, const float2 *W, const F4 &
     float dec = __builtin_fminf(x0 - ip, 0x1.fffffep-1f);
     const float2 p = W[(int)ip];
     return p.x + p.y * dec;
+}
+
+// The triangle table (0, 1, 0) with max index 2 (RS_set_range_weight_to_triangle,
+// rs.c:2620-2623) evaluated in closed form: x0 = fma(x, s0, s1) in [0, 2] gives
+// 1 - |x0 - 1| (the table's mix up to rounding), 0 outside.
+__device__ __forceinline__ float tri_weight(float x, const PulseK &k) {
+    return __builtin_fmaxf(0.0f, 1.0f - __builtin_fabsf(rf_fma(x, k.tri_s0, k.tri_s1)));
 }
 
 // LDS layout of the window folds: gate sums acc[4 R] (binary64), then the range
@@ -724,6 +756,7 @@ __device__ __forceinline__ void wwin_flush(const WaveWin &w, const PulseK &k, do
 // that end in zeros).
 __device__ __forceinline__ int median3(int a, int b, int c) { return max(min(a, b), min(max(a, b), c)); }
 
+template <bool TRI>
 __device__ __forceinline__ void wwin_fold(WaveWin &w, bool on, float r, float4 sa, const PulseK &k, double *acc,
                                           const float2 *wl, const float *grl) {
     int klo = 0x3fffffff, khi = -1;
@@ -745,7 +778,8 @@ __device__ __forceinline__ void wwin_fold(WaveWin &w, bool on, float r, float4 s
         const int g = w.kb + s;
         if (g < 0 || g >= k.R) continue;
         if (__ballot(fit && klo <= g && g <= khi) == 0) continue;
-        const float wr = lds_weight(r - grl[g], wl, k.rangew_desc);
+        const float x = r - grl[g];
+        const float wr = TRI ? tri_weight(x, k) : lds_weight(x, wl, k.rangew_desc);
         w.a[4 * s + 0] = __builtin_fmaf(wr, sa.x, w.a[4 * s + 0]);
         w.a[4 * s + 1] = __builtin_fmaf(wr, sa.y, w.a[4 * s + 1]);
         w.a[4 * s + 2] = __builtin_fmaf(wr, sa.z, w.a[4 * s + 2]);
@@ -753,22 +787,32 @@ __device__ __forceinline__ void wwin_fold(WaveWin &w, bool on, float r, float4 s
     }
 }
 
-// Contiguous range of scatterers of this block: logical block L owns
-// [L chunk, (L + 1) chunk).  Blocks are dispatched round-robin over the 8 XCDs,
-// so L = (b mod 8) (G / 8) + b / 8 gives each XCD one contiguous eighth of the
-// (spatially sorted) streams and keeps its L2 on one region of the wind table.
-// Returns the logical block L.
+// Blocks are dispatched round-robin over the 8 XCDs, so L = (b mod 8) (G / 8) +
+// b / 8 numbers the blocks of each XCD consecutively.
+__device__ __forceinline__ unsigned xcd_order(unsigned b, unsigned G) {
+    return (G & 7u) == 0u ? (b & 7u) * (G >> 3) + (b >> 3) : b;
+}
+
+// Contiguous range of scatterers of block b: chunk L owns [L chunk, (L + 1)
+// chunk).  XCD-contiguous (L = xcd_order): each XCD takes one contiguous part
+// of the streams and keeps its L2 on one region of the tables (debris, batch
+// items).  Interleaved (L = b): consecutive chunks go to different XCDs, so
+// every XCD gets a share of every part of the range-sorted rain streams and
+// the XCDs finish together (the cost per scatterer varies along the streams).
+// Returns xcd_order(b), the block's partial-sum row: the rows of one XCD stay
+// adjacent either way.
 __device__ __forceinline__ unsigned block_range_at(size_t begin, size_t end, unsigned b, unsigned G, size_t &lo,
-                                                   size_t &hi) {
-    const unsigned L = (G & 7u) == 0u ? (b & 7u) * (G >> 3) + (b >> 3) : b;
+                                                   size_t &hi, bool interleave = false) {
+    const unsigned row = xcd_order(b, G);
+    const unsigned L = interleave ? b : row;
     const size_t n = end - begin;
     size_t chunk = (n + G - 1) / G;
-    chunk = (chunk + blockDim.x - 1) / blockDim.x * blockDim.x;
+    chunk = (chunk + 63) / 64 * 64;     // whole waves; every block but the last equal
     lo = begin + (size_t)L * chunk;
     hi = lo + chunk;
     if (lo > end) lo = end;
     if (hi > end) hi = end;
-    return L;
+    return row;
 }
 
 __device__ __forceinline__ void block_range(size_t begin, size_t end, size_t &lo, size_t &hi) {
@@ -780,8 +824,8 @@ __device__ __forceinline__ void acc_clear(double *acc, int n) {
     __syncthreads();
 }
 
-// Row `row` of the transposed partial buffer: one value per column.  Logical
-// blocks on one XCD own consecutive rows, so its L2 merges them into whole lines.
+// Row `row` of the transposed partial buffer: one value per column.  The blocks
+// of one XCD own consecutive rows, so its L2 merges them into whole lines.
 __device__ __forceinline__ void acc_flush_row(const double *acc, const PulseK &k, int row) {
     __syncthreads();
     for (int j = threadIdx.x; j < 4 * k.R; j += blockDim.x) k.partials[(size_t)j * k.rows + row] = acc[j];
@@ -808,7 +852,32 @@ struct RainK {
     uint32_t dragged;
     const float *logtab;    // global copy of the log1p table (3 x 128 floats)
     PulseK pk;
+    // Pulse launches stage the drop RCS table (drop_n entries) and the angular
+    // weight table (angw_n entries) in LDS at byte offset tab_off; tab_lds = 0:
+    // both are read from global memory.  In LDS their reads do not wait behind
+    // the wind texel reads in flight (global reads complete in issue order).
+    int tab_lds, drop_n, angw_n;
+    unsigned tab_off;
 };
+
+// Byte offset of the staged rain tables in the dynamic LDS of a pulse launch:
+// after the 4 R binary64 gate sums and, with the window fold, the weight pairs
+// and gate ranges; 16-byte aligned.
+__host__ __device__ __forceinline__ size_t rain_tab_offset(int R, int win, int rangew_n) {
+    size_t o = (size_t)4 * R * sizeof(double);
+    if (win) o += (size_t)rangew_n * sizeof(float2) + (size_t)R * sizeof(float);
+    return (o + 15) & ~(size_t)15;
+}
+
+__device__ __forceinline__ void stage_rain_tables(double *acc, const RainK &a) {
+    if (a.tab_lds) {
+        float4 *dt = (float4 *)((char *)acc + a.tab_off);
+        float *at = (float *)(dt + a.drop_n);
+        for (int j = threadIdx.x; j < a.drop_n; j += blockDim.x) dt[j] = a.drop[j];
+        for (int j = threadIdx.x; j < a.angw_n; j += blockDim.x) at[j] = a.pk.angw[j];
+    }
+    __syncthreads();
+}
 
 // (hi, lo) += v dt carried as a pair of binary32 values: the product's rounding
 // error from fma (TwoProduct), the sum's from TwoSum, then renormalised so that
@@ -824,15 +893,17 @@ __device__ __forceinline__ void hilo_step(float &hi, float &lo, float v, float d
     lo = l - (hi - s);
 }
 
-// Returns true when the scatterer respawned.  HILO: positions are hi + lo pairs
-// (lo in its own stream); the physics reads hi.
+// One rain step in three parts (el_atts, rs.cl:653-739, when dragged; bg_atts,
+// rs.cl:533-580, otherwise): rain_move updates the position and respawns a
+// scatterer that left the box (returns true; vel = 0 then and the step ends);
+// rain_gather reads the wind texels at the new position; rain_drag turns them
+// into the new velocity.  HILO: positions are hi + lo pairs (lo in its own
+// stream); the physics reads hi.
 template <bool HILO = false>
-__device__ __forceinline__ bool rain_advance(const RainK &a, size_t i, float4 &pos, float4 &vel, const float *logtab,
-                                             float4 &lo) {
+__device__ __forceinline__ bool rain_move(const RainK &a, size_t i, float4 &pos, float4 &vel, float4 &lo) {
     const F16 &sd = a.sd;
     const float dt = sd.s[11];
     if (a.dragged) {
-        const uint32_t concept = __float_as_uint(sd.s[5]);
         if (HILO) {
             hilo_step(pos.x, lo.x, vel.x, dt);
             hilo_step(pos.y, lo.y, vel.y, dt);
@@ -843,84 +914,90 @@ __device__ __forceinline__ bool rain_advance(const RainK &a, size_t i, float4 &p
             pos.z = rf_fma(vel.z, dt, pos.z);
         }
         pos.w = rf_fma(vel.w, 0.0f, pos.w);
-        const float aom = rf_div(0.003006012f, pos.w);
-        if (is_outside(pos, sd)) {
-            uint4 seed = a.rnd[i];
-            float4 r = rand4(seed);
-            a.rnd[i] = seed;
-            pos.x = rf_fma(r.x, sd.s[12], sd.s[8]);
-            pos.y = rf_fma(r.y, sd.s[13], sd.s[9]);
-            pos.z = rf_fma(r.z, sd.s[14], sd.s[10]);
-            vel = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            lo = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            return true;
-        }
-        float4 c = wind_index(pos, a.les, sd, logtab);
-        float4 bg = sample3d(a.wind, c.x, c.y, c.z);
-        float4 dv = make_float4(bg.x - vel.x, bg.y - vel.y, bg.z - vel.z, bg.w - vel.w);
-        float dva = rf_len3(dv.x, dv.y, dv.z);
-        if (dva > 1.0e-3f) {
-            float re = (6.7308e4f * (2.0f * pos.w)) * dva;
-            float cd = (rf_div(24.0f, re) + rf_div(6.0f, 1.0f + rf_sqrt(re))) + 0.4f;
-            float k = (((0.5f * 1.225f) * cd) * aom) * dva;
-            float4 du = make_float4(k * dv.x + 0.0f, k * dv.y + 0.0f, k * dv.z + (-9.8f), k * dv.w + 0.0f);
-            vel.x = vel.x + du.x * dt;
-            vel.y = vel.y + du.y * dt;
-            vel.z = vel.z + du.z * dt;
-            vel.w = vel.w + du.w * 0.0f;
-            // |v| > max(1, 3 |v_bg|) (rs.cl:716) on squared norms
-            float v2 = ((vel.x * vel.x + vel.y * vel.y) + vel.z * vel.z) + vel.w * vel.w;
-            float b2 = ((bg.x * bg.x + bg.y * bg.y) + bg.z * bg.z) + bg.w * bg.w;
-            if ((concept & 8u) && v2 > __builtin_fmaxf(1.0f, 9.0f * b2)) {
-                vel.x = bg.x + 0.0f * dt;
-                vel.y = bg.y + 0.0f * dt;
-                vel.z = bg.z + (-9.8f) * dt;
-                vel.w = bg.w + 0.0f * 0.0f;
-            }
-        } else {
-            vel.x = vel.x + 0.0f * dt;
-            vel.y = vel.y + 0.0f * dt;
-            vel.z = vel.z + (-9.8f) * dt;
-            vel.w = vel.w + 0.0f * 0.0f;
-        }
-        return false;
-    }
-    if (HILO) {
-        hilo_step(pos.x, lo.x, vel.x, dt);
-        hilo_step(pos.y, lo.y, vel.y, dt);
-        hilo_step(pos.z, lo.z, vel.z, dt);
     } else {
-        pos.x = pos.x + vel.x * dt;
-        pos.y = pos.y + vel.y * dt;
-        pos.z = pos.z + vel.z * dt;
+        if (HILO) {
+            hilo_step(pos.x, lo.x, vel.x, dt);
+            hilo_step(pos.y, lo.y, vel.y, dt);
+            hilo_step(pos.z, lo.z, vel.z, dt);
+        } else {
+            pos.x = pos.x + vel.x * dt;
+            pos.y = pos.y + vel.y * dt;
+            pos.z = pos.z + vel.z * dt;
+        }
+        pos.w = pos.w + vel.w * 0.0f;
     }
-    pos.w = pos.w + vel.w * 0.0f;
     if (is_outside(pos, sd)) {
         uint4 seed = a.rnd[i];
         float4 r = rand4(seed);
         a.rnd[i] = seed;
-        pos.x = r.x * sd.s[12] + sd.s[8];
-        pos.y = r.y * sd.s[13] + sd.s[9];
-        pos.z = r.z * sd.s[14] + sd.s[10];
+        if (a.dragged) {
+            pos.x = rf_fma(r.x, sd.s[12], sd.s[8]);
+            pos.y = rf_fma(r.y, sd.s[13], sd.s[9]);
+            pos.z = rf_fma(r.z, sd.s[14], sd.s[10]);
+        } else {
+            pos.x = r.x * sd.s[12] + sd.s[8];
+            pos.y = r.y * sd.s[13] + sd.s[9];
+            pos.z = r.z * sd.s[14] + sd.s[10];
+        }
         vel = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         lo = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         return true;
     }
-    float4 c = wind_index(pos, a.les, sd, logtab);
-    vel = sample3d(a.wind, c.x, c.y, c.z);
     return false;
 }
 
-__device__ __forceinline__ void logtab_load(float *dst, const float *src) {
-    for (int j = threadIdx.x; j < 3 * RF_LOG_TABLE; j += blockDim.x) dst[j] = src[j];
+__device__ __forceinline__ void rain_gather(const RainK &a, float4 pos, const float4 *logtab, Gather3 &g) {
+    float4 c = wind_index(pos, a.les, a.sd, logtab);
+    gather3d(a.wind, c.x, c.y, c.z, g);
 }
+
+__device__ __forceinline__ void rain_drag(const RainK &a, float4 pos, float4 &vel, float4 bg) {
+    if (!a.dragged) {
+        vel = bg;
+        return;
+    }
+    const float dt = a.sd.s[11];
+    const uint32_t concept = __float_as_uint(a.sd.s[5]);
+    const float aom = rf_div(0.003006012f, pos.w);
+    float4 dv = make_float4(bg.x - vel.x, bg.y - vel.y, bg.z - vel.z, bg.w - vel.w);
+    float dva = rf_len3(dv.x, dv.y, dv.z);
+    if (dva > 1.0e-3f) {
+        float re = (6.7308e4f * (2.0f * pos.w)) * dva;
+        float cd = (rf_div(24.0f, re) + rf_div(6.0f, 1.0f + rf_sqrt(re))) + 0.4f;
+        float k = (((0.5f * 1.225f) * cd) * aom) * dva;
+        float4 du = make_float4(k * dv.x + 0.0f, k * dv.y + 0.0f, k * dv.z + (-9.8f), k * dv.w + 0.0f);
+        vel.x = vel.x + du.x * dt;
+        vel.y = vel.y + du.y * dt;
+        vel.z = vel.z + du.z * dt;
+        vel.w = vel.w + du.w * 0.0f;
+        // |v| > max(1, 3 |v_bg|) (rs.cl:716) on squared norms
+        float v2 = ((vel.x * vel.x + vel.y * vel.y) + vel.z * vel.z) + vel.w * vel.w;
+        float b2 = ((bg.x * bg.x + bg.y * bg.y) + bg.z * bg.z) + bg.w * bg.w;
+        if ((concept & 8u) && v2 > __builtin_fmaxf(1.0f, 9.0f * b2)) {
+            vel.x = bg.x + 0.0f * dt;
+            vel.y = bg.y + 0.0f * dt;
+            vel.z = bg.z + (-9.8f) * dt;
+            vel.w = bg.w + 0.0f * 0.0f;
+        }
+    } else {
+        vel.x = vel.x + 0.0f * dt;
+        vel.y = vel.y + 0.0f * dt;
+        vel.z = vel.z + (-9.8f) * dt;
+        vel.w = vel.w + 0.0f * 0.0f;
+    }
+}
+
+__device__ __forceinline__ void logtab_load(float4 *dst, const float *src) { rf_log_rows(dst, src); }
 
 // Scatterers [lo, hi) of one rain population: advance and/or pulse terms into
 // the block's LDS gate sums acc (cleared by the caller).
-template <bool ADV, bool PULSE, bool HILO, bool WIN>
-__device__ __forceinline__ void rain_range_t(const RainK &a, size_t lo, size_t hi, double *acc, const float *logtab) {
+template <bool ADV, bool PULSE, bool HILO, bool WIN, bool TRI>
+__device__ __forceinline__ void rain_range_t(const RainK &a, size_t lo, size_t hi, double *acc, const float4 *logtab) {
     const float2 *wl = win_weights(acc, a.pk);
     const float *grl = win_gates(acc, a.pk);
+    const bool tl = PULSE && a.tab_lds;
+    const float4 *dropt = tl ? (const float4 *)((const char *)acc + a.tab_off) : a.drop;
+    const float *angt = tl ? (const float *)(dropt + a.drop_n) : a.pk.angw;
     WaveWin win;
     win.kb = RSD_WIN_EMPTY;
     wwin_clear(win);
@@ -946,34 +1023,44 @@ __device__ __forceinline__ void rain_range_t(const RainK &a, size_t lo, size_t h
         bool have_rcs = false;
         float4 plo = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (HILO && live) plo = a.plo[i];
+        // Advance: the new position first; the wind texels of that position
+        // are read next and consumed only after the pulse terms (which need
+        // the position, not the velocity), so the reads overlap that work.
+        float4 v = v_in;
+        bool moved = false;     // live, advanced and not respawned
+        Gather3 g;
         if (ADV && live) {
-            float4 v = v_in;
             const float4 p_in = p;
-            bool respawned = rain_advance<HILO>(a, i, p, v, logtab, plo);
+            bool respawned = rain_move<HILO>(a, i, p, v, plo);
             if (HILO) a.plo[i] = plo;
             if (respawned) {
                 if (!stale) {
-                    rcs = ellipsoid_rcs(p_in, a.drop, a.drop_desc);
+                    rcs = ellipsoid_rcs(p_in, dropt, a.drop_desc);
                     a.rcs[i] = rcs;
                     stale = true;
                     have_rcs = true;
                 }
             } else {
                 stale = false;
+                moved = true;
+                rain_gather(a, p, logtab, g);
             }
             a.pos[i] = make_float4(p.x, p.y, p.z, stale ? -p.w : p.w);
-            a.vel[i] = v;
         }
         if (PULSE) {
             float4 sa = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             float r = 0.0f;
             bool on = false;
             if (live) {
-                if (!have_rcs) rcs = stale ? a.rcs[i] : ellipsoid_rcs(p, a.drop, a.drop_desc);
-                on = scat_signal<HILO>(p, rcs, a.pk, sa, r, plo);
+                if (!have_rcs) rcs = stale ? a.rcs[i] : ellipsoid_rcs(p, dropt, a.drop_desc);
+                on = scat_signal<HILO>(p, rcs, a.pk, angt, sa, r, plo);
             }
-            if (WIN) wwin_fold(win, on, r, sa, a.pk, acc, wl, grl);
+            if (WIN) wwin_fold<TRI>(win, on, r, sa, a.pk, acc, wl, grl);
             else fold_gates(on, r, sa, a.pk, acc);
+        }
+        if (ADV && live) {
+            if (moved) rain_drag(a, p, v, lerp3d(g));
+            a.vel[i] = v;
         }
     }
     if (WIN) wwin_flush(win, a.pk, acc);
@@ -983,10 +1070,11 @@ __device__ __forceinline__ void rain_range_t(const RainK &a, size_t lo, size_t h
 // registers has no second fold path (which makes the compiler copy the 32
 // window registers at every step).
 template <bool ADV, bool PULSE, bool HILO>
-__device__ __forceinline__ void rain_range(const RainK &a, size_t lo, size_t hi, double *acc, const float *logtab,
+__device__ __forceinline__ void rain_range(const RainK &a, size_t lo, size_t hi, double *acc, const float4 *logtab,
                                            bool use_win) {
-    if (use_win) rain_range_t<ADV, PULSE, HILO, true>(a, lo, hi, acc, logtab);
-    else rain_range_t<ADV, PULSE, HILO, false>(a, lo, hi, acc, logtab);
+    if (use_win && a.pk.tri) rain_range_t<ADV, PULSE, HILO, true, true>(a, lo, hi, acc, logtab);
+    else if (use_win) rain_range_t<ADV, PULSE, HILO, true, false>(a, lo, hi, acc, logtab);
+    else rain_range_t<ADV, PULSE, HILO, false, false>(a, lo, hi, acc, logtab);
 }
 
 // Many independent rain domains in one persistent launch.  items[t] = (domain,
@@ -998,7 +1086,7 @@ template <bool ADV, bool PULSE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_rain_batch(
     const RainK *doms, const int2 *items, int n_items, int chunk, const float *logtab_g) {
     extern __shared__ double acc[];
-    __shared__ float logtab[3 * RF_LOG_TABLE];
+    __shared__ float4 logtab[RF_LOG_TABLE];
     if (ADV) logtab_load(logtab, logtab_g);
     __syncthreads();
     const unsigned G = gridDim.x, b = blockIdx.x;
@@ -1169,7 +1257,7 @@ __device__ __forceinline__ void debris_forces(const DebrisK &a, float4 ori, floa
     tum = norm4(tq);
 }
 
-__device__ bool debris_advance(const DebrisK &a, size_t i, float4 &pos, float4 &ori, float4 &vel, float4 &tum, float4 &rcs, const float *logtab) {
+__device__ __forceinline__ bool debris_advance(const DebrisK &a, size_t i, float4 &pos, float4 &ori, float4 &vel, float4 &tum, float4 &rcs, const float4 *logtab) {
     const F16 &sd = a.sd;
     const uint32_t concept = __float_as_uint(sd.s[5]);
     if (debris_move(a, i, pos, ori, vel, tum, rcs)) return true;
@@ -1199,7 +1287,7 @@ __device__ bool debris_advance(const DebrisK &a, size_t i, float4 &pos, float4 &
 // block's LDS gate sums (cleared, window tables staged by the caller).  SURR:
 // the surrogate network in LDS (nw, nb) with per-wave scratch nscr.
 template <bool ADV, bool PULSE, bool SURR>
-__device__ __forceinline__ void debris_range(const DebrisK &a, size_t lo, size_t hi, double *acc, const float *logtab,
+__device__ __forceinline__ void debris_range(const DebrisK &a, size_t lo, size_t hi, double *acc, const float4 *logtab,
                                              const uint16_t *nw, const float *nb, float *nscr) {
     const bool use_win = PULSE && a.pk.fold == RSD_FOLD_WINDOW;
     const float2 *wl = win_weights(acc, a.pk);
@@ -1289,7 +1377,7 @@ __device__ __forceinline__ void debris_range(const DebrisK &a, size_t lo, size_t
         if (PULSE) {
             float4 sa = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             float r = 0.0f;
-            bool on = live && scat_signal(p, rcs, a.pk, sa, r);
+            bool on = live && scat_signal(p, rcs, a.pk, a.pk.angw, sa, r);
             if (use_win) win_fold(win, on, r, sa, a.pk, acc, wl, grl);
             else fold_gates(on, r, sa, a.pk, acc);
         }
@@ -1300,7 +1388,7 @@ __device__ __forceinline__ void debris_range(const DebrisK &a, size_t lo, size_t
 template <bool ADV, bool PULSE, bool SURR>
 __global__ __launch_bounds__(256) void k_debris(DebrisK a) {
     extern __shared__ double acc[];
-    __shared__ float logtab[3 * RF_LOG_TABLE];
+    __shared__ float4 logtab[RF_LOG_TABLE];
     __shared__ uint16_t nw[SURR ? MLP_WN : 2];
     __shared__ float nb[SURR ? MLP_BN : 2];
     __shared__ float nscr[SURR ? 4 * 64 * MLP_N4 : 2];
@@ -1322,7 +1410,8 @@ __global__ __launch_bounds__(256) void k_debris(DebrisK a) {
 // Rain launch of one step.  Debris types with small populations ride in the
 // same launch (no second stream, no cross-stream events per pulse): blocks
 // [first[j], first[j + 1]) take debris type j, the rain takes the blocks after
-// first[nt] (a multiple of 8, so the rain blocks keep their XCD mapping).
+// first[nt] (a multiple of 8, so the rain blocks' partial rows keep their XCD
+// order).
 #define RSD_STEP_DEBRIS 3
 struct StepDebris {
     int nt;                             // debris types in this launch
@@ -1333,7 +1422,7 @@ struct StepDebris {
 template <bool ADV, bool PULSE, bool HILO>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_rain(RainK a, StepDebris d) {
     extern __shared__ double acc[];
-    __shared__ float logtab[3 * RF_LOG_TABLE];
+    __shared__ float4 logtab[RF_LOG_TABLE];
     if (ADV) logtab_load(logtab, a.logtab);
     const unsigned nd = (unsigned)d.first[d.nt];
     if (blockIdx.x < nd) {
@@ -1353,8 +1442,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     if (PULSE) acc_clear(acc, 4 * a.pk.R);
     else __syncthreads();
     size_t lo, hi;
-    const unsigned L = block_range_at(0, a.n, blockIdx.x - nd, gridDim.x - nd, lo, hi);
+    const unsigned L = block_range_at(0, a.n, blockIdx.x - nd, gridDim.x - nd, lo, hi, true);
     const bool use_win = PULSE && a.pk.fold == RSD_FOLD_WINDOW;
+    if (PULSE) stage_rain_tables(acc, a);
     if (use_win) stage_window(acc, a.pk);
     rain_range<ADV, PULSE, HILO>(a, lo, hi, acc, logtab, use_win);
     if (PULSE) acc_flush_row(acc, a.pk, a.pk.row0 + (int)L);
@@ -1430,7 +1520,10 @@ __global__ __launch_bounds__(256) void k_io(const float4 *__restrict__ in, float
 }
 
 // Device evaluation of the binary32 functions the kernels use (self-test).
-__global__ void k_math_probe(int fn, const float *a, const float *b, float *out, unsigned long long n, const float *logtab) {
+__global__ void k_math_probe(int fn, const float *a, const float *b, float *out, unsigned long long n, const float *logtab_g) {
+    __shared__ float4 logtab[RF_LOG_TABLE];
+    rf_log_rows(logtab, logtab_g);
+    __syncthreads();
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float x = a[i], y = b[i], s, c;
@@ -1699,7 +1792,11 @@ struct rsd_engine {
     rsd_step pending;
     int have_pulse;
     rsd_pulse last_pulse;
-    int block, max_blocks;
+    int block, max_blocks;   // max_blocks 0: rain launches take one resident round
+    int cus;                 // compute units of the device
+    int rain_per_cu;         // resident fused rain blocks per compute unit (0: not known yet)
+    int rain_occ_hilo;       // variant and LDS size rain_per_cu was computed for
+    size_t rain_occ_lds;
     hipEvent_t ev0, ev1;
     float *logtab;           // 3 x 128 floats of the wind-index log1p
     int prof_on, prof_n;
@@ -1813,7 +1910,8 @@ extern "C" int rsd_open(int device_index, rsd_engine **out) {
     if (!e) return RSD_ERR_ALLOC;
     e->device = device_index;
     e->block = 256;
-    e->max_blocks = 2048;
+    e->max_blocks = 0;
+    if (rsd_device_props(device_index, &e->cus, NULL, NULL) != RSD_OK || e->cus < 1) e->cus = 1;
     e->sort_interval = 32;
     e->fold = RSD_FOLD_WINDOW;
     if (hipSetDevice(device_index) != hipSuccess || hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
@@ -1899,9 +1997,8 @@ extern "C" const char *rsd_last_error(const rsd_engine *e) { return e ? e->err :
 extern "C" int rsd_set_launch(rsd_engine *e, int block_size, int max_blocks) {
     if (!e) return RSD_ERR_ARG;
     if (block_size == 0) block_size = 256;
-    if (max_blocks == 0) max_blocks = 2048;
     if (block_size != 64 && block_size != 128 && block_size != 256) return fail(e, RSD_ERR_ARG, "block size %d not in {64,128,256}", block_size);
-    if (max_blocks < 1 || max_blocks > 65536) return fail(e, RSD_ERR_ARG, "max_blocks %d out of range", max_blocks);
+    if (max_blocks < 0 || max_blocks > 65536) return fail(e, RSD_ERR_ARG, "max_blocks %d out of range", max_blocks);
     e->block = block_size;
     e->max_blocks = max_blocks;
     return RSD_OK;
@@ -1941,11 +2038,31 @@ extern "C" int rsd_alloc_state(rsd_engine *e, size_t num_scats, const size_t *or
     return RSD_OK;
 }
 
+#define RSD_DEFAULT_BLOCKS 2048
+
 static int blocks_for(const rsd_engine *e, size_t n) {
     size_t b = (n + (size_t)e->block - 1) / (size_t)e->block;
+    const size_t cap = e->max_blocks > 0 ? (size_t)e->max_blocks : RSD_DEFAULT_BLOCKS;
     if (b < 1) b = 1;
-    if (b > (size_t)e->max_blocks) b = (size_t)e->max_blocks;
+    if (b > cap) b = cap;
     return (int)b;
+}
+
+// Blocks of the fused rain kernel that are resident on the device at once
+// (occupancy of the launch's variant and LDS size, times the compute units).
+// Launches of a multiple of that many equal blocks run in whole rounds (no
+// block waits alone for a free slot at the end).
+static int rain_resident_blocks(rsd_engine *e, bool hilo, size_t lds) {
+    if (!e->rain_per_cu || e->rain_occ_hilo != (int)hilo || e->rain_occ_lds != lds) {
+        int nb = 0;
+        const void *k = hilo ? reinterpret_cast<const void *>(&k_rain<true, true, true>)
+                             : reinterpret_cast<const void *>(&k_rain<true, true, false>);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, e->block, lds) != hipSuccess || nb < 1) nb = 1;
+        e->rain_per_cu = nb;
+        e->rain_occ_hilo = (int)hilo;
+        e->rain_occ_lds = lds;
+    }
+    return e->rain_per_cu * e->cus;
 }
 
 static int ensure_partials(rsd_engine *e, size_t rows, int R) {
@@ -2038,6 +2155,10 @@ static int fill_pulse(rsd_engine *e, const rsd_pulse *p, PulseK &k) {
     k.gate_start = p->range_start;
     k.gate_inv_delta = p->range_delta != 0.0f ? 1.0f / p->range_delta : 0.0f;
     range_support(e, p->range_delta, &k.sup_lo, &k.sup_hi, &k.dense);
+    k.tri = e->rangew_n == 3 && e->rangew_h[0] == 0.0f && e->rangew_h[1] == 1.0f && e->rangew_h[2] == 0.0f &&
+            e->rangew_desc.s[2] == 2.0f;
+    k.tri_s0 = e->rangew_desc.s[0];
+    k.tri_s1 = e->rangew_desc.s[1] - 1.0f;
     if (p->range_delta == 0.0f) k.dense = 1;
     k.phase_mode = p->phase_mode;
     k.fold = e->fold;
@@ -2124,7 +2245,6 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p) {
     if (p && (R < 1 || R > RSD_MAX_GATES)) return fail(e, RSD_ERR_ARG, "range_count %d out of range", R);
     // Rows of the partial-sum buffer: one per block of every launch, rain
     // first, then each debris type in type order.
-    int rain_blocks = e->n_rain ? blocks_for(e, e->n_rain) : 0;
     int deb_blocks[RSD_MAX_TYPES] = {0};
     bool ride[RSD_MAX_TYPES] = {false};
     int n_ride = 0, ride_blocks = 0, last_ride = -1;
@@ -2138,16 +2258,28 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p) {
             last_ride = k;
         }
     }
-    if (last_ride > 0 && ride_blocks % 8) {  // the rain blocks keep their XCD mapping
+    if (last_ride > 0 && ride_blocks % 8) {  // the rain blocks keep their XCD order
         deb_blocks[last_ride] += 8 - ride_blocks % 8;
         ride_blocks += 8 - ride_blocks % 8;
     }
-    int rows = rain_blocks, row_of[RSD_MAX_TYPES] = {0};
     bool any_debris = false;       // types on the side stream
+    for (int k = 1; k < RSD_MAX_TYPES; k++)
+        if (e->counts[k] && !ride[k]) any_debris = true;
+    // Rain blocks: by default, when no debris kernel shares the device, two
+    // resident rounds of equal blocks (next to the riding debris blocks).
+    int rain_blocks = e->n_rain ? blocks_for(e, e->n_rain) : 0;
+    if (e->n_rain && e->max_blocks == 0 && !any_debris && s && p) {
+        const size_t lds_est = (size_t)4 * R * sizeof(double) + (size_t)e->rangew_n * sizeof(float2) +
+                               (size_t)R * sizeof(float) + 16 + (size_t)RSD_LDS_DROP_MAX * sizeof(float4) +
+                               (size_t)RSD_LDS_ANGW_MAX * sizeof(float);
+        const int round = 2 * rain_resident_blocks(e, e->plo != NULL, lds_est) - ride_blocks;
+        const size_t need = (e->n_rain + 63) / 64;   // at least a wave per block
+        rain_blocks = round < 1 ? 1 : ((size_t)round < need ? round : (int)need);
+    }
+    int rows = rain_blocks, row_of[RSD_MAX_TYPES] = {0};
     for (int k = 1; k < RSD_MAX_TYPES; k++) {
         row_of[k] = rows;
         rows += deb_blocks[k];
-        if (e->counts[k] && !ride[k]) any_debris = true;
     }
     PulseK pk;
     memset(&pk, 0, sizeof(pk));
@@ -2204,17 +2336,29 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p) {
             d.first[d.nt + 1] = d.first[d.nt] + deb_blocks[k];
             d.nt++;
         }
+        size_t lds_rain = lds;
+        if (p) {
+            const int dn = (int)e->drop_desc.s[2] + 1, an = (int)e->angw_desc.s[2] + 1;
+            if (dn >= 1 && dn <= RSD_LDS_DROP_MAX && an >= 1 && an <= RSD_LDS_ANGW_MAX) {
+                a.tab_lds = 1;
+                a.drop_n = dn;
+                a.angw_n = an;
+                const size_t off = rain_tab_offset(R, pk.fold == RSD_FOLD_WINDOW, e->rangew_n);
+                a.tab_off = (unsigned)off;
+                lds_rain = off + (size_t)dn * sizeof(float4) + (size_t)an * sizeof(float);
+            }
+        }
         const dim3 grid(ride_blocks + rain_blocks);
         const bool timed = e->prof_on && e->prof_n < RSD_PROF_CAP && (e->prof_tick++ % e->prof_stride) == 0;
         if (timed) HIPCHK(e, hipEventRecord(e->prof_ev[2 * e->prof_n], e->stream));
         if (e->plo) {
-            if (s && p) hipLaunchKernelGGL((k_rain<true, true, true>), grid, dim3(e->block), lds, e->stream, a, d);
+            if (s && p) hipLaunchKernelGGL((k_rain<true, true, true>), grid, dim3(e->block), lds_rain, e->stream, a, d);
             else if (s) hipLaunchKernelGGL((k_rain<true, false, true>), grid, dim3(e->block), 0, e->stream, a, d);
-            else hipLaunchKernelGGL((k_rain<false, true, true>), grid, dim3(e->block), lds, e->stream, a, d);
+            else hipLaunchKernelGGL((k_rain<false, true, true>), grid, dim3(e->block), lds_rain, e->stream, a, d);
         } else {
-            if (s && p) hipLaunchKernelGGL((k_rain<true, true, false>), grid, dim3(e->block), lds, e->stream, a, d);
+            if (s && p) hipLaunchKernelGGL((k_rain<true, true, false>), grid, dim3(e->block), lds_rain, e->stream, a, d);
             else if (s) hipLaunchKernelGGL((k_rain<true, false, false>), grid, dim3(e->block), 0, e->stream, a, d);
-            else hipLaunchKernelGGL((k_rain<false, true, false>), grid, dim3(e->block), lds, e->stream, a, d);
+            else hipLaunchKernelGGL((k_rain<false, true, false>), grid, dim3(e->block), lds_rain, e->stream, a, d);
         }
         HIPCHK(e, hipGetLastError());
         if (timed) {
