@@ -37,6 +37,7 @@
 // id and uploads first restore the reference order, so callers never see it.
 // aux (read only by downloads) stays in the reference order.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <hipcub/hipcub.hpp>
 
 #include <math.h>
@@ -1096,6 +1097,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         const RainK &a = doms[it.x];
         if (PULSE) acc_clear(acc, 4 * a.pk.R);
         const bool use_win = PULSE && a.pk.fold == RSD_FOLD_WINDOW;
+        if (PULSE) stage_rain_tables(acc, a);
         if (use_win) stage_window(acc, a.pk);
         const size_t lo = (size_t)it.y * (size_t)chunk;
         const size_t hi = lo + (size_t)chunk < a.n ? lo + (size_t)chunk : a.n;
@@ -2191,6 +2193,19 @@ static int fill_rain(rsd_engine *e, const rsd_step *s, RainK &a) {
     return RSD_OK;
 }
 
+// Stage the drop RCS and angular weight tables of a rain pulse launch in LDS
+// when they fit; returns the launch's dynamic LDS bytes (base: without them).
+static size_t set_rain_tables(const rsd_engine *e, RainK &a, int R, size_t base) {
+    const int dn = (int)e->drop_desc.s[2] + 1, an = (int)e->angw_desc.s[2] + 1;
+    if (dn < 1 || dn > RSD_LDS_DROP_MAX || an < 1 || an > RSD_LDS_ANGW_MAX) return base;
+    a.tab_lds = 1;
+    a.drop_n = dn;
+    a.angw_n = an;
+    const size_t off = rain_tab_offset(R, a.pk.fold == RSD_FOLD_WINDOW, e->rangew_n);
+    a.tab_off = (unsigned)off;
+    return off + (size_t)dn * sizeof(float4) + (size_t)an * sizeof(float);
+}
+
 static int fill_debris(rsd_engine *e, const rsd_step *s, const rsd_pulse *p, int type, DebrisK &a) {
     a.pos = e->pos; a.ori = e->ori; a.vel = e->vel; a.tum = e->tum; a.rcs = e->rcs; a.rnd = e->rnd;
     a.begin = e->origins[type];
@@ -2336,33 +2351,24 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p) {
             d.first[d.nt + 1] = d.first[d.nt] + deb_blocks[k];
             d.nt++;
         }
-        size_t lds_rain = lds;
-        if (p) {
-            const int dn = (int)e->drop_desc.s[2] + 1, an = (int)e->angw_desc.s[2] + 1;
-            if (dn >= 1 && dn <= RSD_LDS_DROP_MAX && an >= 1 && an <= RSD_LDS_ANGW_MAX) {
-                a.tab_lds = 1;
-                a.drop_n = dn;
-                a.angw_n = an;
-                const size_t off = rain_tab_offset(R, pk.fold == RSD_FOLD_WINDOW, e->rangew_n);
-                a.tab_off = (unsigned)off;
-                lds_rain = off + (size_t)dn * sizeof(float4) + (size_t)an * sizeof(float);
-            }
-        }
+        const size_t lds_rain = p ? set_rain_tables(e, a, R, lds) : lds;
         const dim3 grid(ride_blocks + rain_blocks);
         const bool timed = e->prof_on && e->prof_n < RSD_PROF_CAP && (e->prof_tick++ % e->prof_stride) == 0;
-        if (timed) HIPCHK(e, hipEventRecord(e->prof_ev[2 * e->prof_n], e->stream));
+        // A timed launch carries its events in the dispatch itself
+        // (hipExtLaunchKernelGGL): they bracket the kernel's execution, not the
+        // queue's processing of separate event packets.
+        hipEvent_t t0 = timed ? e->prof_ev[2 * e->prof_n] : nullptr, t1 = timed ? e->prof_ev[2 * e->prof_n + 1] : nullptr;
         if (e->plo) {
-            if (s && p) hipLaunchKernelGGL((k_rain<true, true, true>), grid, dim3(e->block), lds_rain, e->stream, a, d);
-            else if (s) hipLaunchKernelGGL((k_rain<true, false, true>), grid, dim3(e->block), 0, e->stream, a, d);
-            else hipLaunchKernelGGL((k_rain<false, true, true>), grid, dim3(e->block), lds_rain, e->stream, a, d);
+            if (s && p) hipExtLaunchKernelGGL((k_rain<true, true, true>), grid, dim3(e->block), lds_rain, e->stream, t0, t1, 0, a, d);
+            else if (s) hipExtLaunchKernelGGL((k_rain<true, false, true>), grid, dim3(e->block), 0, e->stream, t0, t1, 0, a, d);
+            else hipExtLaunchKernelGGL((k_rain<false, true, true>), grid, dim3(e->block), lds_rain, e->stream, t0, t1, 0, a, d);
         } else {
-            if (s && p) hipLaunchKernelGGL((k_rain<true, true, false>), grid, dim3(e->block), lds_rain, e->stream, a, d);
-            else if (s) hipLaunchKernelGGL((k_rain<true, false, false>), grid, dim3(e->block), 0, e->stream, a, d);
-            else hipLaunchKernelGGL((k_rain<false, true, false>), grid, dim3(e->block), lds_rain, e->stream, a, d);
+            if (s && p) hipExtLaunchKernelGGL((k_rain<true, true, false>), grid, dim3(e->block), lds_rain, e->stream, t0, t1, 0, a, d);
+            else if (s) hipExtLaunchKernelGGL((k_rain<true, false, false>), grid, dim3(e->block), 0, e->stream, t0, t1, 0, a, d);
+            else hipExtLaunchKernelGGL((k_rain<false, true, false>), grid, dim3(e->block), lds_rain, e->stream, t0, t1, 0, a, d);
         }
         HIPCHK(e, hipGetLastError());
         if (timed) {
-            HIPCHK(e, hipEventRecord(e->prof_ev[2 * e->prof_n + 1], e->stream));
             e->prof_kind[e->prof_n] = (s ? 1 : 0) | (p ? 2 : 0);
             e->prof_n++;
         }
@@ -3281,6 +3287,7 @@ extern "C" int rsd_batch_launch(rsd_batch *b) {
         first[k] = total4r;
         total4r += 4 * R;
         size_t l = (size_t)4 * R * sizeof(double) + (a.pk.fold == RSD_FOLD_WINDOW ? (size_t)e->rangew_n * sizeof(float2) + (size_t)R * sizeof(float) : 0);
+        l = set_rain_tables(e, rk[k], R, l);
         if (l > lds) lds = l;
     }
     first[ns] = total4r;
