@@ -134,7 +134,11 @@ int rsd_profile_read(rsd_engine *e, int kind, double *total_ms, int *count);
  * the achieved read + write rate in GB/s. */
 int rsd_io_test(rsd_engine *e, size_t n, int reps, double *gbps);
 
-/* Launch shape of the scatterer kernels (0 = built-in default). */
+/* Launch shape of the scatterer kernels.  block_size 64, 128 or 256 (0: 256);
+ * max_blocks caps the grid of every scatterer launch (1..65536).  max_blocks 0
+ * (the default): a fused rain launch with no debris kernel beside it takes two
+ * resident rounds of equal blocks (occupancy x compute units, less the blocks
+ * of the riding debris types); other launches take at most 2048 blocks. */
 int rsd_set_launch(rsd_engine *e, int block_size, int max_blocks);
 
 /* Spatial order of the rain streams: every `interval` pulses (default 32,

@@ -48,7 +48,7 @@ def test_device_atan2_div_match(gpu_ready):
     assert np.array_equal(_bits(sa.math_probe(7, y, x)), _bits(y / x))
 
 
-def _run(cfg, pulses, sort=None, reupload_at=None, fold=None):
+def _run(cfg, pulses, sort=None, reupload_at=None, fold=None, launch=None):
     r = sa.Radar(verb=0)
     h.configure(r, cfg)
     r.populate()
@@ -57,6 +57,8 @@ def _run(cfg, pulses, sort=None, reupload_at=None, fold=None):
         r.set_sort(sort)
     if fold is not None:
         r.set_fold(fold)
+    if launch is not None:
+        r.set_launch(*launch)
     st = h.state_from_radar(r)     # host streams still hold the populated state
     o = h.OracleRun(r, cfg, st)
     sd_adv = r.sim_desc().copy()   # populate's zero-length step (rs.c:4941-4955)
@@ -155,6 +157,19 @@ def test_fold_strategies(gpu_ready, fold, sort):
     """Every gate-sum strategy, on range-sorted and on reference-order streams."""
     cfg = h.Config(n_rain=16384, debris=[(sa.OBJ_LEAF, 256)], scan="P:3.0,-1:1:0.01")
     r, o, pg, po = _run(cfg, 4, sort=sort, fold=fold)
+    _check_state(r, o)
+    errs = [h.rel_err(pg[k], po[k]) for k in range(len(pg))]
+    assert max(errs) < IQ_TOL, errs
+    r.free()
+
+
+@pytest.mark.parametrize("launch", [(256, 0), (256, 37), (128, 1016), (64, 3)])
+def test_launch_shapes(gpu_ready, launch):
+    """Block size and grid (0: two resident rounds; grids that are not a
+    multiple of 8; fewer blocks than chunks of a wave) change only which block
+    sums which scatterers: state bit-exact, IQ within tolerance."""
+    cfg = h.Config(n_rain=20000, debris=[(sa.OBJ_LEAF, 300)], scan="P:3.0,-1:1:0.01")
+    r, o, pg, po = _run(cfg, 5, sort=2, launch=launch)
     _check_state(r, o)
     errs = [h.rel_err(pg[k], po[k]) for k in range(len(pg))]
     assert max(errs) < IQ_TOL, errs
