@@ -832,10 +832,6 @@ __device__ __forceinline__ void acc_flush_row(const double *acc, const PulseK &k
     for (int j = threadIdx.x; j < 4 * k.R; j += blockDim.x) k.partials[(size_t)j * k.rows + row] = acc[j];
 }
 
-__device__ __forceinline__ void acc_flush(const double *acc, const PulseK &k) {
-    acc_flush_row(acc, k, k.row0 + blockIdx.x);
-}
-
 // ---------------------------------------------------------------------------
 // Rain (type 0): el_atts (rs.cl:653-739) or bg_atts (rs.cl:533-580), fused with
 // the pulse.  Template flags select advance and/or pulse.
@@ -1387,39 +1383,53 @@ __device__ __forceinline__ void debris_range(const DebrisK &a, size_t lo, size_t
     if (use_win) win_flush(win, a.pk, acc);
 }
 
+// Debris types of one launch: blocks [first[j], first[j + 1]) take type j.
+#define RSD_STEP_DEBRIS 3
+struct StepDebris {
+    int nt;                             // debris types in this launch
+    int first[RSD_STEP_DEBRIS + 1];     // first block of each type; first[nt]: the rest
+    DebrisK t[RSD_STEP_DEBRIS];
+};
+
+__device__ __forceinline__ int step_type(const StepDebris &d) {
+    int j = 0;
+    while (j + 1 < d.nt && (int)blockIdx.x >= d.first[j + 1]) j++;
+    return j;
+}
+
+// Debris launch of one step on the side stream (surrogate-driven types, or
+// table types that do not ride in the rain launch): up to RSD_STEP_DEBRIS
+// types in one grid, so their blocks share the device instead of queueing
+// one type after another.
 template <bool ADV, bool PULSE, bool SURR>
-__global__ __launch_bounds__(256) void k_debris(DebrisK a) {
+__global__ __launch_bounds__(256) void k_debris(StepDebris d) {
     extern __shared__ double acc[];
     __shared__ float4 logtab[RF_LOG_TABLE];
     __shared__ uint16_t nw[SURR ? MLP_WN : 2];
     __shared__ float nb[SURR ? MLP_BN : 2];
     __shared__ float nscr[SURR ? 4 * 64 * MLP_N4 : 2];
+    const int j = step_type(d);
+    const DebrisK &a = d.t[j];
     if (SURR) {
-        for (int j = threadIdx.x; j < MLP_WN; j += blockDim.x) nw[j] = a.net.w[j];
-        for (int j = threadIdx.x; j < MLP_BN; j += blockDim.x) nb[j] = a.net.b[j];
+        for (int q = threadIdx.x; q < MLP_WN; q += blockDim.x) nw[q] = a.net.w[q];
+        for (int q = threadIdx.x; q < MLP_BN; q += blockDim.x) nb[q] = a.net.b[q];
     }
     if (ADV) logtab_load(logtab, a.logtab);
     if (PULSE) acc_clear(acc, 4 * a.pk.R);
     else __syncthreads();
     size_t lo, hi;
-    block_range(a.begin, a.end, lo, hi);
+    const unsigned L = block_range_at(a.begin, a.end, blockIdx.x - (unsigned)d.first[j],
+                                      (unsigned)(d.first[j + 1] - d.first[j]), lo, hi);
     if (PULSE && a.pk.fold == RSD_FOLD_WINDOW) stage_window(acc, a.pk);
     debris_range<ADV, PULSE, SURR>(a, lo, hi, acc, logtab, nw, nb, nscr);
-    if (PULSE) acc_flush(acc, a.pk);
+    if (PULSE) acc_flush_row(acc, a.pk, a.pk.row0 + (int)L);
 }
 
-
-// Rain launch of one step.  Debris types with small populations ride in the
-// same launch (no second stream, no cross-stream events per pulse): blocks
+// Rain launch of one step.  Debris types driven by the tables ride in the same
+// launch (no second stream, no cross-stream events per pulse): blocks
 // [first[j], first[j + 1]) take debris type j, the rain takes the blocks after
 // first[nt] (a multiple of 8, so the rain blocks' partial rows keep their XCD
 // order).
-#define RSD_STEP_DEBRIS 3
-struct StepDebris {
-    int nt;                             // debris types in this launch
-    int first[RSD_STEP_DEBRIS + 1];     // first block of each type; first[nt]: rain
-    DebrisK t[RSD_STEP_DEBRIS];
-};
 
 template <bool ADV, bool PULSE, bool HILO>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_rain(RainK a, StepDebris d) {
@@ -1428,8 +1438,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     if (ADV) logtab_load(logtab, a.logtab);
     const unsigned nd = (unsigned)d.first[d.nt];
     if (blockIdx.x < nd) {
-        int j = 0;
-        while (j + 1 < d.nt && (int)blockIdx.x >= d.first[j + 1]) j++;
+        const int j = step_type(d);
         const DebrisK &da = d.t[j];
         if (PULSE) acc_clear(acc, 4 * da.pk.R);
         else __syncthreads();
@@ -2259,9 +2268,10 @@ static int fill_debris(rsd_engine *e, const rsd_step *s, const rsd_pulse *p, int
 
 // Debris populations small next to the rain ride in the rain launch; larger
 // ones, and types with a surrogate network, run on the side stream.
+// Table-driven debris types ride in the rain launch (up to RSD_STEP_DEBRIS of
+// them); surrogate-driven ones run in k_debris<.., true> on the side stream.
 static bool debris_rides(const rsd_engine *e, int k) {
-    const size_t c = e->counts[k];
-    return e->n_rain > 0 && !e->surr[k] && c > 0 && (c <= 65536 || c * 16 <= e->n_rain);
+    return e->n_rain > 0 && !e->surr[k] && e->counts[k] > 0;
 }
 
 // Launch the kernels of one step: advance (s != NULL) and/or pulse (p != NULL).
@@ -2299,7 +2309,9 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p) {
         const size_t lds_est = (size_t)4 * R * sizeof(double) + (size_t)e->rangew_n * sizeof(float2) +
                                (size_t)R * sizeof(float) + 16 + (size_t)RSD_LDS_DROP_MAX * sizeof(float4) +
                                (size_t)RSD_LDS_ANGW_MAX * sizeof(float);
-        const int round = 2 * rain_resident_blocks(e, e->plo != NULL, lds_est) - ride_blocks;
+        const int res = rain_resident_blocks(e, e->plo != NULL, lds_est);
+        // riders taking a round or more: the rain keeps the default grid
+        const int round = ride_blocks < res ? 2 * res - ride_blocks : rain_blocks;
         const size_t need = (e->n_rain + 63) / 64;   // at least a wave per block
         rain_blocks = round < 1 ? 1 : ((size_t)round < need ? round : (int)need);
     }
@@ -2321,25 +2333,38 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p) {
     if (any_debris) {
         HIPCHK(e, hipEventRecord(e->ev_fork, e->stream));
         HIPCHK(e, hipStreamWaitEvent(e->side, e->ev_fork, 0));
-        for (int k = 1; k < RSD_MAX_TYPES; k++) {
-            if (!e->counts[k] || ride[k]) continue;
-            DebrisK a;
-            memset(&a, 0, sizeof(a));
-            int st = fill_debris(e, s, p, k, a);
-            if (st) return st;
-            a.pk = pk;
-            a.pk.row0 = row_of[k];
-            if (e->surr[k]) {
-                a.net = e->net[k];
-                if (s && p) hipLaunchKernelGGL((k_debris<true, true, true>), dim3(deb_blocks[k]), dim3(256), lds, e->side, a);
-                else if (s) hipLaunchKernelGGL((k_debris<true, false, true>), dim3(deb_blocks[k]), dim3(256), 0, e->side, a);
-                else hipLaunchKernelGGL((k_debris<false, true, true>), dim3(deb_blocks[k]), dim3(256), lds, e->side, a);
-            } else {
-                if (s && p) hipLaunchKernelGGL((k_debris<true, true, false>), dim3(deb_blocks[k]), dim3(e->block), lds, e->side, a);
-                else if (s) hipLaunchKernelGGL((k_debris<true, false, false>), dim3(deb_blocks[k]), dim3(e->block), 0, e->side, a);
-                else hipLaunchKernelGGL((k_debris<false, true, false>), dim3(deb_blocks[k]), dim3(e->block), lds, e->side, a);
+        // side types in launches of up to RSD_STEP_DEBRIS types, surrogate
+        // and table types apart (the surrogate kernel runs 256-thread blocks)
+        for (int surr = 0; surr < 2; surr++) {
+            StepDebris d;
+            memset(&d, 0, sizeof(d));
+            for (int k = 1; k < RSD_MAX_TYPES; k++) {
+                const bool last = k == RSD_MAX_TYPES - 1;
+                if (e->counts[k] && !ride[k] && (e->surr[k] ? 1 : 0) == surr) {
+                    DebrisK &a = d.t[d.nt];
+                    int st = fill_debris(e, s, p, k, a);
+                    if (st) return st;
+                    a.pk = pk;
+                    a.pk.row0 = row_of[k];
+                    if (surr) a.net = e->net[k];
+                    d.first[d.nt + 1] = d.first[d.nt] + deb_blocks[k];
+                    d.nt++;
+                }
+                if (d.nt == RSD_STEP_DEBRIS || (last && d.nt > 0)) {
+                    const dim3 grid(d.first[d.nt]);
+                    if (surr) {
+                        if (s && p) hipLaunchKernelGGL((k_debris<true, true, true>), grid, dim3(256), lds, e->side, d);
+                        else if (s) hipLaunchKernelGGL((k_debris<true, false, true>), grid, dim3(256), 0, e->side, d);
+                        else hipLaunchKernelGGL((k_debris<false, true, true>), grid, dim3(256), lds, e->side, d);
+                    } else {
+                        if (s && p) hipLaunchKernelGGL((k_debris<true, true, false>), grid, dim3(e->block), lds, e->side, d);
+                        else if (s) hipLaunchKernelGGL((k_debris<true, false, false>), grid, dim3(e->block), 0, e->side, d);
+                        else hipLaunchKernelGGL((k_debris<false, true, false>), grid, dim3(e->block), lds, e->side, d);
+                    }
+                    HIPCHK(e, hipGetLastError());
+                    memset(&d, 0, sizeof(d));
+                }
             }
-            HIPCHK(e, hipGetLastError());
         }
         HIPCHK(e, hipEventRecord(e->ev_join, e->side));
     }
