@@ -56,6 +56,25 @@
 #define RSD_PI_2_F 0x1.921fb6p+0f
 #define RSD_SQRT1_2_F 0x1.6a09e6p-1f
 
+// State stream access; NT: non-temporal (the lines are not kept in the caches,
+// which then hold the tables for the scatterers still to come).
+typedef float rsd_v4 __attribute__((ext_vector_type(4)));
+template <bool NT>
+__device__ __forceinline__ float4 ld4(const float4 *p) {
+    if (!NT) return *p;
+    const rsd_v4 v = __builtin_nontemporal_load((const rsd_v4 *)p);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+template <bool NT>
+__device__ __forceinline__ void st4(float4 *p, float4 x) {
+    if (!NT) {
+        *p = x;
+        return;
+    }
+    const rsd_v4 v = {x.x, x.y, x.z, x.w};
+    __builtin_nontemporal_store(v, (rsd_v4 *)p);
+}
+
 struct F16 { float s[16]; };
 struct F4 { float s[4]; };
 struct Tex3 { const float4 *d; int nx, ny, nz; };
@@ -987,7 +1006,9 @@ __device__ __forceinline__ void rain_drag(const RainK &a, float4 pos, float4 &ve
 __device__ __forceinline__ void logtab_load(float4 *dst, const float *src) { rf_log_rows(dst, src); }
 
 // Scatterers [lo, hi) of one rain population: advance and/or pulse terms into
-// the block's LDS gate sums acc (cleared by the caller).
+// the block's LDS gate sums acc (cleared by the caller).  The hi/lo variant
+// (96 B of state per scatterer, meant for populations far beyond the 256 MB
+// Infinity Cache) streams its state non-temporally.
 template <bool ADV, bool PULSE, bool HILO, bool WIN, bool TRI>
 __device__ __forceinline__ void rain_range_t(const RainK &a, size_t lo, size_t hi, double *acc, const float4 *logtab) {
     const float2 *wl = win_weights(acc, a.pk);
@@ -1002,8 +1023,8 @@ __device__ __forceinline__ void rain_range_t(const RainK &a, size_t lo, size_t h
     const float4 idle = make_float4(0.0f, 1.0f, 0.0f, 0.0f);
     float4 pn = idle, vn = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (lo + threadIdx.x < hi) {
-        pn = a.pos[lo + threadIdx.x];
-        if (ADV) vn = a.vel[lo + threadIdx.x];
+        pn = ld4<HILO>(&a.pos[lo + threadIdx.x]);
+        if (ADV) vn = ld4<HILO>(&a.vel[lo + threadIdx.x]);
     }
     for (size_t base = lo; base < hi; base += blockDim.x) {
         const size_t i = base + threadIdx.x;
@@ -1011,15 +1032,15 @@ __device__ __forceinline__ void rain_range_t(const RainK &a, size_t lo, size_t h
         float4 p = pn;
         const float4 v_in = vn;
         if (i + blockDim.x < hi) {
-            pn = a.pos[i + blockDim.x];
-            if (ADV) vn = a.vel[i + blockDim.x];
+            pn = ld4<HILO>(&a.pos[i + blockDim.x]);
+            if (ADV) vn = ld4<HILO>(&a.vel[i + blockDim.x]);
         }
         bool stale = __builtin_signbit(p.w);
         p.w = __builtin_fabsf(p.w);
         float4 rcs;
         bool have_rcs = false;
         float4 plo = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (HILO && live) plo = a.plo[i];
+        if (HILO && live) plo = ld4<HILO>(&a.plo[i]);
         // Advance: the new position first; the wind texels of that position
         // are read next and consumed only after the pulse terms (which need
         // the position, not the velocity), so the reads overlap that work.
@@ -1029,7 +1050,7 @@ __device__ __forceinline__ void rain_range_t(const RainK &a, size_t lo, size_t h
         if (ADV && live) {
             const float4 p_in = p;
             bool respawned = rain_move<HILO>(a, i, p, v, plo);
-            if (HILO) a.plo[i] = plo;
+            if (HILO) st4<HILO>(&a.plo[i], plo);
             if (respawned) {
                 if (!stale) {
                     rcs = ellipsoid_rcs(p_in, dropt, a.drop_desc);
@@ -1042,7 +1063,7 @@ __device__ __forceinline__ void rain_range_t(const RainK &a, size_t lo, size_t h
                 moved = true;
                 rain_gather(a, p, logtab, g);
             }
-            a.pos[i] = make_float4(p.x, p.y, p.z, stale ? -p.w : p.w);
+            st4<HILO>(&a.pos[i], make_float4(p.x, p.y, p.z, stale ? -p.w : p.w));
         }
         if (PULSE) {
             float4 sa = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -1057,7 +1078,7 @@ __device__ __forceinline__ void rain_range_t(const RainK &a, size_t lo, size_t h
         }
         if (ADV && live) {
             if (moved) rain_drag(a, p, v, lerp3d(g));
-            a.vel[i] = v;
+            st4<HILO>(&a.vel[i], v);
         }
     }
     if (WIN) wwin_flush(win, a.pk, acc);
