@@ -32,12 +32,13 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_rain_c2_traffic.json")
 REFERENCE_PULSES_PER_S = 222.3   # README.md:58 (instantaneous, 1,998,848 scatterers, GPU not stated)
 
 
-def cpu_baseline(r, cfg, pulses):
-    """The oracle (CPU restatement of the reference kernels, one thread) timed
-    on a bounded sample of the same workload."""
+def cpu_baseline(r, cfg, pulses, threads):
+    """The oracle (CPU restatement of the reference kernels: the same kernel
+    sequence, dense pass 1 over all gates) timed on a bounded sample of the same
+    workload, each population split over `threads` host threads."""
     from oracle import run as orun
     st = orun.state_from_radar(r)
-    o = orun.OracleRun(r, cfg, st)
+    o = orun.OracleRun(r, cfg, st, threads=threads)
     sd = r.sim_desc()
     t0 = time.perf_counter()
     for _ in range(pulses):
@@ -54,7 +55,9 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-pulses", type=int, default=6)
+    ap.add_argument("--cpu-pulses", type=int, default=24)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="host threads of the CPU baseline (0: the process's CPUs, at most 16)")
     ap.add_argument("--e2e-steps", type=int, default=200)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--max-blocks", type=int, default=0)
@@ -83,11 +86,14 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        print("bench: cpu baseline (%d pulses of the oracle)" % args.cpu_pulses, file=sys.stderr, flush=True)
-        cpu_pps = cpu_baseline(r, cfg, args.cpu_pulses)
-        cpu = {"value": cpu_pps, "unit": "pulses/s", "cores": 1, "kind": "port",
-               "sample": "%d pulses of the full %d-scatterer workload, oracle C restatement, one thread"
-                         % (args.cpu_pulses, cfg.n_total)}
+        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        print("bench: cpu baseline (%d pulses of the oracle on %d threads)" % (args.cpu_pulses, threads),
+              file=sys.stderr, flush=True)
+        cpu_pps = cpu_baseline(r, cfg, args.cpu_pulses, threads)
+        cpu = {"value": cpu_pps, "unit": "pulses/s", "cores": threads, "kind": "port",
+               "sample": "%d pulses of the full %d-scatterer workload, oracle C restatement (dense pass 1 over "
+                         "all gates), scatterers split over %d host threads" % (args.cpu_pulses, cfg.n_total,
+                                                                              threads)}
 
     def step():
         r.make_pulse()

@@ -2,6 +2,8 @@
 own tables (drop RCS, antenna pattern, range weight, LES descriptor) for the
 configuration of a Radar and advances / pulses the reference-layout streams.
 Used by tests/ and by bench.py's cpu_baseline leg only."""
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 
 from . import oracle as orc
@@ -42,9 +44,16 @@ def rcs_desc(na, nb):
 
 class OracleRun:
     """CPU oracle stepping the same run from a given state (driver order of
-    simradar.c:1011-1074: make_pulse, download, advance_time, advance_beam)."""
+    simradar.c:1011-1074: make_pulse, download, advance_time, advance_beam).
 
-    def __init__(self, r, cfg, state):
+    threads > 1 splits every population into that many contiguous ranges run
+    on concurrent host threads (the C calls release the interpreter lock): the
+    advance is unchanged bit for bit, the pulse sums the per-range gate sums in
+    range order (binary64 reassociation only).  Tests use one thread."""
+
+    def __init__(self, r, cfg, state, threads=1):
+        self.threads = max(1, int(threads))
+        self._pool = ThreadPoolExecutor(self.threads) if self.threads > 1 else None
         self.r = r
         self.cfg = cfg
         self.st = state
@@ -74,30 +83,47 @@ class OracleRun:
             tx = [orc.tex(a) for a in (cd, cm, re, im)]
             self.deb.append(dict(tex=tx, adm_desc=ad, rcs_desc=rd))
 
+    def _ranges(self, b, e):
+        n = self.threads
+        cuts = [b + (e - b) * j // n for j in range(n + 1)]
+        return [(cuts[j], cuts[j + 1]) for j in range(n) if cuts[j + 1] > cuts[j]]
+
+    def _run(self, fn, b, e):
+        rs = self._ranges(b, e)
+        if self._pool is None or len(rs) < 2:
+            return [fn(lo, hi) for lo, hi in rs]
+        return list(self._pool.map(lambda t: fn(*t), rs))
+
     def advance(self, sim_desc):
         n0 = self.counts[0]
-        if self.cfg.concept & CONCEPT_DRAGGED:
-            orc.el_atts(self.st, 0, n0, self.wind_tex, self.les_desc, self.drop, self.drop_desc, sim_desc)
-        else:
-            orc.bg_atts(self.st, 0, n0, self.wind_tex, self.les_desc, self.drop, self.drop_desc, sim_desc)
+        atts = orc.el_atts if self.cfg.concept & CONCEPT_DRAGGED else orc.bg_atts
+        self._run(lambda lo, hi: atts(self.st, lo, hi, self.wind_tex, self.les_desc, self.drop, self.drop_desc,
+                                      sim_desc), 0, n0)
         for t, d in enumerate(self.deb):
             k = t + 1
             b, e = self.origins[k], self.origins[k] + self.counts[k]
             (cd, _), (cm, _), (re, _), (im, _) = d["tex"]
-            orc.db_atts(self.st, b, e, self.wind_tex, self.les_desc, cd, cm, d["adm_desc"], re, im, d["rcs_desc"],
-                        sim_desc)
+            self._run(lambda lo, hi: orc.db_atts(self.st, lo, hi, self.wind_tex, self.les_desc, cd, cm,
+                                                 d["adm_desc"], re, im, d["rcs_desc"], sim_desc), b, e)
 
     def refresh_debris_rcs(self, sim_desc):
         for t, d in enumerate(self.deb):
             k = t + 1
             b, e = self.origins[k], self.origins[k] + self.counts[k]
             _, _, (re, _), (im, _) = d["tex"]
-            orc.db_rcs(self.st, b, e, re, im, d["rcs_desc"], sim_desc)
+            self._run(lambda lo, hi: orc.db_rcs(self.st, lo, hi, re, im, d["rcs_desc"], sim_desc), b, e)
 
     def pulse(self, sim_desc, mode=orc.PHASE_F64):
         p = self.params
-        return orc.make_pulse(self.st.pos, self.st.rcs, self.angw, self.angw_desc, self.rangew, self.rangew_desc,
-                              p.range_start, p.range_delta, int(p.range_count), sim_desc, self.kd, mode)
+        pos, rcs = self.st.pos, self.st.rcs
+        parts = self._run(lambda lo, hi: orc.make_pulse(pos[lo:hi], rcs[lo:hi], self.angw, self.angw_desc,
+                                                        self.rangew, self.rangew_desc, p.range_start,
+                                                        p.range_delta, int(p.range_count), sim_desc, self.kd,
+                                                        mode), 0, pos.shape[0])
+        out = parts[0]
+        for q in parts[1:]:
+            out = out + q
+        return out
 
 
 def state_from_radar(r):

@@ -176,6 +176,30 @@ def test_launch_shapes(gpu_ready, launch):
     r.free()
 
 
+def test_threaded_oracle_is_the_oracle(gpu_ready):
+    """The CPU baseline's threaded oracle: the advance is the single-thread
+    oracle's bit for bit, the pulse differs only by binary64 reassociation."""
+    cfg = h.Config(n_rain=6000, debris=[(sa.OBJ_LEAF, 300)], scan="P:3.0,-1:1:0.01")
+    r = sa.Radar(verb=0)
+    h.configure(r, cfg)
+    r.populate()
+    o1 = h.OracleRun(r, cfg, h.state_from_radar(r))
+    o4 = h.OracleRun(r, cfg, h.state_from_radar(r), threads=4)
+    for _ in range(3):
+        sd = r.sim_desc()
+        o1.advance(sd)
+        o4.advance(sd)
+        o1.refresh_debris_rcs(sd)
+        o4.refresh_debris_rcs(sd)
+        a, b = o1.pulse(sd), o4.pulse(sd)
+        assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(a))
+        r.advance_beam()
+    for s in ("pos", "vel", "ori", "tum", "rcs"):
+        assert np.array_equal(_bits(getattr(o1.st, s)), _bits(getattr(o4.st, s))), s
+    assert np.array_equal(o1.st.rnd, o4.st.rnd)
+    r.free()
+
+
 def test_unsorted_engine_parity(gpu_ready):
     """Sorting switched off: the stored order is the reference order throughout."""
     cfg = h.Config(n_rain=8192, scan="P:3.0,-1:1:0.01")
