@@ -151,6 +151,19 @@ def test_sorted_state_upload_roundtrip(gpu_ready):
     r.free()
 
 
+@pytest.mark.parametrize("reupload_at", [None, 8])
+def test_range_sort_interval(gpu_ready, reupload_at):
+    """Range reorders every 6 pulses over 20 pulses, with and without a
+    download / upload round trip between two reorders: state bit-exact, IQ
+    within tolerance."""
+    cfg = h.Config(n_rain=20000, debris=[(sa.OBJ_LEAF, 300)], scan="P:3.0,-1:1:0.01")
+    r, o, pg, po = _run(cfg, 20, sort=6, reupload_at=reupload_at)
+    _check_state(r, o)
+    errs = [h.rel_err(pg[k], po[k]) for k in range(len(pg))]
+    assert max(errs) < IQ_TOL, errs
+    r.free()
+
+
 @pytest.mark.parametrize("fold", [0, 1, 2, 3])
 @pytest.mark.parametrize("sort", [0, 1])
 def test_fold_strategies(gpu_ready, fold, sort):
