@@ -22,10 +22,10 @@
 // reference's.
 //
 // One pulse = one fused kernel (advance of the previous PRT, if pending, then
-// the signal of every scatterer folded into its gates; small debris populations
-// in leading blocks of the rain launch, large ones and surrogate-driven types in
-// kernels of their own on a second stream) plus a finalize kernel that sums the
-// per-block gate partials in binary64.
+// the signal of every scatterer folded into its gates; table-driven debris types
+// in leading blocks of the rain launch, surrogate-driven types in a kernel of
+// their own on a second stream) plus a finalize kernel that sums the per-block
+// gate partials in binary64.
 //
 // Spatial order: every sort_interval pulses the rain streams (pos, vel, rcs,
 // rnd) are reordered by range (shells of 1/8 gate) and, inside a shell, by
