@@ -1830,8 +1830,8 @@ struct rsd_engine {
     double gate_drift;      // max |gate_r[k] - (start + k delta)| of the binary32 sums
     double *partials;
     size_t partial_cap;
-    double *pulse_d;
-    double *pulse_h;
+    double *pulse_d;         // device address of pulse_h: the finalize writes the
+    double *pulse_h;         // pulse straight into pinned host memory
     int have_pending;
     rsd_step pending;
     int have_pulse;
@@ -1963,8 +1963,8 @@ extern "C" int rsd_open(int device_index, rsd_engine **out) {
         hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming) != hipSuccess ||
         hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess ||
-        hipHostMalloc((void **)&e->pulse_h, 4 * RSD_MAX_GATES * sizeof(double), hipHostMallocDefault) != hipSuccess ||
-        hipMalloc((void **)&e->pulse_d, 4 * RSD_MAX_GATES * sizeof(double)) != hipSuccess ||
+        hipHostMalloc((void **)&e->pulse_h, 4 * RSD_MAX_GATES * sizeof(double), hipHostMallocMapped) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&e->pulse_d, e->pulse_h, 0) != hipSuccess ||
         upload_logtab(&e->logtab) != hipSuccess) {
         free(e);
         return RSD_ERR_HIP;
@@ -2016,7 +2016,6 @@ extern "C" void rsd_close(rsd_engine *e) {
         dfree(e->adm_cd[t]); dfree(e->adm_cm[t]); dfree(e->rcs_re[t]); dfree(e->rcs_im[t]);
     }
     dfree(e->drop); dfree(e->rangew); dfree(e->angw); dfree(e->gate_r); dfree(e->partials);
-    dfree(e->pulse_d);
     dfree(e->logtab);
     dfree(e->ring);
     dfree(e->mom_d);
@@ -2918,8 +2917,7 @@ extern "C" int rsd_read_pulse(rsd_engine *e, double *out, uint32_t R) {
     if (!e->have_pulse) return fail(e, RSD_ERR_STATE, "no pulse made yet");
     if (R != e->last_pulse.range_count) return fail(e, RSD_ERR_ARG, "range_count %u != %u of the last pulse", R, e->last_pulse.range_count);
     HIPCHK(e, hipSetDevice(e->device));
-    HIPCHK(e, hipMemcpyAsync(e->pulse_h, e->pulse_d, (size_t)4 * R * sizeof(double), hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(e, hipStreamSynchronize(e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));   // the finalize wrote pulse_h itself
     memcpy(out, e->pulse_h, (size_t)4 * R * sizeof(double));
     return RSD_OK;
 }
