@@ -129,6 +129,18 @@ def test_debris_step_parity(gpu_ready):
     r.free()
 
 
+def test_three_debris_types_ride(gpu_ready):
+    """Rain + leaf, metal sheet and brick: all three table-driven types ride in
+    the rain launch; 6-DoF state bit-exact, IQ within tolerance."""
+    cfg = h.Config(n_rain=8192, debris=[(sa.OBJ_LEAF, 700), (sa.OBJ_METAL_SHEET, 500), (sa.OBJ_BRICK, 300)],
+                   scan="P:3.0,-1:1:0.01")
+    r, o, pg, po = _run(cfg, 6)
+    _check_state(r, o)
+    errs = [h.rel_err(pg[k], po[k]) for k in range(len(pg))]
+    assert max(errs) < IQ_TOL, errs
+    r.free()
+
+
 def test_rain_sorted_every_pulse(gpu_ready):
     """Spatial reorder before every pulse: streams still come back bit-exact in
     the reference order and the gate sums are unchanged."""
