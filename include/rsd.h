@@ -14,6 +14,7 @@
  *   rsd_set_adm                     RS_set_adm_data (rs.c:3642-3762)
  *   rsd_set_rcs                     RS_set_rcs_data (rs.c:3852-3972)
  *   rsd_set_drop_rcs                RS_set_rcs_ellipsoid_table (rs.c:2472-2545)
+ *   rsd_set_dff                     RS_set_debris_flux_field_by_icdf (rs.c:3331-3404)
  *   rsd_set_range_weight            RS_set_range_weight (rs.c:2548-2618)
  *   rsd_set_angular_weight          RS_set_angular_weight (rs.c:2626-2698)
  *   rsd_advance                     enqueue block of RS_advance_time (rs.c:5327-5388)
@@ -103,6 +104,11 @@ int rsd_set_rcs(rsd_engine *e, int table, const float *re, const float *im, int 
 int rsd_set_drop_rcs(rsd_engine *e, const float *table4, int n, const float *desc4);
 int rsd_set_range_weight(rsd_engine *e, const float *w, int n, const float *desc4);
 int rsd_set_angular_weight(rsd_engine *e, const float *w, int n, const float *desc4);
+/* Debris flux field (concept C): inverse CDF of `count` entries and its
+ * descriptor (RS_set_debris_flux_field_by_icdf, rs.c:3331-3404, read by
+ * db_atts at rs.cl:795-836); count 0 removes it.  Used on debris respawn
+ * while the step's concept has bit 7 set. */
+int rsd_set_dff(rsd_engine *e, const float *icdf, int count, const float *desc16);
 
 /* One PRT of every scatterer type.  May be held back and fused with the next
  * rsd_make_pulse; any transfer or rsd_sync performs it first. */

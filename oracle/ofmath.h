@@ -221,4 +221,25 @@ static inline float of_len3(float x, float y, float z) { return sqrtf((x * x + y
 static inline float of_len4(float x, float y, float z, float w) { return sqrtf(((x * x + y * y) + z * z) + w * w); }
 static inline float of_radiansf(float d) { return d * 0x1.1df46ap-6f; }
 
+/* exp(x): Cody-Waite reduction x = k ln2 + r (ln2 in a 16-bit head and a tail),
+ * |r| <= ln2 / 2, Taylor polynomial of e^r to r^8, scaled by 2^k. */
+static inline float of_expf(float x) {
+    const float k = rintf(x * 0x1.715476p+0f);
+    float r = fmaf(-k, OF_LN2_HI, x);
+    r = fmaf(-k, OF_LN2_LO, r);
+    float p = 1.0f / 40320.0f;
+    p = fmaf(p, r, 1.0f / 5040.0f);
+    p = fmaf(p, r, 1.0f / 720.0f);
+    p = fmaf(p, r, 1.0f / 120.0f);
+    p = fmaf(p, r, 1.0f / 24.0f);
+    p = fmaf(p, r, 1.0f / 6.0f);
+    p = fmaf(p, r, 0.5f);
+    p = fmaf(p, r, 1.0f);
+    p = fmaf(p, r, 1.0f);
+    return ldexpf(p, (int)k);
+}
+
+/* pow(a, y) for a > 0 as exp(y log a). */
+static inline float of_powf_pos(float a, float y) { return of_expf(y * of_logf(a)); }
+
 #endif

@@ -47,6 +47,8 @@ def lib():
             "orc_el_atts": (None, [P, P, P, P, S, S, T, P, P, P, P]),
             "orc_bg_atts": (None, [P, P, P, P, S, S, T, P, P, P, P]),
             "orc_db_atts": (None, [P, P, P, P, P, P, S, S, T, P, T, T, P, T, T, P, P]),
+            "orc_set_dff": (None, [P, ctypes.c_int, P]),
+            "orc_dff_icdf_from_pdf": (ctypes.c_int, [P, ctypes.c_int, P, ctypes.c_int]),
             "orc_db_rcs": (None, [P, P, P, S, S, T, T, P, P]),
             "orc_scat_sig_aux": (None, [P, P, P, P, S, P, P, P]),
             "orc_gate_ranges": (None, [F, F, I, P]),
@@ -184,6 +186,22 @@ def db_atts(st, begin, end, wind_tex, les_desc, cd_tex, cm_tex, adm_desc, re_tex
                       ctypes.byref(wind_tex), _p(f32(les_desc)), ctypes.byref(cd_tex), ctypes.byref(cm_tex),
                       _p(f32(adm_desc)), ctypes.byref(re_tex), ctypes.byref(im_tex), _p(f32(rcs_desc)),
                       _p(f32(sim_desc)))
+
+
+def set_dff(icdf, desc16):
+    """Debris flux field of later db_atts calls (rs.cl:795-836); None clears it."""
+    if icdf is None:
+        lib().orc_set_dff(None, 0, None)
+    else:
+        icdf = f32(icdf)
+        lib().orc_set_dff(_p(icdf), icdf.size, _p(f32(desc16)))
+
+
+def dff_icdf_from_pdf(pdf, n=2048):
+    """Inverse-CDF table from a cell PDF (rs.c:3247-3320); None on the error returns."""
+    pdf = f32(pdf)
+    tab = np.zeros(n, np.float32)
+    return tab if lib().orc_dff_icdf_from_pdf(_p(pdf), pdf.size, _p(tab), n) == 0 else None
 
 
 def db_rcs(st, begin, end, re_tex, im_tex, rcs_desc, sim_desc):

@@ -296,6 +296,88 @@ void orc_bg_atts(of4 *p, of4 *v, of4 *x, ou4 *y, size_t begin, size_t end,
     }
 }
 
+/* Debris flux field (concept C): a copy of the inverse CDF and its
+ * descriptor (rs.c:3392-3404 layout); count 0 switches the placement off. */
+static float *orc_dff_tab = NULL;
+static float orc_dff_desc[16];
+
+void orc_set_dff(const float *icdf, int count, const float *desc16) {
+    free(orc_dff_tab);
+    orc_dff_tab = NULL;
+    memset(orc_dff_desc, 0, sizeof(orc_dff_desc));
+    if (count <= 0) return;
+    orc_dff_tab = (float *)malloc((size_t)count * sizeof(float));
+    memcpy(orc_dff_tab, icdf, (size_t)count * sizeof(float));
+    memcpy(orc_dff_desc, desc16, sizeof(orc_dff_desc));
+}
+
+/* Inverse CDF from a cell PDF, rs.c:3247-3320 (CDF summed in binary32, kept in
+ * binary64; a run of empty cells is stepped back over).  0 or -1 as the
+ * reference's two error returns. */
+int orc_dff_icdf_from_pdf(const float *pdf, int pdf_count, float *tab, int n) {
+    const int cdf_count = pdf_count + 1;
+    double *cdf = (double *)malloc((size_t)cdf_count * sizeof(double));
+    float cumsum = 0.0f;
+    int k;
+    for (k = 0; k < pdf_count; k++) {
+        cdf[k] = cumsum;
+        cumsum += pdf[k];
+        if (cumsum > 1.0f) {
+            if (k < pdf_count - 2) { free(cdf); return -1; }
+            if (k < pdf_count - 1) cumsum = 1.0f;
+        }
+    }
+    cdf[pdf_count] = 1.0;
+    for (k = 0; k < n; k++) {
+        float x = (float)k / (float)(n - 1);
+        int b = 0, e;
+        while (b < cdf_count && cdf[b] <= x) b++;
+        b = b > 0 ? b - 1 : 0;
+        e = b + 1 < pdf_count ? b + 1 : pdf_count;
+        if (cdf[b] == cdf[e]) {
+            while (b > 0 && cdf[b] >= x) b--;
+            e = b + 1 < pdf_count ? b + 1 : pdf_count;
+        }
+        if (b == e) { tab[k] = (float)b; continue; }
+        if (!(cdf[b] <= x && x <= cdf[e])) { free(cdf); return -1; }
+        float a = (float)((x - cdf[b]) / (cdf[e] - cdf[b]));
+        tab[k] = (float)b + a * ((float)e - (float)b);
+    }
+    free(cdf);
+    return 0;
+}
+
+/* Respawn placement under the flux field, rs.cl:795-836. */
+static void orc_dff_place(of4 r, const float *sd, of4 *pos) {
+    const float *dd = orc_dff_desc;
+    const float sf = dd[15];
+    float i0, i1;
+    const float f0 = of_clampf(fmaf(r.x, sf, 0.0f), 0.0f, sf);
+    const float f1 = of_clampf(fmaf(r.x, sf, 1.0f), 0.0f, sf);
+    const float dec = of_fractf(f0, &i0);
+    of_fractf(f1, &i1);
+    const float c = of_mixf(orc_dff_tab[(uint32_t)i0], orc_dff_tab[(uint32_t)i1], dec);
+    float y = floorf(c / dd[13]);
+    float x = fmaf(y, -dd[13], c);
+    y = y + r.y;
+    if (dd[14] == 1.0f) {
+        const float rx = x - fmaf(0.5f, dd[8], 0.5f), ry = y - fmaf(0.5f, dd[9], 0.5f);
+        x = fmaf(of_powf_pos(dd[4], fabsf(rx)), -dd[0], dd[0]);
+        y = fmaf(of_powf_pos(dd[5], fabsf(ry)), -dd[1], dd[1]);
+        x = copysignf(x, rx) + fmaf(0.5f, sd[12], sd[8]);
+        y = copysignf(y, ry) + fmaf(0.5f, sd[13], sd[9]);
+    } else if (dd[14] == 2.0f) {
+        x = fmaf(x, sd[12] * dd[0], sd[8]);
+        y = fmaf(y, sd[13] * dd[1], sd[9]);
+    } else {
+        x = fmaf(x, dd[0], sd[8]);
+        y = fmaf(y, dd[1], sd[9]);
+    }
+    pos->x = x;
+    pos->y = y;
+    pos->z = MIN_HEIGHT;
+}
+
 void orc_db_atts(of4 *p, of4 *o, of4 *v, of4 *t, of4 *x, ou4 *y, size_t begin, size_t end,
                  const orc_tex *wind, const float *les_desc,
                  const orc_tex *adm_cd, const orc_tex *adm_cm, const float *ad,
@@ -311,13 +393,17 @@ void orc_db_atts(of4 *p, of4 *o, of4 *v, of4 *t, of4 *x, ou4 *y, size_t begin, s
         pos.z = pos.z + vel.z * dt;
         pos.w = pos.w + vel.w * 0.0f;
         if (is_outside(pos, sd)) {
-            /* rs.cl:790-873 (uniform placement branch) */
+            /* rs.cl:790-873: flux-field or uniform placement */
             ou4 seed = y[i];
             of4 r;
             orc_rand4(&seed, &r);
-            pos.x = fmaf(r.x, sd[12], sd[8]);
-            pos.y = fmaf(r.y, sd[13], sd[9]);
-            pos.z = sd[10] + MIN_HEIGHT;
+            if ((concept & 128u) && orc_dff_tab) {
+                orc_dff_place(r, sd, &pos);
+            } else {
+                pos.x = fmaf(r.x, sd[12], sd[8]);
+                pos.y = fmaf(r.y, sd[13], sd[9]);
+                pos.z = sd[10] + MIN_HEIGHT;
+            }
             orc_rand4(&seed, &r);
             float c0 = sqrtf(-2.0f * of_logf(r.x));
             float c1 = sqrtf(-2.0f * of_logf(r.y));

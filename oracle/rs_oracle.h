@@ -58,6 +58,8 @@ void orc_bg_atts(of4 *p, of4 *v, of4 *x, ou4 *y, size_t begin, size_t end,
                  const orc_tex *wind, const float *les_desc,
                  const of4 *drop_rcs, const float *drop_desc4, const float *sim_desc);
 /* rs.cl:744-907 -- db_atts over [begin, end) (debris flux field concept not covered). */
+void orc_set_dff(const float *icdf, int count, const float *desc16);
+int orc_dff_icdf_from_pdf(const float *pdf, int pdf_count, float *tab, int n);
 void orc_db_atts(of4 *p, of4 *o, of4 *v, of4 *t, of4 *x, ou4 *y, size_t begin, size_t end,
                  const orc_tex *wind, const float *les_desc,
                  const orc_tex *adm_cd, const orc_tex *adm_cm, const float *adm_desc,

@@ -77,7 +77,7 @@ EXPORTED_RS = [
 ]
 EXPORTED_RSD = [
     "rsd_device_count", "rsd_device_props", "rsd_open", "rsd_close", "rsd_last_error", "rsd_alloc_state",
-    "rsd_upload_state", "rsd_download_state", "rsd_set_wind", "rsd_set_adm", "rsd_set_rcs", "rsd_set_drop_rcs",
+    "rsd_upload_state", "rsd_download_state", "rsd_set_wind", "rsd_set_adm", "rsd_set_rcs", "rsd_set_drop_rcs", "rsd_set_dff",
     "rsd_set_range_weight", "rsd_set_angular_weight", "rsd_advance", "rsd_make_pulse", "rsd_read_pulse",
     "rsd_sync", "rsd_timer_start", "rsd_timer_stop", "rsd_set_launch", "rsd_math_probe", "rsd_profile",
     "rsd_profile_read", "rsd_io_test", "rsd_set_sort", "rsd_set_fold", "rsd_set_iq_ring", "rsd_moments",
@@ -162,6 +162,12 @@ def _declare(lib):
         "RS_free": (None, [P]),
         "RS_make_pulse_params": (RSMakePulseParams, [U, U, U, U, F, F, U]),
         "RS_set_concept": (None, [P, U]),
+        "RS_set_debris_flux_field_by_pdf": (None, [P, P, P]),
+        "RS_set_debris_flux_field_by_icdf": (None, [P, P, P]),
+        "RS_set_debris_flux_field_to_center_cell_of_3x3": (None, [P]),
+        "RS_set_debris_flux_field_to_checker_board": (None, [P, I]),
+        "RS_set_debris_flux_field_to_checker_board_stretched": (None, [P, P]),
+        "RS_set_debris_flux_field_from_LES": (None, [P, P]),
         "RS_simulation_concept_string": (ctypes.c_char_p, [P]),
         "RS_set_prt": (None, [P, F]),
         "RS_set_lambda": (None, [P, F]),
@@ -230,6 +236,7 @@ def _declare(lib):
         "rsd_set_adm": (I, [P, I, P, P, I, I, P]),
         "rsd_set_rcs": (I, [P, I, P, P, I, I, P]),
         "rsd_set_drop_rcs": (I, [P, P, I, P]),
+        "rsd_set_dff": (I, [P, P, I, P]),
         "rsd_set_range_weight": (I, [P, P, I, P]),
         "rsd_set_angular_weight": (I, [P, P, I, P]),
         "rsd_advance": (I, [P, ctypes.POINTER(RSDStep)]),

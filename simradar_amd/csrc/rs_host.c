@@ -826,17 +826,216 @@ void RS_set_vel_data_to_cube125(RSHandle *H) {
 
 void RS_clear_vel_data(RSHandle *H) { (void)H; }
 
-/* Debris flux field (concept C, rs.c:3247-3626, rs.cl:795-836) is not part of
- * this engine: debris respawn uniformly over the box floor. */
-static void dff_unsupported(void) {
-    rsprint("WARNING: Debris flux field (concept C) is not part of this engine; ignored.");
+/* ------------------------------------------ Debris flux field (concept C) */
+
+/* Inverse-CDF table of n entries from a cell PDF (rs.c:3247-3320): the CDF
+ * accumulates the PDF in binary32 and is kept in binary64; entry k is the
+ * fractional cell index at which the CDF reaches k / (n - 1), linear inside the
+ * cell, and a run of empty cells is stepped back over to the last cell that
+ * holds flux.  Returns 0, or -1 for a PDF whose sum passes 1 before its last
+ * two cells or a lookup that leaves its bracket (no table is produced). */
+int rs_dff_icdf_from_pdf(const float *pdf, int pdf_count, float *tab, int n) {
+    if (!pdf || !tab || pdf_count < 1 || n < 2) return -1;
+    const int cdf_count = pdf_count + 1;
+    double *cdf = (double *)malloc((size_t)cdf_count * sizeof(double));
+    if (!cdf) return -1;
+    float cumsum = 0.0f;
+    for (int k = 0; k < pdf_count; k++) {
+        cdf[k] = cumsum;
+        cumsum += pdf[k];
+        if (cumsum > 1.0f) {
+            if (k < pdf_count - 2) {
+                rsprint("ERROR: Bad PDF, cumulative sum %.8f > 1 at cell %d of %d.", cumsum, k, pdf_count);
+                free(cdf);
+                return -1;
+            }
+            if (k < pdf_count - 1) {
+                rsprint("WARNING: PDF cumulative sum %.8f at cell %d of %d clamped to 1.", cumsum, k, pdf_count);
+                cumsum = 1.0f;
+            }
+        }
+    }
+    cdf[pdf_count] = 1.0;
+    for (int k = 0; k < n; k++) {
+        const float x = (float)k / (float)(n - 1);
+        int lo = 0;
+        while (lo < cdf_count && cdf[lo] <= x) lo++;
+        lo = lo > 0 ? lo - 1 : 0;
+        int hi = lo + 1 < pdf_count ? lo + 1 : pdf_count;
+        if (cdf[lo] == cdf[hi]) {
+            while (lo > 0 && cdf[lo] >= x) lo--;
+            hi = lo + 1 < pdf_count ? lo + 1 : pdf_count;
+        }
+        const float vl = (float)lo, vh = (float)hi;
+        if (lo == hi) {
+            tab[k] = vl;
+            continue;
+        }
+        if (!(cdf[lo] <= x && x <= cdf[hi])) {
+            rsprint("ERROR: Inverse CDF lookup left its bracket (%d, %d) at %.4f.", lo, hi, x);
+            free(cdf);
+            return -1;
+        }
+        const float w = (float)((x - cdf[lo]) / (cdf[hi] - cdf[lo]));
+        tab[k] = vl + w * (vh - vl);
+    }
+    free(cdf);
+    return 0;
 }
-void RS_set_debris_flux_field_by_pdf(RSHandle *H, RSTable2D *map, const float *pdf) { (void)H; (void)map; (void)pdf; dff_unsupported(); }
-void RS_set_debris_flux_field_by_icdf(RSHandle *H, RSTable2D *map, const float *icdf) { (void)H; (void)map; (void)icdf; dff_unsupported(); }
-void RS_set_debris_flux_field_to_center_cell_of_3x3(RSHandle *H) { (void)H; dff_unsupported(); }
-void RS_set_debris_flux_field_to_checker_board(RSHandle *H, const int n) { (void)H; (void)n; dff_unsupported(); }
-void RS_set_debris_flux_field_to_checker_board_stretched(RSHandle *H, const LESTable *t) { (void)H; (void)t; dff_unsupported(); }
-void RS_set_debris_flux_field_from_LES(RSHandle *H, const LESTable *t) { (void)H; (void)t; dff_unsupported(); }
+
+void RS_set_debris_flux_field_by_pdf(RSHandle *H, RSTable2D *map, const float *pdf) {
+    const int n = 2048;     /* inverse-CDF entries (rs.c:3250) */
+    float *tab = (float *)malloc((size_t)n * sizeof(float));
+    if (!tab) return;
+    if (rs_dff_icdf_from_pdf(pdf, (int)map->x_, tab, n) == 0) {
+        map->x_ = (uint32_t)n;
+        RS_set_debris_flux_field_by_icdf(H, map, tab);
+    }
+    free(tab);
+}
+
+/* rs.c:3331-3404 -- descriptor: scales 0-1, stretch ratios or origins 4-5,
+ * last cell indices 8-9, row length 13, grid kind 14 (0 uniform, 1 stretched,
+ * 2 box-filling test board), last table index 15. */
+void RS_set_debris_flux_field_by_icdf(RSHandle *H, RSTable2D *map, const float *icdf) {
+    RSWorker *C = &H->workers[0];
+    cl_float16 d;
+    memset(&d, 0, sizeof(d));
+    d.s[0] = map->xs;
+    d.s[1] = map->ys;
+    d.s[2] = -1.0f;
+    d.s[4] = map->xo;
+    d.s[5] = map->yo;
+    d.s[6] = -1.0f;
+    d.s[8] = map->xm;
+    d.s[9] = map->ym;
+    d.s[10] = -1.0f;
+    d.s[13] = map->xm + 1.0f;
+    d.s[14] = (float)map->y_;
+    d.s[15] = (float)map->x_ - 1.0f;
+    C->dff_desc = d;
+    if (has_engine(H)) engine_ok(H, rsd_set_dff(C->engine, icdf, (int)map->x_, d.s), "debris flux field upload");
+}
+
+/* rs.c:3431-3459: all flux in the middle cell (4) of a 3 x 3 field. */
+void RS_set_debris_flux_field_to_center_cell_of_3x3(RSHandle *H) {
+    const float icdf[2] = {4.0f, 5.0f};
+    RSTable2D map;
+    memset(&map, 0, sizeof(map));
+    map.x_ = 2;
+    map.xs = 1.0f / 3.0f;
+    map.xo = -1.0f;
+    map.xm = 2.0f;
+    map.ys = 1.0f / 3.0f;
+    map.yo = -1.0f;
+    map.ym = 2.0f;
+    RS_set_debris_flux_field_by_icdf(H, &map, icdf);
+}
+
+/* rs.c:3462-3504: c x c board, flux in the even cells, stretched over the box. */
+void RS_set_debris_flux_field_to_checker_board(RSHandle *H, const int c) {
+    const int count = c * c;
+    if (c < 1) return;
+    float *pdf = (float *)calloc((size_t)count, sizeof(float));
+    if (!pdf) return;
+    for (int k = 0; k < count; k += 2) pdf[k] = 100.0f;
+    float sum = 0.0f;
+    for (int k = 0; k < count; k++) sum += pdf[k];
+    for (int k = 0; k < count; k++) pdf[k] /= sum;
+    RSTable2D map;
+    memset(&map, 0, sizeof(map));
+    map.x_ = (uint32_t)count;
+    map.y_ = 2;
+    map.xs = 1.0f / (float)c;
+    map.ys = 1.0f / (float)c;
+    map.xo = -1.0f;
+    map.yo = -1.0f;
+    map.xm = (float)(c - 1);
+    map.ym = (float)(c - 1);
+    RS_set_debris_flux_field_by_pdf(H, &map, pdf);
+    free(pdf);
+}
+
+/* Grid mapping shared by the LES-derived fields (rs.c:3529-3555, 3600-3623):
+ * stretched grids pass a / (1 - r) and the ratio (times q), uniform grids the
+ * spacing and the half extent. */
+static void dff_les_map(RSTable2D *map, const LESTable *l, float q, int cx, int cy) {
+    if (l->is_stretched) {
+        map->xs = l->ax / (1.0f - l->rx);
+        map->ys = l->ay / (1.0f - l->ry);
+        map->xo = q * l->rx;
+        map->yo = q * l->ry;
+    } else {
+        map->xs = l->rx;
+        map->ys = l->ry;
+        map->xo = (float)(l->nx - 1) * 0.5f * l->rx;
+        map->yo = (float)(l->ny - 1) * 0.5f * l->ry;
+    }
+    map->xm = (float)(cx - 1);
+    map->ym = (float)(cy - 1);
+}
+
+/* rs.c:3507-3562: 9 x 9 board whose even cells carry the area of the stretched
+ * LES cells at that distance from the centre (ratio scaled by q = 1.46). */
+void RS_set_debris_flux_field_to_checker_board_stretched(RSHandle *H, const LESTable *l) {
+    const int c = 9;
+    const float q = 1.46f;
+    const int count = c * c;
+    const float m = 0.5f * (float)(c - 1);
+    float pdf[81];
+    memset(pdf, 0, sizeof(pdf));
+    for (int k = 0; k < count; k += 2) {
+        if (l->is_stretched) {
+            const int iy = k / c, ix = k % c;
+            const float dx = l->ax * powf(q * l->rx, fabsf((float)ix - m));
+            const float dy = l->ay * powf(q * l->ry, fabsf((float)iy - m));
+            pdf[k] = dx * dy;
+        } else {
+            pdf[k] = 1.0f;
+        }
+    }
+    float sum = 0.0f;
+    for (int k = 0; k < count; k++) sum += pdf[k];
+    for (int k = 0; k < count; k++) pdf[k] /= sum;
+    RSTable2D map;
+    memset(&map, 0, sizeof(map));
+    map.x_ = (uint32_t)count;
+    map.y_ = 1;
+    dff_les_map(&map, l, q, c, c);
+    RS_set_debris_flux_field_by_pdf(H, &map, pdf);
+}
+
+/* rs.c:3565-3626: flux of the second LES level, |u|^2 per cell times the cell
+ * area on a stretched grid, normalised to a PDF (also left in l->flux when the
+ * table has that buffer). */
+void RS_set_debris_flux_field_from_LES(RSHandle *H, const LESTable *l) {
+    const int count = (int)(l->nx * l->ny);
+    const float mx = 0.5f * (float)(l->nx - 1), my = 0.5f * (float)(l->ny - 1);
+    if (count < 1 || !l->data.u || !l->data.v || !l->data.w) return;
+    float *flux = l->flux ? l->flux : (float *)malloc((size_t)count * sizeof(float));
+    if (!flux) return;
+    float sum = 0.0f;
+    for (int k = 0; k < count; k++) {
+        const int i = count + k;
+        float v = l->data.u[i] * l->data.u[i] + l->data.v[i] * l->data.v[i] + l->data.w[i] * l->data.w[i];
+        if (l->is_stretched) {
+            const int iy = k / (int)l->nx, ix = k % (int)l->nx;
+            const float dx = l->ax * powf(l->rx, fabsf((float)ix - mx));
+            const float dy = l->ay * powf(l->ry, fabsf((float)iy - my));
+            v *= dx * dy;
+        }
+        flux[k] = v;
+        sum += v;
+    }
+    for (int k = 0; k < count; k++) flux[k] /= sum;
+    RSTable2D map;
+    memset(&map, 0, sizeof(map));
+    map.x_ = (uint32_t)count;
+    map.y_ = l->is_stretched ? 1u : 0u;
+    dff_les_map(&map, l, 1.0f, (int)l->nx, (int)l->ny);
+    RS_set_debris_flux_field_by_pdf(H, &map, flux);
+    if (flux != l->flux) free(flux);
+}
 
 void RS_set_scan_pattern(RSHandle *H, const POSPattern *scan_pattern) {
     /* rs.c:3628-3633: the pattern is borrowed */

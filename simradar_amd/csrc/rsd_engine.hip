@@ -1183,6 +1183,8 @@ struct DebrisK {
     F16 rcs_desc;
     Tex2 prcs_re, prcs_im;  // table of the pulse's refresh
     F16 prcs_desc;
+    const float *dff;       // inverse CDF of the debris flux field, or null
+    F16 dff_desc;
     uint32_t refresh;
     uint32_t pulse_concept;
     uint32_t reuse_rcs;     // advance and refresh use the same table and concept
@@ -1190,6 +1192,44 @@ struct DebrisK {
     MlpDev net;             // surrogate network of this type (k_debris<.., true>)
     PulseK pk;
 };
+
+// Respawn position under the debris flux field (concept C, rs.cl:795-836).
+// r.x goes through the inverse CDF of the cell flux (dd s15 = entries - 1) to
+// a fractional cell index c; the row is floor(c / row length) (dd s13), the
+// column the rest of c, and r.y spreads the body over the row.  The cell is
+// then placed on the stretched grid (dd s14 = 1: size a (1 - q^|i|) / (1 - q)
+// from the centre, dd s0-1 = a / (1 - q), s4-5 = q, s8-9 = last cell index),
+// stretched over the box (dd s14 = 2: dd s0-1 = 1 / cells) or on the uniform
+// grid (dd s0-1 = spacing); the height is MIN_HEIGHT above 0.
+__device__ __forceinline__ void dff_place(const float *icdf, const F16 &dd, const F16 &sd, float4 r, float4 &pos) {
+    const float sf = dd.s[15];
+    float i0, i1;
+    const float f0 = rf_clamp(rf_fma(r.x, sf, 0.0f), 0.0f, sf);
+    const float f1 = rf_clamp(rf_fma(r.x, sf, 1.0f), 0.0f, sf);
+    const float dec = rf_fract(f0, &i0);
+    rf_fract(f1, &i1);
+    const float c = rf_mix(icdf[(uint32_t)i0], icdf[(uint32_t)i1], dec);
+    float y = __builtin_floorf(rf_div(c, dd.s[13]));
+    float x = rf_fma(y, -dd.s[13], c);
+    y = y + r.y;
+    if (dd.s[14] == 1.0f) {
+        const float rx = x - rf_fma(0.5f, dd.s[8], 0.5f);
+        const float ry = y - rf_fma(0.5f, dd.s[9], 0.5f);
+        x = rf_fma(rf_powf_pos(dd.s[4], __builtin_fabsf(rx)), -dd.s[0], dd.s[0]);
+        y = rf_fma(rf_powf_pos(dd.s[5], __builtin_fabsf(ry)), -dd.s[1], dd.s[1]);
+        x = __builtin_copysignf(x, rx) + rf_fma(0.5f, sd.s[12], sd.s[8]);
+        y = __builtin_copysignf(y, ry) + rf_fma(0.5f, sd.s[13], sd.s[9]);
+    } else if (dd.s[14] == 2.0f) {
+        x = rf_fma(x, sd.s[12] * dd.s[0], sd.s[8]);
+        y = rf_fma(y, sd.s[13] * dd.s[1], sd.s[9]);
+    } else {
+        x = rf_fma(x, dd.s[0], sd.s[8]);
+        y = rf_fma(y, dd.s[1], sd.s[9]);
+    }
+    pos.x = x;
+    pos.y = y;
+    pos.z = RSD_MIN_HEIGHT;
+}
 
 // Position / orientation update and respawn of db_atts (rs.cl:775-873);
 // returns true on respawn (state reset, rcs 0).
@@ -1205,9 +1245,13 @@ __device__ __forceinline__ bool debris_move(const DebrisK &a, size_t i, float4 &
     if (is_outside(pos, sd)) {
         uint4 seed = a.rnd[i];
         float4 r = rand4(seed);
-        pos.x = rf_fma(r.x, sd.s[12], sd.s[8]);
-        pos.y = rf_fma(r.y, sd.s[13], sd.s[9]);
-        pos.z = sd.s[10] + RSD_MIN_HEIGHT;
+        if ((__float_as_uint(sd.s[5]) & 128u) && a.dff) {
+            dff_place(a.dff, a.dff_desc, sd, r, pos);
+        } else {
+            pos.x = rf_fma(r.x, sd.s[12], sd.s[8]);
+            pos.y = rf_fma(r.y, sd.s[13], sd.s[9]);
+            pos.z = sd.s[10] + RSD_MIN_HEIGHT;
+        }
         r = rand4(seed);
         float c0 = rf_sqrt(-2.0f * rf_logf(r.x));
         float c1 = rf_sqrt(-2.0f * rf_logf(r.y));
@@ -1819,6 +1863,8 @@ struct rsd_engine {
     F16 rcs_desc[RSD_MAX_TABLES];
     float4 *drop;
     F4 drop_desc;
+    float *dff;             // debris flux field: inverse CDF and descriptor
+    F16 dff_desc;
     float *rangew;
     float rangew_h[1024];
     int rangew_n;
@@ -2017,7 +2063,7 @@ extern "C" void rsd_close(rsd_engine *e) {
     for (int t = 0; t < RSD_MAX_TABLES; t++) {
         dfree(e->adm_cd[t]); dfree(e->adm_cm[t]); dfree(e->rcs_re[t]); dfree(e->rcs_im[t]);
     }
-    dfree(e->drop); dfree(e->rangew); dfree(e->angw); dfree(e->gate_r); dfree(e->partials);
+    dfree(e->drop); dfree(e->dff); dfree(e->rangew); dfree(e->angw); dfree(e->gate_r); dfree(e->partials);
     dfree(e->logtab);
     dfree(e->ring);
     dfree(e->mom_d);
@@ -2254,6 +2300,8 @@ static int fill_debris(rsd_engine *e, const rsd_step *s, const rsd_pulse *p, int
     a.begin = e->origins[type];
     a.end = e->origins[type] + e->counts[type];
     a.logtab = e->logtab;
+    a.dff = e->dff;
+    a.dff_desc = e->dff_desc;
     if (s) {
         int slot = s->wind_slot, ai = s->adm_index[type], ri = s->rcs_index[type];
         if (!e->wind_ok[slot]) return fail(e, RSD_ERR_STATE, "wind slot %d not set", slot);
@@ -2833,6 +2881,26 @@ extern "C" int rsd_set_drop_rcs(rsd_engine *e, const float *table, int n, const 
     st = upload_tex(e, &e->drop, table, (size_t)n);
     if (st) return st;
     memcpy(e->drop_desc.s, desc4, sizeof(e->drop_desc.s));
+    return RSD_OK;
+}
+
+extern "C" int rsd_set_dff(rsd_engine *e, const float *icdf, int count, const float *desc16) {
+    if (!e || count < 0 || (count > 0 && (!icdf || !desc16))) return fail(e, RSD_ERR_ARG, "bad debris flux field");
+    if (count > 0 && !(desc16[15] >= 0.0f && desc16[15] <= (float)(count - 1)))
+        return fail(e, RSD_ERR_ARG, "flux field max index %g beyond %d entries", desc16[15], count);
+    if (count > 0 && !(desc16[13] > 0.0f)) return fail(e, RSD_ERR_ARG, "flux field row length %g", desc16[13]);
+    HIPCHK(e, hipSetDevice(e->device));
+    int st = flush_pending(e);
+    if (st) return st;
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    dfree(e->dff);
+    e->dff = NULL;
+    memset(&e->dff_desc, 0, sizeof(e->dff_desc));
+    if (count == 0) return RSD_OK;
+    HIPCHK(e, hipMalloc((void **)&e->dff, (size_t)count * sizeof(float)));
+    HIPCHK(e, hipMemcpyAsync(e->dff, icdf, (size_t)count * sizeof(float), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    memcpy(e->dff_desc.s, desc16, sizeof(e->dff_desc.s));
     return RSD_OK;
 }
 

@@ -274,3 +274,25 @@ __device__ __forceinline__ float rf_len2(float x, float y) { return rf_sqrt(x * 
 __device__ __forceinline__ float rf_len3(float x, float y, float z) { return rf_sqrt((x * x + y * y) + z * z); }
 __device__ __forceinline__ float rf_len4(float x, float y, float z, float w) { return rf_sqrt(((x * x + y * y) + z * z) + w * w); }
 __device__ __forceinline__ float rf_radians(float d) { return d * 0x1.1df46ap-6f; }
+
+// exp(x): Cody-Waite reduction x = k ln2 + r (ln2 in a 16-bit head and a tail,
+// so k * head is exact for |k| < 256), |r| <= ln2 / 2, Taylor polynomial of e^r
+// to r^8 (truncation below 3e-10 relative), scaled by 2^k.
+__device__ __forceinline__ float rf_expf(float x) {
+    const float k = __builtin_rintf(x * 0x1.715476p+0f);
+    float r = __builtin_fmaf(-k, 0x1.62e400p-1f, x);
+    r = __builtin_fmaf(-k, 0x1.7f7d1cp-20f, r);
+    float p = 1.0f / 40320.0f;
+    p = __builtin_fmaf(p, r, 1.0f / 5040.0f);
+    p = __builtin_fmaf(p, r, 1.0f / 720.0f);
+    p = __builtin_fmaf(p, r, 1.0f / 120.0f);
+    p = __builtin_fmaf(p, r, 1.0f / 24.0f);
+    p = __builtin_fmaf(p, r, 1.0f / 6.0f);
+    p = __builtin_fmaf(p, r, 0.5f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    return __builtin_ldexpf(p, (int)k);
+}
+
+// pow(a, y) for a > 0 as exp(y log a); relative error about |y log a| 2^-24.
+__device__ __forceinline__ float rf_powf_pos(float a, float y) { return rf_expf(y * rf_logf(a)); }

@@ -48,9 +48,11 @@ def test_device_atan2_div_match(gpu_ready):
     assert np.array_equal(_bits(sa.math_probe(7, y, x)), _bits(y / x))
 
 
-def _run(cfg, pulses, sort=None, reupload_at=None, fold=None, launch=None):
+def _run(cfg, pulses, sort=None, reupload_at=None, fold=None, launch=None, setup=None, pre=None):
     r = sa.Radar(verb=0)
     h.configure(r, cfg)
+    if pre is not None:
+        pre(r)
     r.populate()
     r.check()
     if sort is not None:
@@ -65,6 +67,8 @@ def _run(cfg, pulses, sort=None, reupload_at=None, fold=None, launch=None):
     sd_adv[11] = 0.0
     o.advance(sd_adv)
     o.refresh_debris_rcs(r.sim_desc())
+    if setup is not None:
+        setup(r, o)
     pg, po = [], []
     for k in range(pulses):
         if k == reupload_at:
@@ -139,6 +143,96 @@ def test_three_debris_types_ride(gpu_ready):
     errs = [h.rel_err(pg[k], po[k]) for k in range(len(pg))]
     assert max(errs) < IQ_TOL, errs
     r.free()
+
+
+def _lift_debris(r, o):
+    """Move every other debris body above the box (on the GPU and in the oracle)
+    so that the next step respawns it."""
+    r.download()
+    r.check()
+    n0 = o.counts[0]
+    pos = r.stream("pos", copy=False)
+    pos[n0::2, 2] = 1.0e5
+    o.st.pos[n0::2, 2] = 1.0e5
+    r.upload()
+    r.check()
+    return n0
+
+
+def _dff_desc(xs, xo, xm, kind, count):
+    """Descriptor layout of rs.c:3392-3404."""
+    d = np.zeros(16, np.float32)
+    d[0] = d[1] = xs
+    d[4] = d[5] = xo
+    d[2] = d[6] = d[10] = -1.0
+    d[8] = d[9] = xm
+    d[13] = np.float32(xm) + np.float32(1.0)
+    d[14] = kind
+    d[15] = np.float32(count) - np.float32(1.0)
+    return d
+
+
+@pytest.mark.parametrize("mode", ["board", "stretched"])
+def test_debris_flux_field_respawn(gpu_ready, mode):
+    """Concept C: respawned debris are placed through the flux-field inverse CDF
+    (rs.cl:795-836).  "board" goes through the host runtime
+    (RS_set_debris_flux_field_to_checker_board(9), rs.c:3462-3504); "stretched"
+    sets a stretched-grid field through the C-ABI (rsd_set_dff).  State bit-exact
+    with the oracle; after the first step every lifted body has respawned, and
+    on the board almost all land in cells that carry flux."""
+    pdf = np.zeros(81, np.float32)
+    pdf[::2] = 100.0
+    s = np.float32(0.0)
+    for v in pdf:
+        s = np.float32(s + v)
+    pdf = (pdf / s).astype(np.float32)
+    icdf = orc.dff_icdf_from_pdf(pdf)
+    ninth = np.float32(1.0) / np.float32(9.0)
+    lifted = {}
+
+    def pre(r):
+        # concepts enter the simulation descriptor when the population is made
+        r.set_concept(lifted["concept"])
+
+    def setup(r, o):
+        assert int(np.asarray(r.sim_desc()[5], np.float32).view(np.uint32)) & 128
+        if mode == "board":
+            r.set_debris_flux_field_to_checker_board(9)
+            desc = _dff_desc(ninth, -1.0, 8.0, 2.0, icdf.size)
+        else:
+            desc = _dff_desc(-400.0, 1.05, 8.0, 1.0, icdf.size)
+            rc = sa.load().rsd_set_dff(r.engine(), icdf.ctypes.data, icdf.size, desc.ctypes.data)
+            assert rc == 0, sa.load().rsd_last_error(r.engine())
+        r.check()
+        orc.set_dff(icdf, desc)
+        lifted["n0"] = _lift_debris(r, o)
+
+    cfg = h.Config(n_rain=4096, debris=[(sa.OBJ_LEAF, 2048)], scan="P:3.0,-1:1:0.01")
+    r0 = sa.Radar(verb=0)      # the configuration's own concepts, plus C
+    h.configure(r0, cfg)
+    r0.populate()
+    r0.check()
+    lifted["concept"] = int(np.asarray(r0.sim_desc()[5], np.float32).view(np.uint32)) | 128
+    r0.free()
+    try:
+        r, o, pg, po = _run(cfg, 1, setup=setup, pre=pre)
+        _check_state(r, o)
+        sd = r.sim_desc()
+        pos = r.stream("pos")[lifted["n0"]::2]
+        assert np.all(pos[:, 2] == np.float32(2.0)), "lifted bodies not respawned at MIN_HEIGHT"
+        if mode == "board":
+            ix = np.floor((pos[:, 0] - sd[8]) / (sd[12] / 9.0)).astype(int)
+            iy = np.floor((pos[:, 1] - sd[9]) / (sd[13] / 9.0)).astype(int)
+            assert np.mean((iy * 9 + ix) % 2 == 0) > 0.95
+        r.free()
+        r, o, pg, po = _run(cfg, 5, setup=setup, pre=pre)
+        _check_state(r, o)
+        # pulse 0 on the engine was made by the population, before the lift
+        errs = [h.rel_err(pg[k], po[k]) for k in range(1, len(pg))]
+        assert max(errs) < IQ_TOL, errs
+        r.free()
+    finally:
+        orc.set_dff(None, None)
 
 
 def test_rain_sorted_every_pulse(gpu_ready):
