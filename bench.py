@@ -32,21 +32,45 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_rain_c2_traffic.json")
 REFERENCE_PULSES_PER_S = 222.3   # README.md:58 (instantaneous, 1,998,848 scatterers, GPU not stated)
 
 
-def cpu_baseline(r, cfg, pulses, threads):
+def cpu_baseline(r, cfg, pulses, threads, warmup=5, runs=3):
     """The oracle (CPU restatement of the reference kernels: the same kernel
     sequence, dense pass 1 over all gates) timed on a bounded sample of the same
-    workload, each population split over `threads` host threads."""
+    workload, each population split over `threads` host threads: `warmup`
+    untimed pulses, then `runs` runs of `pulses` pulses; the median rate."""
     from oracle import run as orun
     st = orun.state_from_radar(r)
     o = orun.OracleRun(r, cfg, st, threads=threads)
     sd = r.sim_desc()
-    t0 = time.perf_counter()
-    for _ in range(pulses):
+
+    def one():
         o.advance(sd)
         o.refresh_debris_rcs(sd)
         o.pulse(sd)
-    dt = time.perf_counter() - t0
-    return pulses / dt
+    for _ in range(warmup):
+        one()
+    rates = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        for _ in range(pulses):
+            one()
+        rates.append(pulses / (time.perf_counter() - t0))
+    return sorted(rates)[len(rates) // 2], rates
+
+
+def host_cores():
+    """Logical CPUs of the host and the CPUs this process may run on."""
+    return os.cpu_count(), len(os.sched_getaffinity(0))
+
+
+def timed_pulses(requested, interval):
+    """Pulses in the timed window: the request rounded up to whole range-reorder
+    intervals (a reorder runs every `interval` pulses), and at least two of
+    them, so the reorder's cost lands in the per-pulse time whatever the
+    window's phase."""
+    if interval <= 0:
+        return max(1, requested)
+    n = max(requested, 2 * interval)
+    return (n + interval - 1) // interval * interval
 
 
 def main():
@@ -87,33 +111,39 @@ def main():
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
-        print("bench: cpu baseline (%d pulses of the oracle on %d threads)" % (args.cpu_pulses, threads),
-              file=sys.stderr, flush=True)
-        cpu_pps = cpu_baseline(r, cfg, args.cpu_pulses, threads)
+        ncpu, naff = host_cores()
+        print("bench: cpu baseline (5 + 3 x %d pulses of the oracle on %d threads; host has %d CPUs, %d usable)"
+              % (args.cpu_pulses, threads, ncpu, naff), file=sys.stderr, flush=True)
+        cpu_pps, cpu_runs = cpu_baseline(r, cfg, args.cpu_pulses, threads)
         cpu = {"value": cpu_pps, "unit": "pulses/s", "cores": threads, "kind": "port",
-               "sample": "%d pulses of the full %d-scatterer workload, oracle C restatement (dense pass 1 over "
-                         "all gates), scatterers split over %d host threads" % (args.cpu_pulses, cfg.n_total,
-                                                                              threads)}
+               "host_cpus": ncpu, "usable_cpus": naff, "runs": cpu_runs,
+               "sample": "median of 3 runs of %d pulses (after 5 warm-up pulses) of the full %d-scatterer "
+                         "workload, oracle C restatement (dense pass 1 over all gates), scatterers split over "
+                         "%d host threads" % (args.cpu_pulses, cfg.n_total, threads)}
 
+    # One step = one PRT: RS_advance_time + RS_advance_beam + RS_make_pulse.  The
+    # advance is deferred into the pulse's launch, so every step is one fused
+    # advance + pulse launch and nothing is left pending between steps.
     def step():
-        r.make_pulse()
         r.advance_time()
         r.advance_beam()
+        r.make_pulse()
 
     for _ in range(args.warmup):
         step()
     r.sync()
     r.check()
+    steps = timed_pulses(args.steps, r.sort_interval())
     # HIP events around every PROFILE_STRIDE-th rain launch of the timed loop
     # (events on every launch add a few us of queue packets per pulse)
     r.profile(True, stride=PROFILE_STRIDE)
-    print("bench: timing %d pulses" % args.steps, file=sys.stderr, flush=True)
+    print("bench: timing %d pulses (%d requested, whole range-reorder intervals)" % (steps, args.steps),
+          file=sys.stderr, flush=True)
     r.sync()
     t0 = time.perf_counter()
     r.timer_start()
-    for _ in range(args.steps):
+    for _ in range(steps):
         step()
-    r.make_pulse()          # flush the last advance through a fused launch
     ev_ms = r.timer_stop()
     r.sync()
     wall = time.perf_counter() - t0
@@ -132,7 +162,6 @@ def main():
             r.download_pulse_only()
         e2e = args.e2e_steps / (time.perf_counter() - t1)
         r.check()
-    steps = args.steps + 1
     ms_per_step = ev_ms / steps
     pps = 1000.0 / ms_per_step
     n = cfg.n_total
@@ -154,6 +183,7 @@ def main():
         "unit": "pulses/s",
         "n_gpus": 1,
         "steps": steps,
+        "steps_requested": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms_per_step,
         "wall_ms_per_step": 1000.0 * wall / steps,
@@ -195,13 +225,13 @@ def bench_c3(args):
     r = wl.make(cfg)
     r.set_position_precision(1)
     R = int(r.params().range_count)
-    steps = min(args.steps, 2000) if args.steps != 1000 else 2000
+    steps = timed_pulses(min(args.steps, 2000) if args.steps != 1000 else 2000, r.sort_interval())
     warm = min(args.warmup, 50)
 
     def step():
-        r.make_pulse()
         r.advance_time()
         r.advance_beam()
+        r.make_pulse()
     for _ in range(warm):
         step()
     r.sync()
@@ -209,15 +239,14 @@ def bench_c3(args):
     r.timer_start()
     for _ in range(steps):
         step()
-    r.make_pulse()
-    ms = r.timer_stop() / (steps + 1)
+    ms = r.timer_stop() / steps
     k_ms, k_n = r.profile_read(3)
     r.profile(False)
     r.check()
     kern = k_ms / max(1, k_n)
     byts = 96 * cfg.n_rain            # pos + vel + lo read and written
     ach = byts / (kern * 1e-3) / 1e9
-    _line("pulses/s & scatterer-updates/s (C3, hi/lo positions)", 1000.0 / ms, "pulses/s", steps + 1, warm, ms,
+    _line("pulses/s & scatterer-updates/s (C3, hi/lo positions)", 1000.0 / ms, "pulses/s", steps, warm, ms,
           "f32 state, f32 hi/lo positions, f64 range-to-phase and gate sums",
           {"workload": "C3: 8,388,608 rain, 3 km x 3 km x 1 km box at 10 km range, %d gates x 15 m" % R,
            "scatterers": cfg.n_rain, "range_gates": R},
@@ -289,21 +318,21 @@ def bench_c5(args):
     cfg = wl.c5()
     print("bench: populating C5 (%d scatterers)" % cfg.n_total, file=sys.stderr, flush=True)
     r = wl.make(cfg)
-    steps = 500 if args.steps == 1000 else args.steps
+    steps = timed_pulses(500 if args.steps == 1000 else args.steps, r.sort_interval())
+
+    def step():
+        r.advance_time()
+        r.advance_beam()
+        r.make_pulse()
 
     def timed():
         for _ in range(20):
-            r.make_pulse()
-            r.advance_time()
-            r.advance_beam()
+            step()
         r.sync()
         r.timer_start()
         for _ in range(steps):
-            r.make_pulse()
-            r.advance_time()
-            r.advance_beam()
-        r.make_pulse()
-        return r.timer_stop() / (steps + 1)
+            step()
+        return r.timer_stop() / steps
     ms_table = timed()
     t0 = time.perf_counter()
     nets, errs = [], []
@@ -315,7 +344,7 @@ def bench_c5(args):
     for k, m in zip((1, 2, 3), nets):
         r.set_debris_surrogate(k, m)
     ms_surr = timed()
-    _line("pulses/s & scatterer-updates/s (C5, debris surrogate)", 1000.0 / ms_surr, "pulses/s", steps + 1, 20,
+    _line("pulses/s & scatterer-updates/s (C5, debris surrogate)", 1000.0 / ms_surr, "pulses/s", steps, 20,
           ms_surr, "f32 physics, bf16 network (fp32 accumulate), f64 phase and gate sums",
           {"workload": "C5: 2,097,152 rain + leaf, plate, brick at 262,144 each; surrogate networks 7-4x64-12",
            "scatterers": cfg.n_total},

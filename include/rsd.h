@@ -154,6 +154,8 @@ int rsd_set_launch(rsd_engine *e, int block_size, int max_blocks);
  * (its streams stay in index order); it lets each lane sum its scatterers'
  * gate terms in registers and keeps wind-table gathers local. */
 int rsd_set_sort(rsd_engine *e, int interval);
+/* Pulses between reorders currently in effect (0: never). */
+int rsd_get_sort(const rsd_engine *e);
 
 /* Strategy of the per-gate sums inside a work-group (results agree to
  * floating-point reassociation; all end in binary64 LDS and block partials):

@@ -80,7 +80,7 @@ EXPORTED_RSD = [
     "rsd_upload_state", "rsd_download_state", "rsd_set_wind", "rsd_set_adm", "rsd_set_rcs", "rsd_set_drop_rcs", "rsd_set_dff",
     "rsd_set_range_weight", "rsd_set_angular_weight", "rsd_advance", "rsd_make_pulse", "rsd_read_pulse",
     "rsd_sync", "rsd_timer_start", "rsd_timer_stop", "rsd_set_launch", "rsd_math_probe", "rsd_profile",
-    "rsd_profile_read", "rsd_io_test", "rsd_set_sort", "rsd_set_fold", "rsd_set_iq_ring", "rsd_moments",
+    "rsd_profile_read", "rsd_io_test", "rsd_set_sort", "rsd_get_sort", "rsd_set_fold", "rsd_set_iq_ring", "rsd_moments",
     "rsd_set_position_precision", "rsd_download_pos_lo", "rsd_batch_create", "rsd_batch_destroy",
     "rsd_batch_launch", "rsd_batch_sync", "rsd_batch_set_launch", "rsd_batch_get_launch", "rsd_batch_set_graph",
     "rsd_batch_timer_start", "rsd_batch_timer_stop", "rsd_batch_last_error",
@@ -247,6 +247,7 @@ def _declare(lib):
         "rsd_timer_stop": (I, [P, ctypes.POINTER(ctypes.c_double)]),
         "rsd_set_launch": (I, [P, I, I]),
         "rsd_set_sort": (I, [P, I]),
+        "rsd_get_sort": (I, [P]),
         "rsd_set_fold": (I, [P, I]),
         "rsd_set_iq_ring": (I, [P, I, I]),
         "rsd_moments": (I, [P, I, P, P]),
@@ -536,6 +537,10 @@ class Radar:
         e = self.engine()
         if self.lib.rsd_set_sort(e, interval):
             raise RuntimeError(self.lib.rsd_last_error(e).decode())
+
+    def sort_interval(self):
+        """Pulses between spatial reorders of the rain streams in effect."""
+        return int(self.lib.rsd_get_sort(self.engine()))
 
     def sync(self):
         e = self.engine()

@@ -2703,6 +2703,8 @@ extern "C" int rsd_set_sort(rsd_engine *e, int interval) {
     return RSD_OK;
 }
 
+extern "C" int rsd_get_sort(const rsd_engine *e) { return e ? e->sort_interval : 0; }
+
 static int flush_pending(rsd_engine *e) {
     if (e->batch && e->have_pending_pulse) {
         int st = rsd_batch_launch(e->batch);
