@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
 
 PHASE_F32, PHASE_F64, FULL_F64 = 0, 1, 2
-MATH = {"log1p": 0, "log": 1, "sin": 2, "cos": 3, "atan2": 4, "acos": 5}
+MATH = {"log1p": 0, "log": 1, "sin": 2, "cos": 3, "atan2": 4, "acos": 5, "pow": 6, "exp": 7}
 
 _lib = None
 
@@ -48,6 +48,7 @@ def lib():
             "orc_bg_atts": (None, [P, P, P, P, S, S, T, P, P, P, P]),
             "orc_db_atts": (None, [P, P, P, P, P, P, S, S, T, P, T, T, P, T, T, P, P]),
             "orc_set_dff": (None, [P, ctypes.c_int, P]),
+            "orc_dff_place_probe": (ctypes.c_int, [F4, P, P]),
             "orc_dff_icdf_from_pdf": (ctypes.c_int, [P, ctypes.c_int, P, ctypes.c_int]),
             "orc_db_rcs": (None, [P, P, P, S, S, T, T, P, P]),
             "orc_scat_sig_aux": (None, [P, P, P, P, S, P, P, P]),
@@ -195,6 +196,13 @@ def set_dff(icdf, desc16):
     else:
         icdf = f32(icdf)
         lib().orc_set_dff(_p(icdf), icdf.size, _p(f32(desc16)))
+
+
+def dff_place(r4, sim_desc):
+    """Respawn position (x, y, z, w) of one draw r4 under the field of set_dff."""
+    out = np.zeros(4, np.float32)
+    rc = lib().orc_dff_place_probe(_f4(r4), _p(f32(sim_desc)), _p(out))
+    return None if rc else out
 
 
 def dff_icdf_from_pdf(pdf, n=2048):

@@ -6,7 +6,7 @@
 #include <stddef.h>
 #include "ofmath.h"
 
-enum { OF_LOG1P = 0, OF_LOG, OF_SIN, OF_COS, OF_ATAN2, OF_ACOS };
+enum { OF_LOG1P = 0, OF_LOG, OF_SIN, OF_COS, OF_ATAN2, OF_ACOS, OF_POW, OF_EXP };
 
 void orc_math(int fn, const float *a, const float *b, float *out, size_t n) {
     for (size_t i = 0; i < n; i++) {
@@ -17,6 +17,8 @@ void orc_math(int fn, const float *a, const float *b, float *out, size_t n) {
             case OF_COS: out[i] = of_cosf(a[i]); break;
             case OF_ATAN2: out[i] = of_atan2f(a[i], b[i]); break;
             case OF_ACOS: out[i] = of_acosf(a[i]); break;
+            case OF_POW: out[i] = of_powf_pos(a[i], b[i]); break;
+            case OF_EXP: out[i] = of_expf(a[i]); break;
             default: out[i] = 0.0f;
         }
     }

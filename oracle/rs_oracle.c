@@ -378,6 +378,14 @@ static void orc_dff_place(of4 r, const float *sd, of4 *pos) {
     pos->z = MIN_HEIGHT;
 }
 
+/* Test entry: the placement of one respawn draw r under the flux field set by
+ * orc_set_dff (pos.z = MIN_HEIGHT); returns -1 when no field is set. */
+int orc_dff_place_probe(of4 r, const float *sd, of4 *pos) {
+    if (!orc_dff_tab) return -1;
+    orc_dff_place(r, sd, pos);
+    return 0;
+}
+
 void orc_db_atts(of4 *p, of4 *o, of4 *v, of4 *t, of4 *x, ou4 *y, size_t begin, size_t end,
                  const orc_tex *wind, const float *les_desc,
                  const orc_tex *adm_cd, const orc_tex *adm_cm, const float *ad,

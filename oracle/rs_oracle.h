@@ -60,6 +60,9 @@ void orc_bg_atts(of4 *p, of4 *v, of4 *x, ou4 *y, size_t begin, size_t end,
 /* rs.cl:744-907 -- db_atts over [begin, end) (debris flux field concept not covered). */
 void orc_set_dff(const float *icdf, int count, const float *desc16);
 int orc_dff_icdf_from_pdf(const float *pdf, int pdf_count, float *tab, int n);
+/* Respawn placement of one draw r (rs.cl:795-836) under the field of
+ * orc_set_dff; -1 when none is set. */
+int orc_dff_place_probe(of4 r, const float *sim_desc, of4 *pos);
 void orc_db_atts(of4 *p, of4 *o, of4 *v, of4 *t, of4 *x, ou4 *y, size_t begin, size_t end,
                  const orc_tex *wind, const float *les_desc,
                  const orc_tex *adm_cd, const orc_tex *adm_cm, const float *adm_desc,
