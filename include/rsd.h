@@ -90,8 +90,10 @@ const char *rsd_last_error(const rsd_engine *e);
 int rsd_alloc_state(rsd_engine *e, size_t num_scats, const size_t *origins, const size_t *counts);
 
 /* Full-stream transfers (num_scats elements each; any pointer may be NULL on
- * download).  sig and aux on download are computed for the current positions
- * with the beam of the most recent pulse (rs.cl:1050-1110 quantities). */
+ * download).  sig and aux on download are those of the most recent pulse: its
+ * beam and the positions it saw, also after an advance since (rs.cl:1050-1110
+ * writes them in the pulse and the advance leaves them).  Before any pulse sig
+ * is zero and aux is the stored stream. */
 int rsd_upload_state(rsd_engine *e, const float *pos, const float *vel, const float *ori,
                      const float *tum, const float *aux, const float *rcs, const uint32_t *rnd);
 int rsd_download_state(rsd_engine *e, float *pos, float *vel, float *ori, float *tum,

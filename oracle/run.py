@@ -51,7 +51,10 @@ class OracleRun:
     advance is unchanged bit for bit, the pulse sums the per-range gate sums in
     range order (binary64 reassociation only).  Tests use one thread."""
 
-    def __init__(self, r, cfg, state, threads=1):
+    def __init__(self, r, cfg, state, threads=1, wind=None, rangew=None, angw=None):
+        """wind = (float32 [nz, ny, nx, 4] texels, les_desc16) replaces the LES
+        frame 0 / stretched-grid descriptor; rangew / angw = (table, desc4)
+        replace the triangle range weight and the standard antenna pattern."""
         self.threads = max(1, int(threads))
         self._pool = ThreadPoolExecutor(self.threads) if self.threads > 1 else None
         self.r = r
@@ -63,17 +66,24 @@ class OracleRun:
         p = r.params()
         self.params = p
         sd = r.sim_desc()
-        frame = r.les_frame(0)
-        self.wind_tex, self._wind_keep = orc.tex(frame)
-        self.les_desc = orc.les_desc(True, frame.shape[2], frame.shape[1], frame.shape[0], 2.0, 2.0, 2.0,
-                                     1.0212, 1.0212, 1.05, 50.0)
+        if wind is None:
+            self.set_les_frame(r.les_frame(0))
+        else:
+            self.wind_tex, self._wind_keep = orc.tex(wind[0])
+            self.les_desc = np.asarray(wind[1], np.float32)
         dsd_r, dsd_pdf, _ = r.dsd()
         uniform = bool(cfg.concept & CONCEPT_UNIFORM_DSD)
         self.drop, self.drop_desc = orc.drop_rcs_table(sd[4], sd[6], bool(cfg.concept & CONCEPT_TRANSPARENT),
                                                        uniform, dsd_r, dsd_pdf)
-        self.angw, self.angw_desc = orc.angular_weight_standard(p.antenna_bw_rad)
-        self.rangew = np.array([0.0, 1.0, 0.0], np.float32)
-        self.rangew_desc = orc.table1d_desc(-p.dr, p.dr, 3)
+        if angw is None:
+            self.angw, self.angw_desc = orc.angular_weight_standard(p.antenna_bw_rad)
+        else:
+            self.angw, self.angw_desc = np.asarray(angw[0], np.float32), np.asarray(angw[1], np.float32)
+        if rangew is None:
+            self.rangew = np.array([0.0, 1.0, 0.0], np.float32)
+            self.rangew_desc = orc.table1d_desc(-p.dr, p.dr, 3)
+        else:
+            self.rangew, self.rangew_desc = np.asarray(rangew[0], np.float32), np.asarray(rangew[1], np.float32)
         self.kd = 4.0 * np.pi / float(np.float32(p.lambda_))
         self.deb = []
         for t in range(len(cfg.debris)):
@@ -82,6 +92,13 @@ class OracleRun:
             rd = r.descriptor(32 + t)
             tx = [orc.tex(a) for a in (cd, cm, re, im)]
             self.deb.append(dict(tex=tx, adm_desc=ad, rcs_desc=rd))
+
+    def set_les_frame(self, frame):
+        """Wind of later advances: one suctvort-geometry LES frame (stretched
+        grid, les.c:164-170), as the host uploads at a volume switch."""
+        self.wind_tex, self._wind_keep = orc.tex(frame)
+        self.les_desc = orc.les_desc(True, frame.shape[2], frame.shape[1], frame.shape[0], 2.0, 2.0, 2.0,
+                                     1.0212, 1.0212, 1.05, 50.0)
 
     def _ranges(self, b, e):
         n = self.threads

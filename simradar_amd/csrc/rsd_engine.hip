@@ -1777,12 +1777,15 @@ __global__ void k_view(ViewK a) {
     } else {
         rcs = a.rcs[i];
     }
-    a.pos_out[o] = p;
-    a.rcs_out[o] = rcs;
+    if (a.pos_out) {
+        a.pos_out[o] = p;
+        a.rcs_out[o] = rcs;
+    }
     if (a.vel_out) {
         a.vel_out[o] = a.vel[i];
         a.rnd_out[o] = a.rnd[i];
     }
+    if (!a.sig_out) return;
     float4 aux = a.aux_in[o];    // aux stays in the reference order
     float4 sig = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (a.have_pulse) {
@@ -1825,7 +1828,11 @@ struct rsd_engine {
     size_t origins[RSD_MAX_TYPES], counts[RSD_MAX_TYPES];
     float4 *pos, *vel, *ori, *tum, *rcs, *aux;
     uint4 *rnd;
-    float4 *view;           // download staging: 6 x n float4
+    float4 *view;           // download staging: 4 x n float4 (pos, rcs, vel, rnd)
+    // signal and aux streams of the last pulse (rs.cl:1050-1110 writes them at
+    // the pulse): 2 x n float4, reference order, valid while sigaux_saved
+    float4 *sigaux;
+    int sigaux_saved;
     // spatial order (rain range only): ping-pong streams, reference index,
     // radix-sort keys / permutation and hipCUB scratch
     float4 *pos2, *vel2, *rcs2;
@@ -2023,7 +2030,9 @@ extern "C" int rsd_open(int device_index, rsd_engine **out) {
 
 static void free_state(rsd_engine *e) {
     dfree(e->pos); dfree(e->vel); dfree(e->ori); dfree(e->tum); dfree(e->rcs); dfree(e->aux);
-    dfree(e->rnd); dfree(e->view);
+    dfree(e->rnd); dfree(e->view); dfree(e->sigaux);
+    e->sigaux = NULL;
+    e->sigaux_saved = 0;
     dfree(e->pos2); dfree(e->vel2); dfree(e->rcs2); dfree(e->rnd2);
     dfree(e->id); dfree(e->id2); dfree(e->skey); dfree(e->skey2); dfree(e->sperm); dfree(e->sperm2);
     dfree(e->sort_tmp);
@@ -2493,6 +2502,7 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p) {
         HIPCHK(e, hipGetLastError());
         e->last_pulse = *p;
         e->have_pulse = 1;
+        e->sigaux_saved = 0;
     }
     return RSD_OK;
 }
@@ -2629,7 +2639,7 @@ extern "C" int rsd_download_pos_lo(rsd_engine *e, float *lo) {
     HIPCHK(e, hipSetDevice(e->device));
     join_batch(e);
     const size_t b = e->n * sizeof(float4);
-    if (!e->view) HIPCHK(e, hipMalloc((void **)&e->view, 6 * b));
+    if (!e->view) HIPCHK(e, hipMalloc((void **)&e->view, 4 * b));
     unsigned nb = (unsigned)((e->n + 255) / 256);
     hipLaunchKernelGGL(k_lo_view, dim3(nb), dim3(256), 0, e->stream, (const float4 *)e->plo,
                        (const uint32_t *)(e->permuted ? e->id : NULL), (unsigned long long)e->n_rain,
@@ -2705,12 +2715,21 @@ extern "C" int rsd_set_sort(rsd_engine *e, int interval) {
 
 extern "C" int rsd_get_sort(const rsd_engine *e) { return e ? e->sort_interval : 0; }
 
+static int save_signal(rsd_engine *e);
+
+// Runs a held-back advance on its own.  The signal and aux streams of the last
+// pulse are first taken from the positions that pulse saw (the reference
+// computes them in the pulse and keeps them through the advance).
 static int flush_pending(rsd_engine *e) {
     if (e->batch && e->have_pending_pulse) {
         int st = rsd_batch_launch(e->batch);
         if (st) return fail(e, st, "batch launch failed");
     }
     if (!e->have_pending) return RSD_OK;
+    if (!e->sigaux_saved) {
+        int st = save_signal(e);
+        if (st) return st;
+    }
     e->have_pending = 0;
     return launch_step(e, &e->pending, NULL);
 }
@@ -2738,51 +2757,83 @@ extern "C" int rsd_upload_state(rsd_engine *e, const float *pos, const float *ve
         hipLaunchKernelGGL(k_flag_rain, dim3(nb), dim3(256), 0, e->stream, e->pos, (unsigned long long)e->n_rain);
         HIPCHK(e, hipGetLastError());
     }
+    if (pos || aux || rcs) e->sigaux_saved = 0;   // recomputed from the uploaded state
     HIPCHK(e, hipStreamSynchronize(e->stream));
+    return RSD_OK;
+}
+
+static void view_common(rsd_engine *e, ViewK &v) {
+    memset(&v, 0, sizeof(v));
+    v.pos = e->pos; v.rcs = e->rcs; v.aux_in = e->aux; v.vel = e->vel; v.rnd = e->rnd;
+    v.id = e->permuted ? e->id : NULL;
+    v.n = e->n;
+    v.n_rain = e->n_rain;
+    v.drop = e->drop;
+    v.drop_desc = e->drop_desc;
+}
+
+// Signal and aux streams of the last pulse from the stored state into sigaux
+// (zero signal and the stored aux before any pulse).  Valid until the next
+// pulse; the stored positions must be the ones the pulse saw.
+static int save_signal(rsd_engine *e) {
+    if (e->n_rain && !e->drop) return fail(e, RSD_ERR_STATE, "drop RCS table not set");
+    HIPCHK(e, hipSetDevice(e->device));
+    join_batch(e);
+    if (!e->sigaux) HIPCHK(e, hipMalloc((void **)&e->sigaux, 2 * e->n * sizeof(float4)));
+    ViewK v;
+    view_common(e, v);
+    v.sig_out = e->sigaux;
+    v.aux_out = e->sigaux + e->n;
+    if (e->have_pulse) {
+        int st = fill_pulse(e, &e->last_pulse, v.pk);
+        if (st) return st;
+        v.have_pulse = 1;
+    }
+    unsigned nb = (unsigned)((e->n + 255) / 256);
+    hipLaunchKernelGGL(k_view, dim3(nb), dim3(256), 0, e->stream, v);
+    HIPCHK(e, hipGetLastError());
+    e->sigaux_saved = 1;
     return RSD_OK;
 }
 
 extern "C" int rsd_download_state(rsd_engine *e, float *pos, float *vel, float *ori, float *tum,
                                   float *aux, float *rcs, float *sig, uint32_t *rnd) {
     if (!e || !e->pos) return fail(e, RSD_ERR_STATE, "state not allocated");
-    int st = flush_pending(e);
+    int st;
+    if ((sig || aux) && !e->sigaux_saved && !(e->batch && e->have_pending_pulse)) {
+        st = save_signal(e);    // no advance since the last pulse: its positions are stored
+        if (st) return st;
+    }
+    st = flush_pending(e);
     if (st) return st;
     join_batch(e);
+    if ((sig || aux) && !e->sigaux_saved) {
+        st = save_signal(e);    // the batch launch above made the pulse
+        if (st) return st;
+    }
     HIPCHK(e, hipSetDevice(e->device));
     const size_t b = e->n * sizeof(float4);
     const bool perm = e->permuted && (vel || rnd);
-    if (pos || rcs || sig || aux || perm) {
-        if (!e->view) HIPCHK(e, hipMalloc((void **)&e->view, 6 * b));
-        ViewK v;
-        memset(&v, 0, sizeof(v));
-        v.pos = e->pos; v.rcs = e->rcs; v.aux_in = e->aux; v.vel = e->vel; v.rnd = e->rnd;
-        v.id = e->permuted ? e->id : NULL;
-        v.pos_out = e->view; v.rcs_out = e->view + e->n; v.sig_out = e->view + 2 * e->n; v.aux_out = e->view + 3 * e->n;
-        if (perm) {
-            v.vel_out = e->view + 4 * e->n;
-            v.rnd_out = (uint4 *)(e->view + 5 * e->n);
-        }
-        v.n = e->n;
-        v.n_rain = e->n_rain;
-        v.drop = e->drop;
-        v.drop_desc = e->drop_desc;
-        v.have_pulse = 0;
-        if (e->have_pulse) {
-            st = fill_pulse(e, &e->last_pulse, v.pk);
-            if (st) return st;
-            v.have_pulse = 1;
-        }
+    if (pos || rcs || perm) {
+        if (!e->view) HIPCHK(e, hipMalloc((void **)&e->view, 4 * b));
         if (e->n_rain && !e->drop) return fail(e, RSD_ERR_STATE, "drop RCS table not set");
+        ViewK v;
+        view_common(e, v);
+        v.pos_out = e->view; v.rcs_out = e->view + e->n;
+        if (perm) {
+            v.vel_out = e->view + 2 * e->n;
+            v.rnd_out = (uint4 *)(e->view + 3 * e->n);
+        }
         unsigned nb = (unsigned)((e->n + 255) / 256);
         hipLaunchKernelGGL(k_view, dim3(nb), dim3(256), 0, e->stream, v);
         HIPCHK(e, hipGetLastError());
         if (pos) HIPCHK(e, hipMemcpyAsync(pos, v.pos_out, b, hipMemcpyDeviceToHost, e->stream));
         if (rcs) HIPCHK(e, hipMemcpyAsync(rcs, v.rcs_out, b, hipMemcpyDeviceToHost, e->stream));
-        if (sig) HIPCHK(e, hipMemcpyAsync(sig, v.sig_out, b, hipMemcpyDeviceToHost, e->stream));
-        if (aux) HIPCHK(e, hipMemcpyAsync(aux, v.aux_out, b, hipMemcpyDeviceToHost, e->stream));
         if (perm && vel) HIPCHK(e, hipMemcpyAsync(vel, v.vel_out, b, hipMemcpyDeviceToHost, e->stream));
         if (perm && rnd) HIPCHK(e, hipMemcpyAsync(rnd, v.rnd_out, e->n * sizeof(uint4), hipMemcpyDeviceToHost, e->stream));
     }
+    if (sig) HIPCHK(e, hipMemcpyAsync(sig, e->sigaux, b, hipMemcpyDeviceToHost, e->stream));
+    if (aux) HIPCHK(e, hipMemcpyAsync(aux, e->sigaux + e->n, b, hipMemcpyDeviceToHost, e->stream));
     if (vel && !perm) HIPCHK(e, hipMemcpyAsync(vel, e->vel, b, hipMemcpyDeviceToHost, e->stream));
     if (ori) HIPCHK(e, hipMemcpyAsync(ori, e->ori, b, hipMemcpyDeviceToHost, e->stream));
     if (tum) HIPCHK(e, hipMemcpyAsync(tum, e->tum, b, hipMemcpyDeviceToHost, e->stream));
@@ -3487,6 +3538,7 @@ extern "C" int rsd_batch_launch(rsd_batch *b) {
         e->have_pending_pulse = 0;
         e->last_pulse = e->pending_pulse;
         e->have_pulse = 1;
+        e->sigaux_saved = 0;
         e->batch_wait = 1;
     }
     return RSD_OK;

@@ -23,6 +23,7 @@ int main(int argc, char **argv) {
     RS_set_prt(H, 1.0e-3f);
     RS_set_lambda(H, 0.1f);
     RS_set_sampling_spacing(H, 15.0f, 1.0f, 1.0f);
+    RS_set_dsd_to_mp(H);                          /* simple_ppi.c:37 */
     RS_set_vel_data_to_config(H, LESConfigSuctionVortices);
     RS_set_scan_box(H, RS_suggest_scan_domain(H));
     RS_set_random_seed(H, 19760520);

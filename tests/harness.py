@@ -25,3 +25,72 @@ def rel_err(a, b):
     if scale == 0.0:
         return float(np.max(np.abs(a - b)))
     return float(np.max(np.abs(a - b)) / scale)
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+def run_pair(cfg, pulses, sort=None, fold=None, launch=None, pre=None, setup=None, oracle_kw=None,
+             pulse_mode=None, les_switch=False, on_pulse=None):
+    """Populate an engine run of cfg and step it `pulses` times next to the
+    oracle from the same populated state (the driver order of
+    simradar.c:1011-1074).  pre(r) runs before RS_populate, setup(r, o) after
+    the oracle is made; oracle_kw go to OracleRun; pulse_mode selects the
+    oracle's phase precision; les_switch: the oracle follows the host's LES
+    volume switches; on_pulse(k, r, o) runs after pulse k is compared.
+    Returns (radar, oracle, engine pulses, oracle pulses)."""
+    r = sa.Radar(verb=0)
+    configure(r, cfg)
+    if pre is not None:
+        pre(r)
+    r.populate()
+    r.check()
+    if sort is not None:
+        r.set_sort(sort)
+    if fold is not None:
+        r.set_fold(fold)
+    if launch is not None:
+        r.set_launch(*launch)
+    o = OracleRun(r, cfg, state_from_radar(r), **(oracle_kw or {}))
+    sd_adv = r.sim_desc().copy()   # populate's zero-length step (rs.c:4941-4955)
+    sd_adv[11] = 0.0
+    o.advance(sd_adv)
+    o.refresh_debris_rcs(r.sim_desc())
+    if setup is not None:
+        setup(r, o)
+    mode = orc.PHASE_F64 if pulse_mode is None else pulse_mode
+    pg, po = [], []
+    for k in range(pulses):
+        if k > 0:
+            r.make_pulse()
+            o.refresh_debris_rcs(r.sim_desc())
+        sd = r.sim_desc()
+        r.download_pulse_only()
+        r.check()
+        pg.append(r.pulse_f64())
+        po.append(o.pulse(sd, mode))
+        if on_pulse is not None:
+            on_pulse(k, r, o)
+        vi = int(r.info().vel_idx)
+        r.advance_time()
+        if les_switch and int(r.info().vel_idx) != vi:
+            o.set_les_frame(r.les_frame(vi))
+        o.advance(r.sim_desc())
+        r.advance_beam()
+    r.download()
+    r.check()
+    return r, o, np.array(pg), np.array(po)
+
+
+def check_state(r, o, streams=("pos", "vel", "ori", "tum")):
+    for s in streams:
+        a, b = r.stream(s), getattr(o.st, s)
+        bad = np.nonzero(np.any(bits(a) != bits(b), axis=1))[0]
+        assert bad.size == 0, "%s differs at %d scatterers, first %s: %s vs %s" % (s, bad.size, bad[:3], a[bad[:1]],
+                                                                                b[bad[:1]])
+    assert np.array_equal(r.stream("rnd"), o.st.rnd), "seeds differ"
+
+
+def iq_errs(pg, po, first=0):
+    return [rel_err(pg[k], po[k]) for k in range(first, len(pg))]

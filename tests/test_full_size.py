@@ -1,0 +1,76 @@
+"""Each workload configuration at its full size on the GPU against the oracle:
+
+C2  1,997,824 rain + 1,024 leaves: 3 pulses, state bit-exact, IQ within 1e-4
+C3  8,388,608 rain in the 10 km box with hi/lo positions: 3 pulses against the
+    binary64 oracle (IQ within 1e-4, Z within 0.1 dB, V within 0.05 m/s), the
+    all-binary32 variant far worse
+C4  all 64 domains (2^16..2^21 rain each): one persistent launch per round gives
+    every domain its run-alone state bit for bit; the largest domain run alone
+    matches the oracle
+C5  2,097,152 rain + 3 x 262,144 debris on the table path: 2 pulses, state
+    bit-exact, IQ within 1e-4
+
+The oracle runs on the host's threads (each population split into contiguous
+pieces; the advance is unchanged bit for bit, the pulse sums the pieces' gate
+sums in order)."""
+import os
+
+import numpy as np
+import pytest
+
+import harness as h
+import precision_study as ps
+from simradar_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+IQ_TOL = 1e-4
+THREADS = max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def _full(cfg, pulses):
+    r, o, pg, po = h.run_pair(cfg, pulses, oracle_kw={"threads": THREADS})
+    assert int(r.info().num_scats) == cfg.n_total
+    h.check_state(r, o)
+    errs = h.iq_errs(pg, po)
+    assert max(errs) < IQ_TOL, errs
+    r.free()
+
+
+def test_c2_full_size(gpu_ready):
+    _full(wl.c2(scan="P:3.0,-1:1:0.01"), 3)
+
+
+def test_c5_table_path_full_size(gpu_ready):
+    _full(wl.c5(scan="P:3.0,-1:1:0.01"), 2)
+
+
+def test_c3_full_size_hilo_against_the_binary64_oracle(gpu_ready):
+    rows, live, R = ps.run(8388608, 3, threads=THREADS)
+    print()
+    print(ps.table(rows))
+    assert R == 202 and live >= 20
+    hl = rows["hilo"]
+    assert hl["iq"] < IQ_TOL, rows
+    assert hl["z"] < 0.1 and hl["v"] < 0.05 and hl["zdr"] < 0.1, rows
+    assert rows["fp32"]["iq"] > 10 * hl["iq"], rows
+
+
+def test_c4_all_64_domains_batched_equal_run_alone(gpu_ready):
+    from test_batch import _compare, _run_alone, _run_batched
+    cfgs = wl.c4(64)
+    assert len(cfgs) == 64 and min(c.n_rain for c in cfgs) >= 1 << 16 and max(c.n_rain for c in cfgs) <= 1 << 21
+    rounds = 2
+    alone = [_run_alone(c, rounds) for c in cfgs]
+    batched, _ = _run_batched(cfgs, rounds, graph=True, shape=(2048, 4))
+    _compare(alone, batched)
+
+
+def test_c4_largest_domain_against_the_oracle(gpu_ready):
+    cfgs = wl.c4(64)
+    big = max(cfgs, key=lambda c: c.n_rain)
+    r, o, pg, po = h.run_pair(big, 3, oracle_kw={"threads": THREADS})
+    h.check_state(r, o, ("pos", "vel"))
+    errs = h.iq_errs(pg, po)
+    assert max(errs) < IQ_TOL, errs
+    r.free()

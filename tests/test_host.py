@@ -50,7 +50,7 @@ def test_reference_style_caller_compiles_and_reports_no_device(tmp_path):
                     "-lsimradar_amd", "-Wl,-rpath," + os.path.join(ROOT, "simradar_amd"), "-lm", "-o", exe],
                    check=True)
     if sa.gpu_count() > 0:
-        pytest.skip("a GPU is present; the caller is run by the GPU tests")
+        pytest.skip("a GPU is present: tests/test_c_caller_gpu.py runs the caller on it")
     p = subprocess.run([exe], capture_output=True, text=True, timeout=60)
     assert p.returncode == 3, (p.returncode, p.stdout, p.stderr)
     assert "No GPU devices found" in p.stderr
