@@ -120,6 +120,21 @@ int rsd_advance(rsd_engine *e, const rsd_step *step);
  * engine's pulse buffer.  Enqueued; rsd_read_pulse waits for it. */
 int rsd_make_pulse(rsd_engine *e, const rsd_pulse *pulse);
 
+/* rsd_make_pulse without allocating: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same advance and
+ * pulse as rsd_make_pulse (a held-back advance is fused in), with its per-block
+ * partial sums and gate ranges in a 16 MiB array of each GPU that the library
+ * holds until the process ends.  The result, out[4 k + c] = (Ih, Qh, Iv, Qv) of
+ * gate k before the radar gain, goes to `out` (device memory, or host memory
+ * the device can write) and to the engine's pulse buffer (rsd_read_pulse).
+ * Never reorders the streams and does not append to the IQ ring.  The rain
+ * grid shrinks to the array's rows (4 range_count doubles each); a population
+ * whose debris blocks alone need more rows is rejected with RSD_ERR_ARG.
+ * Engines in a batch are rejected.  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU race. */
+int rsd_make_pulse_noalloc(rsd_engine *e, const rsd_pulse *pulse, double *out);
+
 /* Blocking read of the last pulse: out[4 k + c] = (Ih, Qh, Iv, Qv) of gate k. */
 int rsd_read_pulse(rsd_engine *e, double *out, uint32_t range_count);
 

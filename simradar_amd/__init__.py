@@ -78,7 +78,7 @@ EXPORTED_RS = [
 EXPORTED_RSD = [
     "rsd_device_count", "rsd_device_props", "rsd_open", "rsd_close", "rsd_last_error", "rsd_alloc_state",
     "rsd_upload_state", "rsd_download_state", "rsd_set_wind", "rsd_set_adm", "rsd_set_rcs", "rsd_set_drop_rcs", "rsd_set_dff",
-    "rsd_set_range_weight", "rsd_set_angular_weight", "rsd_advance", "rsd_make_pulse", "rsd_read_pulse",
+    "rsd_set_range_weight", "rsd_set_angular_weight", "rsd_advance", "rsd_make_pulse", "rsd_make_pulse_noalloc", "rsd_read_pulse",
     "rsd_sync", "rsd_timer_start", "rsd_timer_stop", "rsd_set_launch", "rsd_math_probe", "rsd_profile",
     "rsd_profile_read", "rsd_io_test", "rsd_set_sort", "rsd_get_sort", "rsd_set_fold", "rsd_set_iq_ring", "rsd_moments",
     "rsd_set_position_precision", "rsd_download_pos_lo", "rsd_batch_create", "rsd_batch_destroy",
@@ -241,6 +241,7 @@ def _declare(lib):
         "rsd_set_angular_weight": (I, [P, P, I, P]),
         "rsd_advance": (I, [P, ctypes.POINTER(RSDStep)]),
         "rsd_make_pulse": (I, [P, ctypes.POINTER(RSDPulse)]),
+        "rsd_make_pulse_noalloc": (I, [P, ctypes.POINTER(RSDPulse), P]),
         "rsd_read_pulse": (I, [P, P, ctypes.c_uint32]),
         "rsd_sync": (I, [P]),
         "rsd_timer_start": (I, [P]),

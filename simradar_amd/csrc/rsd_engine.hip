@@ -41,6 +41,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <math.h>
+#include <pthread.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -1812,6 +1813,39 @@ __global__ void k_view(ViewK a) {
 }
 
 // ---------------------------------------------------------------------------
+// Scratch of rsd_make_pulse_noalloc: one array per GPU, held until the process
+// ends.  Bytes [0, RSD_SCRATCH_GATE_BYTES) hold the binary32 gate ranges, the
+// rest the per-block binary64 partial gate sums.  Host code uses the address
+// hipGetSymbolAddress returns for the current device (scratch_dev), never the
+// array's name.
+// ---------------------------------------------------------------------------
+#define RSD_SCRATCH_BYTES (16u << 20)
+#define RSD_SCRATCH_GATE_BYTES (RSD_MAX_GATES * sizeof(float))
+#define RSD_SCRATCH_PARTIALS ((RSD_SCRATCH_BYTES - RSD_SCRATCH_GATE_BYTES) / sizeof(double))
+#define RSD_SCRATCH_DEVICES 64
+static __device__ __align__(256) unsigned char scratch[RSD_SCRATCH_BYTES];
+static void *scratch_dev[RSD_SCRATCH_DEVICES];
+static pthread_mutex_t scratch_lock = PTHREAD_MUTEX_INITIALIZER;
+
+// rs.cl:1189 / 1214: gate k's range as start plus k binary32 additions of delta.
+__global__ void k_gate_ranges(float start, float delta, int R, float *out) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        float r = start;
+        for (int k = 0; k < R; k++) {
+            out[k] = r;
+            r += delta;
+        }
+    }
+}
+
+// Where launch_step takes its scratch and output when no allocation is allowed.
+struct StaticPulse {
+    double *partials;       // RSD_SCRATCH_PARTIALS doubles
+    float *gate_r;          // RSD_MAX_GATES floats
+    double *out;            // 4 R doubles of the caller
+};
+
+// ---------------------------------------------------------------------------
 // Host side of the C-ABI.
 // ---------------------------------------------------------------------------
 struct rsd_engine {
@@ -2024,6 +2058,21 @@ extern "C" int rsd_open(int device_index, rsd_engine **out) {
         free(e);
         return RSD_ERR_HIP;
     }
+    // the device's scratch address for rsd_make_pulse_noalloc, once per device
+    if (device_index < RSD_SCRATCH_DEVICES) {
+        pthread_mutex_lock(&scratch_lock);
+        hipError_t st = hipSuccess;
+        if (!scratch_dev[device_index]) {
+            void *ptr = NULL;
+            st = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+            if (st == hipSuccess) scratch_dev[device_index] = ptr;
+        }
+        pthread_mutex_unlock(&scratch_lock);
+        if (st != hipSuccess) {
+            free(e);
+            return RSD_ERR_HIP;
+        }
+    }
     *out = e;
     return RSD_OK;
 }
@@ -2176,6 +2225,18 @@ static int ensure_partials(rsd_engine *e, size_t rows, int R) {
     return RSD_OK;
 }
 
+// Largest |start + k delta (binary32 sums) - (start + k delta)| over the gates.
+static double gate_drift_of(float start, float delta, int R) {
+    float r = start;
+    double drift = 0.0;
+    for (int k = 0; k < R; k++) {
+        double d = fabs((double)r - ((double)start + (double)k * (double)delta));
+        if (d > drift) drift = d;
+        r += delta;
+    }
+    return drift;
+}
+
 static int ensure_gates(rsd_engine *e, float start, float delta, int R) {
     if (e->gate_r && e->gate_R == R && e->gate_start == start && e->gate_delta == delta) return RSD_OK;
     float *h = (float *)malloc((size_t)R * sizeof(float));
@@ -2201,8 +2262,10 @@ static int ensure_gates(rsd_engine *e, float start, float delta, int R) {
     return RSD_OK;
 }
 
-// Support of the range weight in metres: outside [lo, hi] every gate weight is 0.
-static void range_support(const rsd_engine *e, float delta, float *lo, float *hi, int *dense) {
+// Support of the range weight in metres: outside [lo, hi] every gate weight is 0
+// (gates start + k delta, k < R, summed in binary32 with the given drift).
+static void range_support(const rsd_engine *e, float start, float delta, int gates, double drift, float *lo, float *hi,
+                          int *dense) {
     const int n = e->rangew_n;
     const float *W = e->rangew_h;
     const float xs = e->rangew_desc.s[0], x0 = e->rangew_desc.s[1];
@@ -2233,16 +2296,23 @@ static void range_support(const rsd_engine *e, float delta, float *lo, float *hi
     }
     // Margin for the drift of the summed gate ranges and the rounding of the
     // binary32 gate-index bounds computed from r in the kernels.
-    const double R = (double)e->gate_R;
-    const double pad = e->gate_drift + 1e-3 * delta + 1e-6 * (fabs((double)e->gate_start) + R * delta) + 1e-3;
+    const double R = (double)gates;
+    const double pad = drift + 1e-3 * delta + 1e-6 * (fabs((double)start) + R * delta) + 1e-3;
     *lo = (float)(((double)(f - 1) - x0) / xs - pad);
     *hi = (float)(((double)(l + 1) - x0) / xs + pad);
 }
 
-static int fill_pulse(rsd_engine *e, const rsd_pulse *p, PulseK &k) {
+// Pulse arguments; sp != NULL: gate ranges in the scratch (no allocation).
+static int fill_pulse(rsd_engine *e, const rsd_pulse *p, PulseK &k, const StaticPulse *sp = NULL) {
     if (!e->angw || !e->rangew) return fail(e, RSD_ERR_STATE, "weight tables not set");
-    int st = ensure_gates(e, p->range_start, p->range_delta, (int)p->range_count);
-    if (st) return st;
+    double drift;
+    if (sp) {
+        drift = gate_drift_of(p->range_start, p->range_delta, (int)p->range_count);
+    } else {
+        int st = ensure_gates(e, p->range_start, p->range_delta, (int)p->range_count);
+        if (st) return st;
+        drift = e->gate_drift;
+    }
     memcpy(k.sd.s, p->sim_desc, sizeof(k.sd.s));
     k.two_over_lambda = p->two_over_lambda;
     k.angw = e->angw;
@@ -2250,11 +2320,11 @@ static int fill_pulse(rsd_engine *e, const rsd_pulse *p, PulseK &k) {
     k.rangew = e->rangew;
     k.rangew_n = e->rangew_n;
     k.rangew_desc = e->rangew_desc;
-    k.gate_r = e->gate_r;
+    k.gate_r = sp ? sp->gate_r : e->gate_r;
     k.R = (int)p->range_count;
     k.gate_start = p->range_start;
     k.gate_inv_delta = p->range_delta != 0.0f ? 1.0f / p->range_delta : 0.0f;
-    range_support(e, p->range_delta, &k.sup_lo, &k.sup_hi, &k.dense);
+    range_support(e, p->range_start, p->range_delta, k.R, drift, &k.sup_lo, &k.sup_hi, &k.dense);
     k.tri = e->rangew_n == 3 && e->rangew_h[0] == 0.0f && e->rangew_h[1] == 1.0f && e->rangew_h[2] == 0.0f &&
             e->rangew_desc.s[2] == 2.0f;
     k.tri_s0 = e->rangew_desc.s[0];
@@ -2270,7 +2340,7 @@ static int fill_pulse(rsd_engine *e, const rsd_pulse *p, PulseK &k) {
         (k.dense || (k.sup_hi - k.sup_lo) * k.gate_inv_delta >= (float)(RSD_WIN - 2) ||
          (size_t)k.R * (4 * sizeof(double) + sizeof(float)) + (size_t)k.rangew_n * sizeof(float2) > 96 * 1024))
         k.fold = RSD_FOLD_WAVE32;
-    k.partials = e->partials;
+    k.partials = sp ? sp->partials : e->partials;
     k.row0 = 0;
     return RSD_OK;
 }
@@ -2354,7 +2424,10 @@ static bool debris_rides(const rsd_engine *e, int k) {
 }
 
 // Launch the kernels of one step: advance (s != NULL) and/or pulse (p != NULL).
-static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p) {
+// sp != NULL: the pulse's partial sums and gate ranges go to the scratch, the
+// rain grid shrinks to the scratch's rows, the result goes to sp->out (and the
+// engine's pulse buffer) and nothing is allocated.
+static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p, const StaticPulse *sp = NULL) {
     HIPCHK(e, hipSetDevice(e->device));
     join_batch(e);
     const int R = p ? (int)p->range_count : 0;
@@ -2394,6 +2467,16 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p) {
         const size_t need = (e->n_rain + 63) / 64;   // at least a wave per block
         rain_blocks = round < 1 ? 1 : ((size_t)round < need ? round : (int)need);
     }
+    if (sp && p) {
+        // rows of the scratch at 4 R doubles each; the debris keep theirs
+        size_t deb_rows = 0;
+        for (int k = 1; k < RSD_MAX_TYPES; k++) deb_rows += (size_t)deb_blocks[k];
+        const size_t cap_rows = RSD_SCRATCH_PARTIALS / (4 * (size_t)R);
+        if (deb_rows + (e->n_rain ? 1 : 0) > cap_rows)
+            return fail(e, RSD_ERR_ARG, "the scratch holds %zu partial rows of %d gates; %zu debris blocks need more "
+                        "(at most %u bytes of scratch)", cap_rows, R, deb_rows, (unsigned)RSD_SCRATCH_BYTES);
+        if (e->n_rain && (size_t)rain_blocks > cap_rows - deb_rows) rain_blocks = (int)(cap_rows - deb_rows);
+    }
     int rows = rain_blocks, row_of[RSD_MAX_TYPES] = {0};
     for (int k = 1; k < RSD_MAX_TYPES; k++) {
         row_of[k] = rows;
@@ -2402,11 +2485,16 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p) {
     PulseK pk;
     memset(&pk, 0, sizeof(pk));
     if (p) {
-        int st = ensure_partials(e, (size_t)rows, R);
+        int st = sp ? RSD_OK : ensure_partials(e, (size_t)rows, R);
         if (st) return st;
-        st = fill_pulse(e, p, pk);
+        st = fill_pulse(e, p, pk, sp);
         if (st) return st;
         pk.rows = rows;
+        if (sp) {
+            hipLaunchKernelGGL(k_gate_ranges, dim3(1), dim3(64), 0, e->stream, p->range_start, p->range_delta, R,
+                               sp->gate_r);
+            HIPCHK(e, hipGetLastError());
+        }
     }
     const size_t lds = p ? (size_t)4 * R * sizeof(double) + (pk.fold == RSD_FOLD_WINDOW ? (size_t)e->rangew_n * sizeof(float2) + (size_t)R * sizeof(float) : 0) : 0;
     if (any_debris) {
@@ -2490,7 +2578,14 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p) {
         }
     }
     if (any_debris) HIPCHK(e, hipStreamWaitEvent(e->stream, e->ev_join, 0));
-    if (p) {
+    if (p && sp) {
+        hipLaunchKernelGGL(k_finalize, dim3(4 * R), dim3(256), 0, e->stream, (const double *)sp->partials, rows, 4 * R,
+                           sp->out, e->pulse_d);
+        HIPCHK(e, hipGetLastError());
+        e->last_pulse = *p;
+        e->have_pulse = 1;
+        e->sigaux_saved = 0;
+    } else if (p) {
         double *slot = NULL;
         if (e->ring && e->ring_R == R) {
             slot = e->ring + (size_t)e->ring_next * 4 * R;
@@ -3028,6 +3123,26 @@ extern "C" int rsd_make_pulse(rsd_engine *e, const rsd_pulse *p) {
         return launch_step(e, &e->pending, p);
     }
     return launch_step(e, NULL, p);
+}
+
+extern "C" int rsd_make_pulse_noalloc(rsd_engine *e, const rsd_pulse *p, double *out) {
+    if (!e || !p || !out) return fail(e, RSD_ERR_ARG, "null engine, pulse or output");
+    if (!e->pos) return fail(e, RSD_ERR_STATE, "state not allocated");
+    if (e->batch) return fail(e, RSD_ERR_STATE, "engine is in a batch");
+    if (p->range_count < 1 || p->range_count > RSD_MAX_GATES)
+        return fail(e, RSD_ERR_ARG, "range_count %u out of range (1..%d)", p->range_count, RSD_MAX_GATES);
+    if (e->device >= RSD_SCRATCH_DEVICES || !scratch_dev[e->device])
+        return fail(e, RSD_ERR_STATE, "no scratch address for device %d", e->device);
+    StaticPulse sp;
+    sp.gate_r = (float *)scratch_dev[e->device];
+    sp.partials = (double *)((unsigned char *)scratch_dev[e->device] + RSD_SCRATCH_GATE_BYTES);
+    sp.out = out;
+    e->since_sort++;
+    if (e->have_pending) {
+        e->have_pending = 0;
+        return launch_step(e, &e->pending, p, &sp);
+    }
+    return launch_step(e, NULL, p, &sp);
 }
 
 extern "C" int rsd_read_pulse(rsd_engine *e, double *out, uint32_t R) {
