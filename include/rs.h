@@ -369,6 +369,9 @@ void RS_set_scan_box_values(RSHandle *H, const float *box8);
 /* addition: host LES volume `frame` of the handle's provider (float4 texels,
  * x fastest) and its dimensions; NULL when there is none. */
 const float *RS_get_les_frame(RSHandle *H, int frame, uint32_t *dims3);
+/* addition: the (cn2, p, _, _) volume of an LES frame, laid out as
+ * RS_get_les_frame (made for a synthetic frame on the first request). */
+const float *RS_get_les_cpxx(RSHandle *H, int frame, uint32_t *dims3);
 /* addition: float4 texels of the t-th debris object's tables as uploaded
  * (which: 0 ADM cd, 1 ADM cm, 2 RCS real, 3 RCS imag); dims2 = (nx, ny).
  * out4 may be NULL to query the size; returns 0 on success. */

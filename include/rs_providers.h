@@ -69,6 +69,9 @@ LESHandle LES_init_with_config_path(const LESConfig config, const char *path);
 LESHandle LES_init(void);
 void LES_free(LESHandle);
 LESTable *LES_get_frame(const LESHandle, const int n);
+/* The frame's (cn2, p, 0, 0) volume, made on the first request for a synthetic
+ * frame (the synthetic Cn^2 profile is documented in rs_providers.c). */
+const LESFloat4 *LES_get_cpxx(const LESHandle, LESTable *t);
 float LES_get_table_period(const LESHandle);
 size_t LES_get_table_count(const LESHandle);
 /* Synthetic generator parameters (seed of the turbulence, number of frames). */

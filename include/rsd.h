@@ -54,6 +54,8 @@ typedef struct rsd_engine rsd_engine;
 /* Advance kind of the background (type 0) scatterers. */
 #define RSD_ADVANCE_DRAGGED     1u   /* el_atts (concept D, rs.cl:653-739) */
 #define RSD_ADVANCE_BACKGROUND  0u   /* bg_atts (rs.cl:533-580) */
+#define RSD_ADVANCE_FIXED       2u   /* fp_atts (concept F, rs.cl:590-648): fixed positions,
+                                        rcs from the Cn^2 volume with an accumulated phase */
 
 /* Precision of the phase term of the pulse. */
 #define RSD_PHASE_F64           0u   /* 4 pi r / lambda reduced mod 2 pi in binary64 */
@@ -101,6 +103,10 @@ int rsd_download_state(rsd_engine *e, float *pos, float *vel, float *ori, float 
 
 /* Tables.  Texels are float4, x fastest.  Descriptors use the reference slots. */
 int rsd_set_wind(rsd_engine *e, int slot, const float *uvwt, int nx, int ny, int nz, const float *les_desc16);
+/* The (cn2, p, _, _) volume of the LES slot (the second image RS_set_vel_data
+ * uploads, rs.c:2925), read with the slot's wind descriptor by the concept-F
+ * advance; same shape as the slot's wind table.  Waits for the device. */
+int rsd_set_cpxx(rsd_engine *e, int slot, const float *cpxx, int nx, int ny, int nz);
 int rsd_set_adm(rsd_engine *e, int table, const float *cd, const float *cm, int nx, int ny, const float *adm_desc16);
 int rsd_set_rcs(rsd_engine *e, int table, const float *re, const float *im, int nx, int ny, const float *rcs_desc16);
 int rsd_set_drop_rcs(rsd_engine *e, const float *table4, int n, const float *desc4);

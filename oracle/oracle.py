@@ -47,6 +47,7 @@ def lib():
             "orc_el_atts": (None, [P, P, P, P, S, S, T, P, P, P, P]),
             "orc_bg_atts": (None, [P, P, P, P, S, S, T, P, P, P, P]),
             "orc_db_atts": (None, [P, P, P, P, P, P, S, S, T, P, T, T, P, T, T, P, P]),
+            "orc_fp_atts": (None, [P, P, P, S, S, T, T, P, P]),
             "orc_set_dff": (None, [P, ctypes.c_int, P]),
             "orc_dff_place_probe": (ctypes.c_int, [F4, P, P]),
             "orc_dff_icdf_from_pdf": (ctypes.c_int, [P, ctypes.c_int, P, ctypes.c_int]),
@@ -180,6 +181,11 @@ def el_atts(st, begin, end, wind_tex, les_desc, drop, drop_desc, sim_desc):
 def bg_atts(st, begin, end, wind_tex, les_desc, drop, drop_desc, sim_desc):
     lib().orc_bg_atts(_p(st.pos), _p(st.vel), _p(st.rcs), _p(st.rnd), begin, end, ctypes.byref(wind_tex),
                       _p(f32(les_desc)), _p(f32(drop)), _p(f32(drop_desc)), _p(f32(sim_desc)))
+
+
+def fp_atts(st, begin, end, wind_tex, cpxx_tex, les_desc, sim_desc):
+    lib().orc_fp_atts(_p(st.pos), _p(st.vel), _p(st.rcs), begin, end, ctypes.byref(wind_tex), ctypes.byref(cpxx_tex),
+                      _p(f32(les_desc)), _p(f32(sim_desc)))
 
 
 def db_atts(st, begin, end, wind_tex, les_desc, cd_tex, cm_tex, adm_desc, re_tex, im_tex, rcs_desc, sim_desc):

@@ -296,6 +296,37 @@ void orc_bg_atts(of4 *p, of4 *v, of4 *x, ou4 *y, size_t begin, size_t end,
     }
 }
 
+/* rs.cl:590-648 -- fp_atts over [begin, end): positions fixed, vel = wind at
+ * the position, phase turned by the radial wind over dt, rcs = (a cos phi,
+ * a sin phi, cn2, phi) with a = 10^cn2 from the (cn2, p, _, _) volume. */
+void orc_fp_atts(of4 *p, of4 *v, of4 *x, size_t begin, size_t end, const orc_tex *wind, const orc_tex *cpxx,
+                 const float *les_desc, const float *sd) {
+    const float wav_num = sd[4], dt = sd[11];
+    for (size_t i = begin; i < end; i++) {
+        const of4 pos = p[i];
+        of4 vel = v[i], rcs = x[i];
+        of4 c = orc_wind_index(pos, les_desc, sd);
+        of4 uvwt = orc_sample3d(wind, c.x, c.y, c.z);
+        of4 cp = orc_sample3d(cpxx, c.x, c.y, c.z);
+        float len = of_len3(pos.x, pos.y, pos.z);
+        float nx = pos.x / len, ny = pos.y / len, nz = pos.z / len;
+        float d = (nx * uvwt.x + ny * uvwt.y) + nz * uvwt.z;
+        float phi = rcs.w - (wav_num * d) * dt;
+        float cn2 = cp.x;
+        float a = of_powf_pos(10.0f, cn2);
+        float s = of_sinf(phi), cc = of_cosf(phi);
+        rcs.x = a * cc;
+        rcs.y = a * s;
+        rcs.z = cn2;
+        rcs.w = of_atan2f(s, cc);
+        vel.x = uvwt.x;
+        vel.y = uvwt.y;
+        vel.z = uvwt.z;
+        v[i] = vel;
+        x[i] = rcs;
+    }
+}
+
 /* Debris flux field (concept C): a copy of the inverse CDF and its
  * descriptor (rs.c:3392-3404 layout); count 0 switches the placement off. */
 static float *orc_dff_tab = NULL;

@@ -57,6 +57,10 @@ void orc_el_atts(of4 *p, of4 *v, of4 *x, ou4 *y, size_t begin, size_t end,
 void orc_bg_atts(of4 *p, of4 *v, of4 *x, ou4 *y, size_t begin, size_t end,
                  const orc_tex *wind, const float *les_desc,
                  const of4 *drop_rcs, const float *drop_desc4, const float *sim_desc);
+/* rs.cl:590-648 -- fp_atts over [begin, end) (concept F): cpxx is the
+ * (cn2, p, _, _) volume sampled like the wind. */
+void orc_fp_atts(of4 *p, of4 *v, of4 *x, size_t begin, size_t end, const orc_tex *wind, const orc_tex *cpxx,
+                 const float *les_desc, const float *sim_desc);
 /* rs.cl:744-907 -- db_atts over [begin, end) (debris flux field concept not covered). */
 void orc_set_dff(const float *icdf, int count, const float *desc16);
 int orc_dff_icdf_from_pdf(const float *pdf, int pdf_count, float *tab, int n);

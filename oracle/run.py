@@ -11,6 +11,7 @@ from . import oracle as orc
 CONCEPT_DRAGGED = 1 << 1
 CONCEPT_TRANSPARENT = 1 << 2
 CONCEPT_UNIFORM_DSD = 1 << 4
+CONCEPT_FIXED = 1 << 5
 
 
 def adm_desc(nb, na, phys):
@@ -66,8 +67,10 @@ class OracleRun:
         p = r.params()
         self.params = p
         sd = r.sim_desc()
+        self.fixed = bool(cfg.concept & CONCEPT_FIXED) and not cfg.concept & CONCEPT_DRAGGED
+        self.cpxx_tex = None
         if wind is None:
-            self.set_les_frame(r.les_frame(0))
+            self.set_les_frame(r.les_frame(0), r.les_cpxx(0) if self.fixed else None)
         else:
             self.wind_tex, self._wind_keep = orc.tex(wind[0])
             self.les_desc = np.asarray(wind[1], np.float32)
@@ -93,9 +96,12 @@ class OracleRun:
             tx = [orc.tex(a) for a in (cd, cm, re, im)]
             self.deb.append(dict(tex=tx, adm_desc=ad, rcs_desc=rd))
 
-    def set_les_frame(self, frame):
+    def set_les_frame(self, frame, cpxx=None):
         """Wind of later advances: one suctvort-geometry LES frame (stretched
-        grid, les.c:164-170), as the host uploads at a volume switch."""
+        grid, les.c:164-170), as the host uploads at a volume switch; cpxx its
+        (cn2, p, _, _) volume for the fixed-position background."""
+        if cpxx is not None:
+            self.cpxx_tex, self._cpxx_keep = orc.tex(cpxx)
         self.wind_tex, self._wind_keep = orc.tex(frame)
         self.les_desc = orc.les_desc(True, frame.shape[2], frame.shape[1], frame.shape[0], 2.0, 2.0, 2.0,
                                      1.0212, 1.0212, 1.05, 50.0)
@@ -113,9 +119,13 @@ class OracleRun:
 
     def advance(self, sim_desc):
         n0 = self.counts[0]
-        atts = orc.el_atts if self.cfg.concept & CONCEPT_DRAGGED else orc.bg_atts
-        self._run(lambda lo, hi: atts(self.st, lo, hi, self.wind_tex, self.les_desc, self.drop, self.drop_desc,
-                                      sim_desc), 0, n0)
+        if self.fixed:      # rs.c:5343-5347: fp_atts under F without D
+            self._run(lambda lo, hi: orc.fp_atts(self.st, lo, hi, self.wind_tex, self.cpxx_tex, self.les_desc,
+                                                 sim_desc), 0, n0)
+        else:
+            atts = orc.el_atts if self.cfg.concept & CONCEPT_DRAGGED else orc.bg_atts
+            self._run(lambda lo, hi: atts(self.st, lo, hi, self.wind_tex, self.les_desc, self.drop, self.drop_desc,
+                                          sim_desc), 0, n0)
         for t, d in enumerate(self.deb):
             k = t + 1
             b, e = self.origins[k], self.origins[k] + self.counts[k]

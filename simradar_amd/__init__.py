@@ -73,11 +73,11 @@ EXPORTED_RS = [
     "RS_init_host_only", "RS_set_population", "RS_set_phase_precision", "RS_get_pulse_f64",
     "RS_set_moment_pulses", "RS_get_moments", "RS_set_range_cap", "RS_batch_round",
     "RS_get_descriptor", "RS_radar_gain", "RS_engine", "RS_get_info", "RS_get_host_stream", "RS_get_params",
-    "RS_get_dsd", "RS_suggest_scan_domain_values", "RS_set_scan_box_values", "RS_get_les_frame", "RS_get_debris_table",
+    "RS_get_dsd", "RS_suggest_scan_domain_values", "RS_set_scan_box_values", "RS_get_les_frame", "RS_get_les_cpxx", "RS_get_debris_table",
 ]
 EXPORTED_RSD = [
     "rsd_device_count", "rsd_device_props", "rsd_open", "rsd_close", "rsd_last_error", "rsd_alloc_state",
-    "rsd_upload_state", "rsd_download_state", "rsd_set_wind", "rsd_set_adm", "rsd_set_rcs", "rsd_set_drop_rcs", "rsd_set_dff",
+    "rsd_upload_state", "rsd_download_state", "rsd_set_wind", "rsd_set_cpxx", "rsd_set_adm", "rsd_set_rcs", "rsd_set_drop_rcs", "rsd_set_dff",
     "rsd_set_range_weight", "rsd_set_angular_weight", "rsd_advance", "rsd_make_pulse", "rsd_make_pulse_noalloc", "rsd_read_pulse",
     "rsd_sync", "rsd_timer_start", "rsd_timer_stop", "rsd_set_launch", "rsd_math_probe", "rsd_profile",
     "rsd_profile_read", "rsd_io_test", "rsd_set_sort", "rsd_get_sort", "rsd_set_fold", "rsd_set_iq_ring", "rsd_moments",
@@ -222,6 +222,7 @@ def _declare(lib):
         "RS_get_dsd": (I, [P, P, P, P, I]),
         "RS_suggest_scan_domain_values": (None, [P, P]),
         "RS_get_les_frame": (P, [P, I, P]),
+        "RS_get_les_cpxx": (P, [P, I, P]),
         "RS_get_debris_table": (I, [P, I, I, P, P]),
         "RS_set_scan_box_values": (None, [P, P]),
         "rsd_device_count": (I, []),
@@ -233,6 +234,7 @@ def _declare(lib):
         "rsd_upload_state": (I, [P, P, P, P, P, P, P, P]),
         "rsd_download_state": (I, [P, P, P, P, P, P, P, P, P]),
         "rsd_set_wind": (I, [P, I, P, I, I, I, P]),
+        "rsd_set_cpxx": (I, [P, I, P, I, I, I]),
         "rsd_set_adm": (I, [P, I, P, P, I, I, P]),
         "rsd_set_rcs": (I, [P, I, P, P, I, I, P]),
         "rsd_set_drop_rcs": (I, [P, P, I, P]),
@@ -432,6 +434,16 @@ class Radar:
     def les_frame(self, frame):
         dims = np.zeros(3, np.uint32)
         p = self.lib.RS_get_les_frame(self.H, frame, _ptr(dims))
+        if not p:
+            return None
+        nx, ny, nz = (int(v) for v in dims)
+        buf = (ctypes.c_char * (nx * ny * nz * 16)).from_address(p)
+        return np.frombuffer(buf, dtype=np.float32).reshape(nz, ny, nx, 4).copy()
+
+    def les_cpxx(self, frame):
+        """(nz, ny, nx, 4) float32 (cn2, p, _, _) volume of an LES frame."""
+        dims = np.zeros(3, np.uint32)
+        p = self.lib.RS_get_les_cpxx(self.H, frame, _ptr(dims))
         if not p:
             return None
         nx, ny, nz = (int(v) for v in dims)

@@ -412,9 +412,44 @@ void RS_set_scan_extent(RSHandle *H,
     H->params.range_count = MIN(RS_MAX_GATES, nr);
 
     if (H->sim_concept & RSSimulationConceptVerticallyPointingRadar) {
-        rsprint("ERROR: Vertically pointing radar (concept V) is not part of this engine.");
-        return;
-    }
+        /* rs.c:1602-1690: a profiler's raster of zenith offsets (xx, yy) in
+         * degrees, spacing min(elevation, azimuth delta), out to 90 - el_lo;
+         * each point gives az = atan2(yy, xx) (radians), el = 90 - |(xx, yy)| */
+        const RSfloat delta = MIN(elevation_delta, azimuth_delta);
+        const RSfloat edge = el_lo;
+        RSfloat xx, yy = edge - 90.0f;
+        while (yy < (90.0f - edge) + tiny) {
+            xx = el_lo - 90.0f;
+            while (xx < (90.0f - edge) + tiny) {
+                xx += delta;
+                ii++;
+            }
+            yy += delta;
+        }
+        H->num_anchors = 2 * ii + 1;
+        free(H->anchor_pos);
+        H->anchor_pos = (cl_float4 *)malloc((size_t)H->num_anchors * sizeof(cl_float4));
+        if (!H->anchor_pos) {
+            rsprint("ERROR: Unable to allocate memory for anchors.");
+            return;
+        }
+        ii = 0;
+        yy = edge - 90.0f;
+        while (yy < (90.0f - edge) + tiny) {
+            xx = edge - 90.0f;
+            while (xx < (90.0f - edge) + tiny) {
+                az = atan2f(yy, xx);
+                el = 90.0f - sqrtf(yy * yy + xx * xx);
+                el = el / 180.0f * M_PI;
+                anchor_add(H, ii++, r_lo, el, az, &xmin, &xmax, &ymin, &ymax, &zmin, &zmax);
+                anchor_add(H, ii++, r_hi, el, az, &xmin, &xmax, &ymin, &ymax, &zmin, &zmax);
+                xx += delta;
+            }
+            yy += delta;
+        }
+        naz = 360;
+        nel = H->P ? (int)((POSPattern *)H->P)->count : 1;
+    } else {
     az = az_lo;
     while (az <= az_hi + tiny) {
         az += P->azimuth_delta_deg;
@@ -446,6 +481,7 @@ void RS_set_scan_extent(RSHandle *H,
             az += azimuth_delta / 180.0f * M_PI;
         }
         el += elevation_delta / 180.0f * M_PI;
+    }
     }
     H->anchor_pos[ii].x = 0.0f;
     H->anchor_pos[ii].y = 0.0f;
@@ -523,6 +559,16 @@ void RS_set_phase_precision(RSHandle *H, uint32_t phase_mode) {
 
 /* rs.c:2134-2211 */
 void RS_revise_population(RSHandle *H) {
+    if (H->sim_concept & RSSimulationConceptVerticallyPointingRadar) {
+        /* rs.c:2143-2157: one scatterer per anchor of a plane and gate */
+        const size_t per_plane = H->num_anchors > 0 ? (H->num_anchors - 1) / 2 : 0;
+        H->counts[0] = H->population_override ? H->population_override : H->params.range_count * per_plane;
+        H->num_scats = H->counts[0];
+        for (int k = 1; k < RS_MAX_DEBRIS_TYPES; k++) H->num_scats += H->counts[k];
+        if (H->verb) rsprint("Anchors / Plane = %s", commaint((long long)per_plane));
+        H->status |= RSStatusPopulationDefined;
+        return;
+    }
     const size_t max_wg = 1024;
     const size_t mul = (size_t)H->num_cus[0] * H->num_workers * max_wg * 2;
     RSfloat r = H->params.range_start;
@@ -723,6 +769,21 @@ void RS_set_vel_data(RSHandle *H, const RSTable3D table) {
     }
 }
 
+/* The (cn2, p, 0, 0) volume of an LES frame into the active wind slot (the
+ * second image of RS_set_vel_data, rs.c:2925); only fp_atts (concept F) reads
+ * it, so it is uploaded under concept F only. */
+static void upload_cpxx(RSHandle *H, const LESTable *leslie) {
+    if (!has_engine(H) || !H->L || !(H->sim_concept & RSSimulationConceptFixedScattererPosition)) return;
+    const LESFloat4 *c = LES_get_cpxx(H->L, (LESTable *)leslie);
+    if (!c) {
+        rsprint("ERROR: No Cn^2 volume for the LES frame.");
+        return;
+    }
+    RSWorker *C = &H->workers[0];
+    engine_ok(H, rsd_set_cpxx(C->engine, (int)C->les_id, (const float *)c, (int)leslie->nx, (int)leslie->ny,
+                              (int)leslie->nz), "Cn2 volume upload");
+}
+
 void RS_set_vel_data_to_LES_table(RSHandle *H, const LESTable *leslie) {
     if (!leslie) {
         rsprint("ERROR: LES input data cannot be NULL.");
@@ -748,6 +809,7 @@ void RS_set_vel_data_to_LES_table(RSHandle *H, const LESTable *leslie) {
         rsprint("WARNING: Debris flux field (concept C) is not part of this engine; uniform respawn is used.");
     }
     RS_set_vel_data(H, t);
+    upload_cpxx(H, leslie);
     H->vel_desc = *leslie;
     memset(&H->vel_desc.data, 0, sizeof(LESValue));
     H->vel_desc.uvwt = NULL;
@@ -1392,6 +1454,66 @@ static void populate_streams(RSHandle *H) {
     }
 }
 
+/* rs.c:4574-4647: concept F.  The low anchors (even indices) give one plane of
+ * directions scaled to range_start; the plane is repeated at every gate range
+ * start + n delta.  Each scatterer: random age, rcs (1e-10, 0, 1e-10, phase
+ * uniform in [0, 2 pi)), four random seeds.  Debris are not emulated. */
+static int populate_fixed(RSHandle *H) {
+    const size_t per_plane = H->num_anchors > 0 ? (H->num_anchors - 1) / 2 : 0;
+    if (H->num_types > 1) {
+        rsprint("ERROR: Debris are not emulated with fixed scatterer positions (concept F).");
+        return 0;
+    }
+    if (per_plane == 0 || H->counts[0] != H->params.range_count * per_plane) {
+        rsprint("ERROR: Concept F needs %u gates x %zu anchors = %zu scatterers, not %zu.", H->params.range_count,
+                per_plane, (size_t)H->params.range_count * per_plane, H->counts[0]);
+        return 0;
+    }
+    srand(H->random_seed);
+    const cl_float4 *A = H->anchor_pos;
+    const RSfloat r_lo = sqrtf(A[0].x * A[0].x + A[0].y * A[0].y + A[0].z * A[0].z);
+    const RSfloat rs = H->params.range_start, dr = H->params.range_delta;
+    size_t i = 0;
+    for (size_t k = 0; k + 1 < (size_t)H->num_anchors; k += 2, i++) {
+        H->scat_pos[i].x = A[k].x / r_lo * rs;
+        H->scat_pos[i].y = A[k].y / r_lo * rs;
+        H->scat_pos[i].z = A[k].z / r_lo * rs;
+        H->scat_pos[i].w = 0.0f;
+    }
+    uint32_t uid = 0;
+    i = 0;
+    for (uint32_t n = 0; n < H->params.range_count; n++) {
+        for (size_t k = 0; k < per_plane; k++, i++) {
+            H->scat_uid[i].s0 = uid++;
+            H->scat_uid[i].s1 = (cl_uint)H->num_scats;
+            H->scat_uid[i].s2 = (cl_uint)k;
+            H->scat_uid[i].s3 = 0;
+            H->scat_pos[i].x = H->scat_pos[k].x / rs * (rs + (float)n * dr);
+            H->scat_pos[i].y = H->scat_pos[k].y / rs * (rs + (float)n * dr);
+            H->scat_pos[i].z = H->scat_pos[k].z / rs * (rs + (float)n * dr);
+            H->scat_pos[i].w = 0.0f;
+            H->scat_aux[i].s0 = 0.0f;
+            H->scat_aux[i].s1 = (float)rand() / RAND_MAX;
+            H->scat_aux[i].s2 = 0.0f;
+            H->scat_aux[i].s3 = 1.0f;
+            H->scat_vel[i].x = H->scat_vel[i].y = H->scat_vel[i].z = H->scat_vel[i].w = 0.0f;
+            H->scat_ori[i].x = H->scat_ori[i].y = H->scat_ori[i].z = 0.0f;
+            H->scat_ori[i].w = 1.0f;
+            H->scat_tum[i].x = H->scat_tum[i].y = H->scat_tum[i].z = 0.0f;
+            H->scat_tum[i].w = 1.0f;
+            H->scat_rcs[i].s0 = 1.0e-10f;
+            H->scat_rcs[i].s1 = 0.0f;
+            H->scat_rcs[i].s2 = 1.0e-10f;
+            H->scat_rcs[i].s3 = (float)rand() / RAND_MAX * 2.0 * M_PI;
+            H->scat_rnd[i].s0 = (cl_uint)rand();
+            H->scat_rnd[i].s1 = (cl_uint)rand();
+            H->scat_rnd[i].s2 = (cl_uint)rand();
+            H->scat_rnd[i].s3 = (cl_uint)rand();
+        }
+    }
+    return 1;
+}
+
 void RS_populate(RSHandle *H) {
     if (H->num_scats > RS_MAX_NUM_SCATS) {
         rsprint("ERROR: Too many scatterers (%s).", commaint((long long)H->num_scats));
@@ -1419,10 +1541,12 @@ void RS_populate(RSHandle *H) {
     if (!alloc_host_streams(H)) return;
     RS_update_origins_offsets(H);
     if (H->sim_concept & RSSimulationConceptFixedScattererPosition) {
-        rsprint("ERROR: Fixed scatterer positions (concept F) are not part of this engine.");
-        return;
+        if (!populate_fixed(H)) return;
+        /* the frame in the wind slot (concept F may have been set after it) */
+        if (H->L && H->vel_count) upload_cpxx(H, LES_get_frame(H->L, (int)((H->vel_idx + H->vel_count - 1) % H->vel_count)));
+    } else {
+        populate_streams(H);
     }
-    populate_streams(H);
     H->sim_tic = 0.0f;
     H->sim_toc = H->vel_desc.tp;
     H->sim_desc.s[RSSimulationDescriptionBeamUnitX] = 0.0f;
@@ -1540,7 +1664,10 @@ void RS_advance_time(RSHandle *H) {
         memcpy(s.sim_desc, H->sim_desc.s, sizeof(s.sim_desc));
         s.wind_slot = (int32_t)H->workers[0].les_id;
         fill_table_indices(H, s.adm_index, s.rcs_index);
-        s.background_kind = (H->sim_concept & RSSimulationConceptDraggedBackground) ? RSD_ADVANCE_DRAGGED : RSD_ADVANCE_BACKGROUND;
+        /* rs.c:5337-5350: el_atts under D, else fp_atts under F, else bg_atts */
+        s.background_kind = (H->sim_concept & RSSimulationConceptDraggedBackground) ? RSD_ADVANCE_DRAGGED
+                            : (H->sim_concept & RSSimulationConceptFixedScattererPosition) ? RSD_ADVANCE_FIXED
+                                                                                           : RSD_ADVANCE_BACKGROUND;
         engine_ok(H, rsd_advance(H->workers[0].engine, &s), "advance");
     }
     H->sim_tic += H->params.prt;
@@ -1763,11 +1890,30 @@ RSBox RS_suggest_scan_domain(RSHandle *H) {
         w = 0.5f * H->vel_desc.nx * H->vel_desc.rx;
         h = H->vel_desc.nz * H->vel_desc.rz;
     }
-    (void)h;
     if (H->sim_concept & RSSimulationConceptVerticallyPointingRadar) {
-        rsprint("ERROR: Vertically pointing radar (concept V) is not part of this engine.");
+        /* rs.c:5830-5866: elevations of the pattern, ranges 100 m .. 2 km */
+        const POSPattern *scan = (const POSPattern *)H->P;
+        float emin = 90.0f, emax = 0.0f;
+        const float rmin = 100.0f;
+        const float rmax = ceilf((2000.0f - rmin) / (2.0f * H->params.range_delta)) * 2.0f * H->params.range_delta;
+        for (uint32_t j = 0; j < scan->count; j++) {
+            emin = MIN(emin, scan->positions[j].el);
+            emax = MAX(emax, scan->positions[j].el);
+        }
+        if (rmax * cos(emin / 180.0f * M_PI) > w)
+            rsprint("WARNING. Elevation %.2f at %.2f m exceeds the LES domain width = %.2f.", emin, rmax, w);
+        if (rmax * sin(emax / 180.0f * M_PI) > h)
+            rsprint("WARNING. Elevation %.2f at %.2f m exceeds the LES domain height = %.2f.", emax, rmax, h);
+        const float nr = ceilf((rmax - rmin) / H->params.dr * 0.5f) * 2.0f;
+        box.origin.a = 0.0f;
+        box.size.a = 360.0f;
+        box.origin.e = emin;
+        box.size.e = emax - emin;
+        box.origin.r = rmax - (nr + RS_DOMAIN_PAD - 1.0f) * H->params.dr;
+        box.size.r = floorf(nr - RS_DOMAIN_PAD - 1.0f) * H->params.dr;
         return box;
     }
+    (void)h;
     const int nbeams = 16;
     float na = 0.5f * (float)nbeams + RS_DOMAIN_PAD + 0.5f;
     float ne = 18.0f;
@@ -1798,6 +1944,18 @@ const float *RS_get_les_frame(RSHandle *H, int frame, uint32_t *dims3) {
     dims3[1] = t->ny;
     dims3[2] = t->nz;
     return (const float *)t->uvwt;
+}
+
+const float *RS_get_les_cpxx(RSHandle *H, int frame, uint32_t *dims3) {
+    if (!H->L) return NULL;
+    LESTable *t = LES_get_frame(H->L, frame);
+    if (!t) return NULL;
+    const LESFloat4 *c = LES_get_cpxx(H->L, t);
+    if (!c) return NULL;
+    dims3[0] = t->nx;
+    dims3[1] = t->ny;
+    dims3[2] = t->nz;
+    return (const float *)c;
 }
 
 int RS_get_debris_table(RSHandle *H, int t, int which, float *out4, uint32_t *dims2) {

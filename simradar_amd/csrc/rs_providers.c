@@ -468,6 +468,33 @@ static LESTable *les_read_frame(LESMem *h, uint32_t f) {
 
 static LESTable *les_frame(LESMem *h, uint32_t f) { return h->file_mode ? les_read_frame(h, f) : les_make_frame(h, f); }
 
+/* The (cn2, p, 0, 0) volume of a frame (les.c:389-405; read by fp_atts,
+ * rs.cl:590-648).  File frames carry it; a synthetic frame gets it on the
+ * first request: log10 Cn^2 = -13 - z / 1 km + 0.25 sin(2 pi x / 180 m)
+ * cos(2 pi y / 240 m) at the texel positions of the stretched grid, p = 0. */
+const LESFloat4 *LES_get_cpxx(const LESHandle L, LESTable *t) {
+    LESMem *h = (LESMem *)L;
+    if (!h || !t) return NULL;
+    if (t->cpxx) return t->cpxx;
+    LESFloat4 *c = (LESFloat4 *)malloc((size_t)t->nn * sizeof(LESFloat4));
+    if (!c) return NULL;
+    const double offx = 0.5 * (double)(t->nx - 1), offy = 0.5 * (double)(t->ny - 1);
+    size_t idx = 0;
+    for (uint32_t k = 0; k < t->nz; k++) {
+        const double z = texel_position((int)k, 0.0, t->az, t->rz);
+        for (uint32_t j = 0; j < t->ny; j++) {
+            const double y = texel_position((int)j, offy, t->ay, t->ry);
+            for (uint32_t i = 0; i < t->nx; i++, idx++) {
+                const double x = texel_position((int)i, offx, t->ax, t->rx);
+                c[idx][0] = (float)(-13.0 - 1.0e-3 * z + 0.25 * sin(2.0 * M_PI * x / 180.0) * cos(2.0 * M_PI * y / 240.0));
+                c[idx][1] = c[idx][2] = c[idx][3] = 0.0f;
+            }
+        }
+    }
+    t->cpxx = c;
+    return c;
+}
+
 static void *les_ahead_run(void *arg) {
     LESMem *h = (LESMem *)arg;
     h->tables[h->ahead] = les_frame(h, h->ahead);

@@ -1529,6 +1529,94 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     if (PULSE) acc_flush_row(acc, a.pk, a.pk.row0 + (int)L);
 }
 
+// ---------------------------------------------------------------------------
+// Fixed-position background (concept F): fp_atts (rs.cl:590-648) fused with the
+// pulse.  Positions never move; each advance reads the wind and the Cn^2
+// volume at the position, sets vel = wind, turns the stored phase by the
+// radial wind's Doppler over dt and writes rcs = (a cos phi, a sin phi, cn2,
+// phi) with a = 10^cn2 (the V half of rcs carries cn2 and the phase, as the
+// reference's does).  The population is built gate by gate, so the scatterers
+// of a wave share a gate window like range-sorted rain.
+// ---------------------------------------------------------------------------
+struct FixedK {
+    const float4 *pos;
+    float4 *vel, *rcs;
+    unsigned long long n;
+    Tex3 wind, cpxx;
+    F16 les;
+    F16 sd;                 // sim_desc of the advance
+    const float *logtab;
+    PulseK pk;
+};
+
+__device__ __forceinline__ void fixed_step(const F16 &sd, float4 p, float4 uvwt, float4 cp, float4 &rcs) {
+    const float wav_num = sd.s[4], dt = sd.s[11];
+    const float len = rf_len3(p.x, p.y, p.z);
+    const float nx = rf_div(p.x, len), ny = rf_div(p.y, len), nz = rf_div(p.z, len);
+    const float d = (nx * uvwt.x + ny * uvwt.y) + nz * uvwt.z;
+    const float phi = rcs.w - (wav_num * d) * dt;
+    const float cn2 = cp.x;
+    const float a = rf_powf_pos(10.0f, cn2);
+    float sn, cs;
+    rf_sincosf(phi, &sn, &cs);
+    rcs = make_float4(a * cs, a * sn, cn2, rf_atan2f(sn, cs));
+}
+
+template <bool ADV, bool PULSE, bool WIN, bool TRI>
+__device__ __forceinline__ void fixed_range(const FixedK &a, size_t lo, size_t hi, double *acc, const float4 *logtab) {
+    const float2 *wl = win_weights(acc, a.pk);
+    const float *grl = win_gates(acc, a.pk);
+    WaveWin win;
+    win.kb = RSD_WIN_EMPTY;
+    wwin_clear(win);
+    for (size_t base = lo; base < hi; base += blockDim.x) {
+        const size_t i = base + threadIdx.x;
+        const bool live = i < hi;
+        float4 p = make_float4(0.0f, 1.0f, 0.0f, 0.0f), rcs = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (live) {
+            p = a.pos[i];
+            rcs = a.rcs[i];
+        }
+        if (ADV && live) {
+            const float4 c = wind_index(p, a.les, a.sd, logtab);
+            const float4 uvwt = sample3d(a.wind, c.x, c.y, c.z);
+            const float4 cp = sample3d(a.cpxx, c.x, c.y, c.z);
+            fixed_step(a.sd, p, uvwt, cp, rcs);
+            float4 v = a.vel[i];
+            v.x = uvwt.x;
+            v.y = uvwt.y;
+            v.z = uvwt.z;
+            a.vel[i] = v;
+            a.rcs[i] = rcs;
+        }
+        if (PULSE) {
+            float4 sa = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            float r = 0.0f;
+            bool on = live && scat_signal(p, rcs, a.pk, a.pk.angw, sa, r);
+            if (WIN) wwin_fold<TRI>(win, on, r, sa, a.pk, acc, wl, grl);
+            else fold_gates(on, r, sa, a.pk, acc);
+        }
+    }
+    if (WIN) wwin_flush(win, a.pk, acc);
+}
+
+template <bool ADV, bool PULSE>
+__global__ __launch_bounds__(256) void k_fixed(FixedK a) {
+    extern __shared__ double acc[];
+    __shared__ float4 logtab[RF_LOG_TABLE];
+    if (ADV) logtab_load(logtab, a.logtab);
+    if (PULSE) acc_clear(acc, 4 * a.pk.R);
+    else __syncthreads();
+    size_t lo, hi;
+    const unsigned L = block_range_at(0, a.n, blockIdx.x, gridDim.x, lo, hi, true);
+    const bool use_win = PULSE && a.pk.fold == RSD_FOLD_WINDOW;
+    if (use_win) stage_window(acc, a.pk);
+    if (use_win && a.pk.tri) fixed_range<ADV, PULSE, true, true>(a, lo, hi, acc, logtab);
+    else if (use_win) fixed_range<ADV, PULSE, true, false>(a, lo, hi, acc, logtab);
+    else fixed_range<ADV, PULSE, false, false>(a, lo, hi, acc, logtab);
+    if (PULSE) acc_flush_row(acc, a.pk, a.pk.row0 + (int)L);
+}
+
 // Sum the per-block partial rows of one pulse: one block per output value,
 // reading its contiguous column.  Columns are [component][gate]; the pulse
 // buffer is [gate][component].
@@ -1763,6 +1851,7 @@ struct ViewK {
     F4 drop_desc;
     PulseK pk;
     int have_pulse;
+    int fixed;              // background of fixed positions: its rcs is the stream's
 };
 
 __global__ void k_view(ViewK a) {
@@ -1774,7 +1863,7 @@ __global__ void k_view(ViewK a) {
     float4 rcs;
     if (i < a.n_rain) {
         p.w = __builtin_fabsf(p.w);
-        rcs = stale ? a.rcs[i] : ellipsoid_rcs(p, a.drop, a.drop_desc);
+        rcs = (stale || a.fixed) ? a.rcs[i] : ellipsoid_rcs(p, a.drop, a.drop_desc);
     } else {
         rcs = a.rcs[i];
     }
@@ -1895,6 +1984,9 @@ struct rsd_engine {
     int permuted;           // stored rain order differs from the reference order
     float4 *wind[2];
     int wnx[2], wny[2], wnz[2];
+    float4 *cpxx[2];        // (cn2, p, _, _) volume of each LES slot (concept F)
+    int cnx[2], cny[2], cnz[2];
+    int bg_fixed;           // the background advances as fp_atts (last step's kind)
     F16 les[2];
     float4 *adm_cd[RSD_MAX_TABLES], *adm_cm[RSD_MAX_TABLES];
     int adm_nx[RSD_MAX_TABLES], adm_ny[RSD_MAX_TABLES];
@@ -2115,6 +2207,7 @@ extern "C" void rsd_close(rsd_engine *e) {
     free_state(e);
     for (int s = 0; s < 2; s++) {
         dfree(e->wind[s]);
+        dfree(e->cpxx[s]);
         if (e->wind_stage[s]) (void)hipHostFree(e->wind_stage[s]);
         if (e->ev_wind[s]) (void)hipEventDestroy(e->ev_wind[s]);
     }
@@ -2535,7 +2628,30 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p, con
         }
         HIPCHK(e, hipEventRecord(e->ev_join, e->side));
     }
-    if (e->n_rain) {
+    const bool fixed = s ? s->background_kind == RSD_ADVANCE_FIXED : e->bg_fixed != 0;
+    if (e->n_rain && fixed) {
+        if (ride_blocks || any_debris) return fail(e, RSD_ERR_STATE, "a fixed-position background takes no debris");
+        FixedK a;
+        memset(&a, 0, sizeof(a));
+        const int slot = s ? s->wind_slot : 0;
+        if (s && (!e->wind_ok[slot] || !e->cpxx[slot] || e->cnx[slot] != e->wnx[slot] || e->cny[slot] != e->wny[slot] ||
+                  e->cnz[slot] != e->wnz[slot]))
+            return fail(e, RSD_ERR_STATE, "wind and Cn2 volume of slot %d not set", slot);
+        a.pos = e->pos; a.vel = e->vel; a.rcs = e->rcs;
+        a.n = e->n_rain;
+        a.wind.d = e->wind[slot]; a.wind.nx = e->wnx[slot]; a.wind.ny = e->wny[slot]; a.wind.nz = e->wnz[slot];
+        a.cpxx.d = e->cpxx[slot]; a.cpxx.nx = e->cnx[slot]; a.cpxx.ny = e->cny[slot]; a.cpxx.nz = e->cnz[slot];
+        a.les = e->les[slot];
+        if (s) memcpy(a.sd.s, s->sim_desc, sizeof(a.sd.s));
+        a.logtab = e->logtab;
+        a.pk = pk;
+        a.pk.row0 = 0;
+        const dim3 grid(rain_blocks);
+        if (s && p) hipLaunchKernelGGL((k_fixed<true, true>), grid, dim3(e->block), lds, e->stream, a);
+        else if (s) hipLaunchKernelGGL((k_fixed<true, false>), grid, dim3(e->block), 0, e->stream, a);
+        else hipLaunchKernelGGL((k_fixed<false, true>), grid, dim3(e->block), lds, e->stream, a);
+        HIPCHK(e, hipGetLastError());
+    } else if (e->n_rain) {
         RainK a;
         memset(&a, 0, sizeof(a));
         int st = fill_rain(e, s, a);
@@ -2865,13 +2981,14 @@ static void view_common(rsd_engine *e, ViewK &v) {
     v.n_rain = e->n_rain;
     v.drop = e->drop;
     v.drop_desc = e->drop_desc;
+    v.fixed = e->bg_fixed;
 }
 
 // Signal and aux streams of the last pulse from the stored state into sigaux
 // (zero signal and the stored aux before any pulse).  Valid until the next
 // pulse; the stored positions must be the ones the pulse saw.
 static int save_signal(rsd_engine *e) {
-    if (e->n_rain && !e->drop) return fail(e, RSD_ERR_STATE, "drop RCS table not set");
+    if (e->n_rain && !e->drop && !e->bg_fixed) return fail(e, RSD_ERR_STATE, "drop RCS table not set");
     HIPCHK(e, hipSetDevice(e->device));
     join_batch(e);
     if (!e->sigaux) HIPCHK(e, hipMalloc((void **)&e->sigaux, 2 * e->n * sizeof(float4)));
@@ -2911,7 +3028,7 @@ extern "C" int rsd_download_state(rsd_engine *e, float *pos, float *vel, float *
     const bool perm = e->permuted && (vel || rnd);
     if (pos || rcs || perm) {
         if (!e->view) HIPCHK(e, hipMalloc((void **)&e->view, 4 * b));
-        if (e->n_rain && !e->drop) return fail(e, RSD_ERR_STATE, "drop RCS table not set");
+        if (e->n_rain && !e->drop && !e->bg_fixed) return fail(e, RSD_ERR_STATE, "drop RCS table not set");
         ViewK v;
         view_common(e, v);
         v.pos_out = e->view; v.rcs_out = e->view + e->n;
@@ -2989,6 +3106,26 @@ extern "C" int rsd_set_wind(rsd_engine *e, int slot, const float *uvwt, int nx, 
     e->wnx[slot] = nx; e->wny[slot] = ny; e->wnz[slot] = nz;
     e->wind_ok[slot] = 1;
     memcpy(e->les[slot].s, desc, sizeof(e->les[slot].s));
+    return RSD_OK;
+}
+
+extern "C" int rsd_set_cpxx(rsd_engine *e, int slot, const float *cpxx, int nx, int ny, int nz) {
+    if (!e || !cpxx || slot < 0 || slot > 1 || nx < 1 || ny < 1 || nz < 1) return fail(e, RSD_ERR_ARG, "bad Cn2 volume");
+    int st = flush_pending(e);
+    if (st) return st;
+    HIPCHK(e, hipSetDevice(e->device));
+    join_batch(e);
+    HIPCHK(e, hipStreamSynchronize(e->stream));      // kernels of the slot's previous volume are done
+    const size_t texels = (size_t)nx * ny * nz;
+    if (!e->cpxx[slot] || (size_t)e->cnx[slot] * e->cny[slot] * e->cnz[slot] != texels) {
+        dfree(e->cpxx[slot]);
+        e->cpxx[slot] = NULL;
+        HIPCHK(e, hipMalloc((void **)&e->cpxx[slot], texels * sizeof(float4)));
+    }
+    HIPCHK(e, hipMemcpy(e->cpxx[slot], cpxx, texels * sizeof(float4), hipMemcpyHostToDevice));
+    e->cnx[slot] = nx;
+    e->cny[slot] = ny;
+    e->cnz[slot] = nz;
     return RSD_OK;
 }
 
@@ -3095,6 +3232,7 @@ extern "C" int rsd_advance(rsd_engine *e, const rsd_step *step) {
     if (st) return st;
     e->pending = *step;
     e->have_pending = 1;
+    e->bg_fixed = step->background_kind == RSD_ADVANCE_FIXED;
     return RSD_OK;
 }
 
@@ -3112,7 +3250,7 @@ extern "C" int rsd_make_pulse(rsd_engine *e, const rsd_pulse *p) {
         return RSD_OK;
     }
     join_batch(e);
-    if (e->sort_interval > 0 && e->n_rain > 1 && (!e->permuted || e->since_sort >= e->sort_interval)) {
+    if (e->sort_interval > 0 && e->n_rain > 1 && !e->bg_fixed && (!e->permuted || e->since_sort >= e->sort_interval)) {
         HIPCHK(e, hipSetDevice(e->device));
         int st = sort_rain(e, e->have_pending ? e->pending.sim_desc : p->sim_desc, p->range_delta);
         if (st) return st;

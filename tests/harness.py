@@ -75,7 +75,7 @@ def run_pair(cfg, pulses, sort=None, fold=None, launch=None, pre=None, setup=Non
         vi = int(r.info().vel_idx)
         r.advance_time()
         if les_switch and int(r.info().vel_idx) != vi:
-            o.set_les_frame(r.les_frame(vi))
+            o.set_les_frame(r.les_frame(vi), r.les_cpxx(vi) if o.fixed else None)
         o.advance(r.sim_desc())
         r.advance_beam()
     r.download()

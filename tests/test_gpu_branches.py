@@ -335,3 +335,22 @@ def test_batched_domain_against_the_oracle(gpu_ready):
     b.free()
     for r in rs:
         r.free()
+
+
+def test_fixed_position_profiler(gpu_ready):
+    """Concepts F | V (simple_dbs.c): a vertically pointing profiler over
+    fixed scatterers on a zenith raster, DBS beams at 75 and 90 deg.  Each
+    advance is fp_atts (rs.cl:590-648): velocity = wind, phase turned by the
+    radial wind, rcs = 10^cn2 e^{j phi} from the Cn^2 volume.  pos / vel / rcs
+    bit-exact; H and V IQ each within 1e-4 (the V half of a fixed scatterer's
+    rcs carries (cn2, phi), as the reference's)."""
+    cfg = h.Config(n_rain=0, concept=sa.CONCEPT_FIXED | sa.CONCEPT_VERTICAL,
+                   scan="D:0,75,3/90,75,3/0,90,3")
+    r, o, pg, po = h.run_pair(cfg, 8)
+    assert o.fixed and int(r.info().num_scats) > 100000
+    h.check_state(r, o, ("pos", "vel", "rcs"))
+    assert np.max(np.abs(pg[..., :2])) > 0.0
+    for half in (slice(0, 2), slice(2, 4)):
+        errs = [h.rel_err(pg[k][:, half], po[k][:, half]) for k in range(len(pg))]
+        assert max(errs) < IQ_TOL, errs
+    r.free()
