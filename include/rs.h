@@ -185,6 +185,8 @@ struct _rs_handle {
     double              *pulse_acc;             /* binary64 gate sums of the last pulse, R x 4 */
     int                 last_error;             /* RSD_* code of the last failed engine call */
     char                owns_pattern;
+    uint32_t            les_blend;              /* RS_set_les_time_interpolation */
+    float               wind_alpha;             /* time-interpolation weight of the last advance */
 };
 
 unsigned int RS_gpu_count(void);
@@ -354,8 +356,15 @@ typedef struct {
     int32_t  dsd_count;
     uint32_t adm_count;
     uint32_t rcs_count;
+    uint32_t les_blend;     /* 1: LES volumes interpolated in time */
+    float    wind_alpha;    /* weight of the next volume in the last advance */
 } RSInfo;
 void RS_get_info(RSHandle *H, RSInfo *out);
+/* addition: on = 1 interpolates the wind linearly in time between LES volume k
+ * (at k tp) and volume k + 1, both resident in the two slots, with weight
+ * (sim_tic - k tp) / tp; on = 0 (default) keeps the reference's volume switch
+ * every tp (rs.c:5189-5203). */
+void RS_set_les_time_interpolation(RSHandle *H, int on);
 /* addition: host copies of the streams (which: 0 pos, 1 vel, 2 ori, 3 tum,
  * 4 aux, 5 rcs, 6 sig, 7 rnd, 8 pulse, 9 anchors); NULL before RS_populate. */
 void *RS_get_host_stream(RSHandle *H, int which);

@@ -66,7 +66,9 @@ typedef struct {
     int32_t  wind_slot;              /* active LES slot (0 or 1) */
     int32_t  adm_index[RSD_MAX_TYPES];   /* ADM table of each debris type (entry 0 unused) */
     int32_t  rcs_index[RSD_MAX_TYPES];   /* RCS table of each debris type (entry 0 unused) */
-    uint32_t background_kind;        /* RSD_ADVANCE_DRAGGED or RSD_ADVANCE_BACKGROUND */
+    uint32_t background_kind;        /* RSD_ADVANCE_DRAGGED, RSD_ADVANCE_BACKGROUND or RSD_ADVANCE_FIXED */
+    uint32_t wind_blend;             /* 1: wind = lerp(slot wind_slot, the other slot, wind_alpha); 0: the slot alone */
+    float    wind_alpha;             /* time-interpolation weight in [0, 1] (wind_blend only) */
 } rsd_step;
 
 typedef struct {

@@ -48,6 +48,7 @@ def lib():
             "orc_bg_atts": (None, [P, P, P, P, S, S, T, P, P, P, P]),
             "orc_db_atts": (None, [P, P, P, P, P, P, S, S, T, P, T, T, P, T, T, P, P]),
             "orc_fp_atts": (None, [P, P, P, S, S, T, T, P, P]),
+            "orc_set_blend": (None, [T, T, F]),
             "orc_set_dff": (None, [P, ctypes.c_int, P]),
             "orc_dff_place_probe": (ctypes.c_int, [F4, P, P]),
             "orc_dff_icdf_from_pdf": (ctypes.c_int, [P, ctypes.c_int, P, ctypes.c_int]),
@@ -181,6 +182,18 @@ def el_atts(st, begin, end, wind_tex, les_desc, drop, drop_desc, sim_desc):
 def bg_atts(st, begin, end, wind_tex, les_desc, drop, drop_desc, sim_desc):
     lib().orc_bg_atts(_p(st.pos), _p(st.vel), _p(st.rcs), _p(st.rnd), begin, end, ctypes.byref(wind_tex),
                       _p(f32(les_desc)), _p(f32(drop)), _p(f32(drop_desc)), _p(f32(sim_desc)))
+
+
+_blend_keep = None
+
+
+def set_blend(wind_next=None, cpxx_next=None, alpha=0.0):
+    """Blend later advances' wind (and Cn^2) towards the next LES volume with
+    weight alpha (textures from tex()); wind_next None switches it off."""
+    global _blend_keep
+    _blend_keep = (wind_next, cpxx_next)
+    lib().orc_set_blend(None if wind_next is None else ctypes.byref(wind_next),
+                        None if cpxx_next is None else ctypes.byref(cpxx_next), float(alpha))
 
 
 def fp_atts(st, begin, end, wind_tex, cpxx_tex, les_desc, sim_desc):

@@ -98,6 +98,31 @@ of4 orc_sample3d(const orc_tex *t, float u, float v, float w) {
     return lerp4(y0, y1, c);
 }
 
+/* Time interpolation between LES volumes: the wind (and Cn^2 volume) of the
+ * advances blend towards the next volume with weight orc_blend_alpha,
+ * lerp(sample of the current volume, sample of the next, alpha) per channel,
+ * when a next volume is set. */
+static const orc_tex *orc_blend_wind = NULL, *orc_blend_cpxx = NULL;
+static float orc_blend_alpha = 0.0f;
+
+void orc_set_blend(const orc_tex *wind_next, const orc_tex *cpxx_next, float alpha) {
+    orc_blend_wind = wind_next;
+    orc_blend_cpxx = cpxx_next;
+    orc_blend_alpha = alpha;
+}
+
+static of4 wind_at(const orc_tex *t, of4 c) {
+    of4 a = orc_sample3d(t, c.x, c.y, c.z);
+    if (!orc_blend_wind) return a;
+    return lerp4(a, orc_sample3d(orc_blend_wind, c.x, c.y, c.z), orc_blend_alpha);
+}
+
+static of4 cpxx_at(const orc_tex *t, of4 c) {
+    of4 a = orc_sample3d(t, c.x, c.y, c.z);
+    if (!orc_blend_wind || !orc_blend_cpxx) return a;
+    return lerp4(a, orc_sample3d(orc_blend_cpxx, c.x, c.y, c.z), orc_blend_alpha);
+}
+
 of4 orc_sample2d(const orc_tex *t, float u, float v) {
     int i0, i1, j0, j1;
     float a, b;
@@ -229,7 +254,7 @@ void orc_el_atts(of4 *p, of4 *v, of4 *x, ou4 *y, size_t begin, size_t end,
             vel.x = vel.y = vel.z = vel.w = 0.0f;
         } else {
             of4 c = orc_wind_index(pos, les_desc, sd);
-            of4 bg = orc_sample3d(wind, c.x, c.y, c.z);
+            of4 bg = wind_at(wind, c);
             of4 dv = {bg.x - vel.x, bg.y - vel.y, bg.z - vel.z, bg.w - vel.w};
             float dva = of_len3(dv.x, dv.y, dv.z);
             if (dva > 1.0e-3f) {
@@ -289,7 +314,7 @@ void orc_bg_atts(of4 *p, of4 *v, of4 *x, ou4 *y, size_t begin, size_t end,
             continue;
         }
         of4 c = orc_wind_index(pos, les_desc, sd);
-        vel = orc_sample3d(wind, c.x, c.y, c.z);
+        vel = wind_at(wind, c);
         p[i] = pos;
         v[i] = vel;
         x[i] = orc_ellipsoid_rcs(pos, drop, dd);
@@ -306,8 +331,8 @@ void orc_fp_atts(of4 *p, of4 *v, of4 *x, size_t begin, size_t end, const orc_tex
         const of4 pos = p[i];
         of4 vel = v[i], rcs = x[i];
         of4 c = orc_wind_index(pos, les_desc, sd);
-        of4 uvwt = orc_sample3d(wind, c.x, c.y, c.z);
-        of4 cp = orc_sample3d(cpxx, c.x, c.y, c.z);
+        of4 uvwt = wind_at(wind, c);
+        of4 cp = cpxx_at(cpxx, c);
         float len = of_len3(pos.x, pos.y, pos.z);
         float nx = pos.x / len, ny = pos.y / len, nz = pos.z / len;
         float d = (nx * uvwt.x + ny * uvwt.y) + nz * uvwt.z;
@@ -478,7 +503,7 @@ void orc_db_atts(of4 *p, of4 *o, of4 *v, of4 *t, of4 *x, ou4 *y, size_t begin, s
             continue;
         }
         of4 c = orc_wind_index(pos, les_desc, sd);
-        of4 bg = orc_sample3d(wind, c.x, c.y, c.z);
+        of4 bg = wind_at(wind, c);
         /* rs.cl:195-261 compute_dudt_dwdt */
         of4 ur = {bg.x - vel.x, bg.y - vel.y, bg.z - vel.z, bg.w - vel.w};
         of4 uh = qrot(norm4(ur), qconj(ori));

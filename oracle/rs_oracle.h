@@ -35,6 +35,11 @@ void orc_rand4(ou4 *seed, of4 *r);
 /* rs.cl:123-176 -- table coordinates of a position. */
 of4 orc_wind_index(of4 pos, const float *les_desc, const float *sim_desc);
 
+/* Time interpolation of the advances' wind (and Cn^2) towards the next LES
+ * volume with weight alpha; wind_next NULL switches it off (the reference's
+ * piecewise-constant volumes, rs.c:5189-5203). */
+void orc_set_blend(const orc_tex *wind_next, const orc_tex *cpxx_next, float alpha);
+
 /* OpenCL linear filter, unnormalised coordinates, clamp-to-edge (rs.cl:8). */
 of4 orc_sample3d(const orc_tex *t, float u, float v, float w);
 of4 orc_sample2d(const orc_tex *t, float u, float v);

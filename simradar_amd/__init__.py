@@ -70,7 +70,7 @@ EXPORTED_RS = [
     "RS_set_adm_data_to_ADM_table", "RS_set_adm_data_to_unity", "RS_clear_adm_data",
     "RS_set_rcs_data_to_RCS_table", "RS_set_rcs_data_to_unity", "RS_clear_rcs_data",
     "RS_set_rcs_ellipsoid_table", "RS_revise_population", "RS_update_origins_offsets", "RS_merge_pulse_tmp",
-    "RS_init_host_only", "RS_set_population", "RS_set_phase_precision", "RS_get_pulse_f64",
+    "RS_init_host_only", "RS_set_population", "RS_set_phase_precision", "RS_set_les_time_interpolation", "RS_get_pulse_f64",
     "RS_set_moment_pulses", "RS_get_moments", "RS_set_range_cap", "RS_batch_round",
     "RS_get_descriptor", "RS_radar_gain", "RS_engine", "RS_get_info", "RS_get_host_stream", "RS_get_params",
     "RS_get_dsd", "RS_suggest_scan_domain_values", "RS_set_scan_box_values", "RS_get_les_frame", "RS_get_les_cpxx", "RS_get_debris_table",
@@ -104,7 +104,8 @@ class RSInfo(ctypes.Structure):
                 ("sim_tic", ctypes.c_float), ("sim_toc", ctypes.c_float), ("status", ctypes.c_uint32),
                 ("sim_concept", ctypes.c_uint32), ("random_seed", ctypes.c_uint32), ("vel_idx", ctypes.c_uint32),
                 ("les_id", ctypes.c_uint32), ("range_count", ctypes.c_uint32), ("last_error", ctypes.c_int32),
-                ("dsd_count", ctypes.c_int32), ("adm_count", ctypes.c_uint32), ("rcs_count", ctypes.c_uint32)]
+                ("dsd_count", ctypes.c_int32), ("adm_count", ctypes.c_uint32), ("rcs_count", ctypes.c_uint32),
+                ("les_blend", ctypes.c_uint32), ("wind_alpha", ctypes.c_float)]
 
 
 class RSPolar(ctypes.Structure):
@@ -135,7 +136,7 @@ class RSMakePulseParams(ctypes.Structure):
 class RSDStep(ctypes.Structure):
     _fields_ = [("sim_desc", ctypes.c_float * 16), ("wind_slot", ctypes.c_int32),
                 ("adm_index", ctypes.c_int32 * 8), ("rcs_index", ctypes.c_int32 * 8),
-                ("background_kind", ctypes.c_uint32)]
+                ("background_kind", ctypes.c_uint32), ("wind_blend", ctypes.c_uint32), ("wind_alpha", ctypes.c_float)]
 
 
 class RSDPulse(ctypes.Structure):
@@ -207,6 +208,7 @@ def _declare(lib):
         "RS_revise_debris_counts_to_gpu_preference": (None, [P]),
         "RS_simulation_description": (ctypes.c_char_p, [P]),
         "RS_set_population": (None, [P, S]),
+        "RS_set_les_time_interpolation": (None, [P, I]),
         "RS_set_phase_precision": (None, [P, ctypes.c_uint32]),
         "RS_get_pulse_f64": (I, [P, P]),
         "RS_set_moment_pulses": (None, [P, I]),

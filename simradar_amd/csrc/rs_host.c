@@ -817,6 +817,17 @@ void RS_set_vel_data_to_LES_table(RSHandle *H, const LESTable *leslie) {
     H->vel_desc.flux = NULL;
 }
 
+/* Time interpolation: LES volume vel_idx into the slot that is not active
+ * (the next volume), then vel_idx moves on. */
+static void les_load_next(RSHandle *H) {
+    RSWorker *C = &H->workers[0];
+    const uint32_t cur = C->les_id;
+    C->les_id = 1 - cur;
+    RS_set_vel_data_to_LES_table(H, LES_get_frame(H->L, (int)H->vel_idx));
+    C->les_id = cur;
+    H->vel_idx = H->vel_idx == H->vel_count - 1 ? 0 : H->vel_idx + 1;
+}
+
 void RS_set_vel_data_to_config(RSHandle *H, LESConfig c) {
     if (H->L) LES_free(H->L);
     H->L = LES_init_with_config_path(c, NULL);
@@ -825,6 +836,18 @@ void RS_set_vel_data_to_config(RSHandle *H, LESConfig c) {
     H->vel_count = (uint32_t)LES_get_table_count(H->L);
     RS_set_vel_data_to_LES_table(H, LES_get_frame(H->L, 0));
     H->vel_idx = 1;
+    if (H->les_blend) les_load_next(H);
+}
+
+void RS_set_les_time_interpolation(RSHandle *H, int on) {
+    if (H->status & RSStatusDomainPopulated) {
+        rsprint("Simulation domain has been populated. The time interpolation cannot be changed.");
+        return;
+    }
+    if ((on != 0) == (H->les_blend != 0)) return;
+    H->les_blend = on != 0;
+    if (H->les_blend && H->L) les_load_next(H);
+    if (!H->les_blend && H->L) H->vel_idx = H->vel_idx == 0 ? H->vel_count - 1 : H->vel_idx - 1;
 }
 
 static void set_small_wind(RSHandle *H, RSTable3D *t, int n, const cl_float4 *uvwt) {
@@ -1542,8 +1565,17 @@ void RS_populate(RSHandle *H) {
     RS_update_origins_offsets(H);
     if (H->sim_concept & RSSimulationConceptFixedScattererPosition) {
         if (!populate_fixed(H)) return;
-        /* the frame in the wind slot (concept F may have been set after it) */
-        if (H->L && H->vel_count) upload_cpxx(H, LES_get_frame(H->L, (int)((H->vel_idx + H->vel_count - 1) % H->vel_count)));
+        /* the frames in the wind slots (concept F may have been set after them) */
+        if (H->L && H->vel_count) {
+            const uint32_t k = H->vel_count, back = H->les_blend ? 2u : 1u;
+            upload_cpxx(H, LES_get_frame(H->L, (int)((H->vel_idx + 2 * k - back) % k)));
+            if (H->les_blend) {
+                RSWorker *C = &H->workers[0];
+                C->les_id = 1 - C->les_id;
+                upload_cpxx(H, LES_get_frame(H->L, (int)((H->vel_idx + k - 1) % k)));
+                C->les_id = 1 - C->les_id;
+            }
+        }
     } else {
         populate_streams(H);
     }
@@ -1655,8 +1687,19 @@ void RS_advance_time(RSHandle *H) {
     if (H->L && H->sim_tic >= H->sim_toc) {
         H->sim_toc += H->vel_desc.tp;
         H->workers[0].les_id = H->workers[0].les_id == 1 ? 0 : 1;
-        RS_set_vel_data_to_LES_table(H, LES_get_frame(H->L, (int)H->vel_idx));
-        H->vel_idx = H->vel_idx == H->vel_count - 1 ? 0 : H->vel_idx + 1;
+        if (H->les_blend) {
+            /* the slot now active already holds this volume; the next one
+             * goes to the other slot */
+            les_load_next(H);
+        } else {
+            RS_set_vel_data_to_LES_table(H, LES_get_frame(H->L, (int)H->vel_idx));
+            H->vel_idx = H->vel_idx == H->vel_count - 1 ? 0 : H->vel_idx + 1;
+        }
+    }
+    H->wind_alpha = 0.0f;
+    if (H->les_blend && H->L) {
+        const float tp = H->vel_desc.tp;
+        H->wind_alpha = fminf(fmaxf((H->sim_tic - (H->sim_toc - tp)) / tp, 0.0f), 1.0f);
     }
     if (has_engine(H)) {
         rsd_step s;
@@ -1668,6 +1711,8 @@ void RS_advance_time(RSHandle *H) {
         s.background_kind = (H->sim_concept & RSSimulationConceptDraggedBackground) ? RSD_ADVANCE_DRAGGED
                             : (H->sim_concept & RSSimulationConceptFixedScattererPosition) ? RSD_ADVANCE_FIXED
                                                                                            : RSD_ADVANCE_BACKGROUND;
+        s.wind_blend = H->les_blend && H->L ? 1u : 0u;
+        s.wind_alpha = H->wind_alpha;
         engine_ok(H, rsd_advance(H->workers[0].engine, &s), "advance");
     }
     H->sim_tic += H->params.prt;
@@ -1786,6 +1831,8 @@ void RS_get_info(RSHandle *H, RSInfo *o) {
     o->dsd_count = H->dsd_count;
     o->adm_count = H->workers[0].adm_count;
     o->rcs_count = H->workers[0].rcs_count;
+    o->les_blend = H->les_blend;
+    o->wind_alpha = H->wind_alpha;
 }
 
 void *RS_get_host_stream(RSHandle *H, int which) {

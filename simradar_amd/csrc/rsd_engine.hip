@@ -78,7 +78,9 @@ __device__ __forceinline__ void st4(float4 *p, float4 x) {
 
 struct F16 { float s[16]; };
 struct F4 { float s[4]; };
-struct Tex3 { const float4 *d; int nx, ny, nz; };
+// d2 != NULL: a second volume of the same shape (the next LES frame) and the
+// weight of the linear time interpolation between the two samples.
+struct Tex3 { const float4 *d; int nx, ny, nz; const float4 *d2; float alpha; };
 struct Tex2 { const float4 *d; int nx, ny; };
 
 // ---------------------------------------------------------------------------
@@ -132,7 +134,12 @@ __device__ __forceinline__ float4 lerp3d(const Gather3 &g) {
 __device__ __forceinline__ float4 sample3d(const Tex3 &t, float u, float v, float w) {
     Gather3 g;
     gather3d(t, u, v, w, g);
-    return lerp3d(g);
+    const float4 a = lerp3d(g);
+    if (!t.d2) return a;
+    Tex3 t2 = t;
+    t2.d = t.d2;
+    gather3d(t2, u, v, w, g);
+    return lerp4(a, lerp3d(g), t.alpha);
 }
 
 __device__ __forceinline__ float4 sample2d(const Tex2 &t, float u, float v) {
@@ -963,9 +970,15 @@ __device__ __forceinline__ bool rain_move(const RainK &a, size_t i, float4 &pos,
     return false;
 }
 
-__device__ __forceinline__ void rain_gather(const RainK &a, float4 pos, const float4 *logtab, Gather3 &g) {
+template <bool TI = false>
+__device__ __forceinline__ void rain_gather(const RainK &a, float4 pos, const float4 *logtab, Gather3 &g, Gather3 *g2 = nullptr) {
     float4 c = wind_index(pos, a.les, a.sd, logtab);
     gather3d(a.wind, c.x, c.y, c.z, g);
+    if (TI) {
+        Tex3 t2 = a.wind;
+        t2.d = a.wind.d2;
+        gather3d(t2, c.x, c.y, c.z, *g2);
+    }
 }
 
 __device__ __forceinline__ void rain_drag(const RainK &a, float4 pos, float4 &vel, float4 bg) {
@@ -1010,7 +1023,7 @@ __device__ __forceinline__ void logtab_load(float4 *dst, const float *src) { rf_
 // the block's LDS gate sums acc (cleared by the caller).  The hi/lo variant
 // (96 B of state per scatterer, meant for populations far beyond the 256 MB
 // Infinity Cache) streams its state non-temporally.
-template <bool ADV, bool PULSE, bool HILO, bool WIN, bool TRI>
+template <bool ADV, bool PULSE, bool HILO, bool WIN, bool TRI, bool TI>
 __device__ __forceinline__ void rain_range_t(const RainK &a, size_t lo, size_t hi, double *acc, const float4 *logtab) {
     const float2 *wl = win_weights(acc, a.pk);
     const float *grl = win_gates(acc, a.pk);
@@ -1047,7 +1060,7 @@ __device__ __forceinline__ void rain_range_t(const RainK &a, size_t lo, size_t h
         // the position, not the velocity), so the reads overlap that work.
         float4 v = v_in;
         bool moved = false;     // live, advanced and not respawned
-        Gather3 g;
+        Gather3 g, g2;
         if (ADV && live) {
             const float4 p_in = p;
             bool respawned = rain_move<HILO>(a, i, p, v, plo);
@@ -1062,7 +1075,7 @@ __device__ __forceinline__ void rain_range_t(const RainK &a, size_t lo, size_t h
             } else {
                 stale = false;
                 moved = true;
-                rain_gather(a, p, logtab, g);
+                rain_gather<TI>(a, p, logtab, g, &g2);
             }
             st4<HILO>(&a.pos[i], make_float4(p.x, p.y, p.z, stale ? -p.w : p.w));
         }
@@ -1078,7 +1091,7 @@ __device__ __forceinline__ void rain_range_t(const RainK &a, size_t lo, size_t h
             else fold_gates(on, r, sa, a.pk, acc);
         }
         if (ADV && live) {
-            if (moved) rain_drag(a, p, v, lerp3d(g));
+            if (moved) rain_drag(a, p, v, TI ? lerp4(lerp3d(g), lerp3d(g2), a.wind.alpha) : lerp3d(g));
             st4<HILO>(&a.vel[i], v);
         }
     }
@@ -1088,12 +1101,12 @@ __device__ __forceinline__ void rain_range_t(const RainK &a, size_t lo, size_t h
 // The fold strategy is a template flag so that the loop carrying the window
 // registers has no second fold path (which makes the compiler copy the 32
 // window registers at every step).
-template <bool ADV, bool PULSE, bool HILO>
+template <bool ADV, bool PULSE, bool HILO, bool TI = false>
 __device__ __forceinline__ void rain_range(const RainK &a, size_t lo, size_t hi, double *acc, const float4 *logtab,
                                            bool use_win) {
-    if (use_win && a.pk.tri) rain_range_t<ADV, PULSE, HILO, true, true>(a, lo, hi, acc, logtab);
-    else if (use_win) rain_range_t<ADV, PULSE, HILO, true, false>(a, lo, hi, acc, logtab);
-    else rain_range_t<ADV, PULSE, HILO, false, false>(a, lo, hi, acc, logtab);
+    if (use_win && a.pk.tri) rain_range_t<ADV, PULSE, HILO, true, true, TI>(a, lo, hi, acc, logtab);
+    else if (use_win) rain_range_t<ADV, PULSE, HILO, true, false, TI>(a, lo, hi, acc, logtab);
+    else rain_range_t<ADV, PULSE, HILO, false, false, TI>(a, lo, hi, acc, logtab);
 }
 
 // Many independent rain domains in one persistent launch.  items[t] = (domain,
@@ -1499,7 +1512,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 // first[nt] (a multiple of 8, so the rain blocks' partial rows keep their XCD
 // order).
 
-template <bool ADV, bool PULSE, bool HILO>
+template <bool ADV, bool PULSE, bool HILO, bool TI = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_rain(RainK a, StepDebris d) {
     extern __shared__ double acc[];
     __shared__ float4 logtab[RF_LOG_TABLE];
@@ -1525,7 +1538,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const bool use_win = PULSE && a.pk.fold == RSD_FOLD_WINDOW;
     if (PULSE) stage_rain_tables(acc, a);
     if (use_win) stage_window(acc, a.pk);
-    rain_range<ADV, PULSE, HILO>(a, lo, hi, acc, logtab, use_win);
+    rain_range<ADV, PULSE, HILO, TI>(a, lo, hi, acc, logtab, use_win);
     if (PULSE) acc_flush_row(acc, a.pk, a.pk.row0 + (int)L);
 }
 
@@ -2438,6 +2451,17 @@ static int fill_pulse(rsd_engine *e, const rsd_pulse *p, PulseK &k, const Static
     return RSD_OK;
 }
 
+// Time interpolation towards the other LES slot (same shape), weight alpha.
+static int blend_slot(rsd_engine *e, int slot, Tex3 *t, float alpha) {
+    const int o = 1 - slot;
+    if (!e->wind_ok[o] || e->wnx[o] != e->wnx[slot] || e->wny[o] != e->wny[slot] || e->wnz[o] != e->wnz[slot])
+        return fail(e, RSD_ERR_STATE, "wind slot %d (the next volume) not set for the time interpolation", o);
+    if (!(alpha >= 0.0f && alpha <= 1.0f)) return fail(e, RSD_ERR_ARG, "time interpolation weight %g outside [0, 1]", alpha);
+    t->d2 = e->wind[o];
+    t->alpha = alpha;
+    return RSD_OK;
+}
+
 static int fill_rain(rsd_engine *e, const rsd_step *s, RainK &a) {
     int slot = s ? s->wind_slot : 0;
     if (s && (!e->wind_ok[slot])) return fail(e, RSD_ERR_STATE, "wind slot %d not set", slot);
@@ -2445,6 +2469,10 @@ static int fill_rain(rsd_engine *e, const rsd_step *s, RainK &a) {
     a.pos = e->pos; a.vel = e->vel; a.rcs = e->rcs; a.rnd = e->rnd;
     a.n = e->n_rain;
     a.wind.d = e->wind[slot]; a.wind.nx = e->wnx[slot]; a.wind.ny = e->wny[slot]; a.wind.nz = e->wnz[slot];
+    if (s && s->wind_blend) {
+        int st = blend_slot(e, slot, &a.wind, s->wind_alpha);
+        if (st) return st;
+    }
     a.les = e->les[slot];
     if (s) memcpy(a.sd.s, s->sim_desc, sizeof(a.sd.s));
     a.drop = e->drop;
@@ -2480,6 +2508,10 @@ static int fill_debris(rsd_engine *e, const rsd_step *s, const rsd_pulse *p, int
         if (ai < 0 || ai >= RSD_MAX_TABLES || !e->adm_cd[ai]) return fail(e, RSD_ERR_STATE, "ADM table %d not set", ai);
         if (ri < 0 || ri >= RSD_MAX_TABLES || !e->rcs_re[ri]) return fail(e, RSD_ERR_STATE, "RCS table %d not set", ri);
         a.wind.d = e->wind[slot]; a.wind.nx = e->wnx[slot]; a.wind.ny = e->wny[slot]; a.wind.nz = e->wnz[slot];
+        if (s->wind_blend) {
+            int st = blend_slot(e, slot, &a.wind, s->wind_alpha);
+            if (st) return st;
+        }
         a.les = e->les[slot];
         memcpy(a.sd.s, s->sim_desc, sizeof(a.sd.s));
         a.adm_cd.d = e->adm_cd[ai]; a.adm_cm.d = e->adm_cm[ai];
@@ -2641,6 +2673,15 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p, con
         a.n = e->n_rain;
         a.wind.d = e->wind[slot]; a.wind.nx = e->wnx[slot]; a.wind.ny = e->wny[slot]; a.wind.nz = e->wnz[slot];
         a.cpxx.d = e->cpxx[slot]; a.cpxx.nx = e->cnx[slot]; a.cpxx.ny = e->cny[slot]; a.cpxx.nz = e->cnz[slot];
+        if (s && s->wind_blend) {
+            int st = blend_slot(e, slot, &a.wind, s->wind_alpha);
+            if (st) return st;
+            if (!e->cpxx[1 - slot] || e->cnx[1 - slot] != e->cnx[slot] || e->cny[1 - slot] != e->cny[slot] ||
+                e->cnz[1 - slot] != e->cnz[slot])
+                return fail(e, RSD_ERR_STATE, "Cn2 volume of slot %d not set for the time interpolation", 1 - slot);
+            a.cpxx.d2 = e->cpxx[1 - slot];
+            a.cpxx.alpha = s->wind_alpha;
+        }
         a.les = e->les[slot];
         if (s) memcpy(a.sd.s, s->sim_desc, sizeof(a.sd.s));
         a.logtab = e->logtab;
@@ -2678,7 +2719,16 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p, con
         // (hipExtLaunchKernelGGL): they bracket the kernel's execution, not the
         // queue's processing of separate event packets.
         hipEvent_t t0 = timed ? e->prof_ev[2 * e->prof_n] : nullptr, t1 = timed ? e->prof_ev[2 * e->prof_n + 1] : nullptr;
-        if (e->plo) {
+        const bool ti = a.wind.d2 != nullptr;
+        if (e->plo && ti) {
+            if (s && p) hipExtLaunchKernelGGL((k_rain<true, true, true, true>), grid, dim3(e->block), lds_rain, e->stream, t0, t1, 0, a, d);
+            else if (s) hipExtLaunchKernelGGL((k_rain<true, false, true, true>), grid, dim3(e->block), 0, e->stream, t0, t1, 0, a, d);
+            else hipExtLaunchKernelGGL((k_rain<false, true, true>), grid, dim3(e->block), lds_rain, e->stream, t0, t1, 0, a, d);
+        } else if (ti) {
+            if (s && p) hipExtLaunchKernelGGL((k_rain<true, true, false, true>), grid, dim3(e->block), lds_rain, e->stream, t0, t1, 0, a, d);
+            else if (s) hipExtLaunchKernelGGL((k_rain<true, false, false, true>), grid, dim3(e->block), 0, e->stream, t0, t1, 0, a, d);
+            else hipExtLaunchKernelGGL((k_rain<false, true, false>), grid, dim3(e->block), lds_rain, e->stream, t0, t1, 0, a, d);
+        } else if (e->plo) {
             if (s && p) hipExtLaunchKernelGGL((k_rain<true, true, true>), grid, dim3(e->block), lds_rain, e->stream, t0, t1, 0, a, d);
             else if (s) hipExtLaunchKernelGGL((k_rain<true, false, true>), grid, dim3(e->block), 0, e->stream, t0, t1, 0, a, d);
             else hipExtLaunchKernelGGL((k_rain<false, true, true>), grid, dim3(e->block), lds_rain, e->stream, t0, t1, 0, a, d);
@@ -3707,6 +3757,7 @@ extern "C" int rsd_batch_launch(rsd_batch *b) {
         memset(&a, 0, sizeof(a));
         st = fill_rain(e, any_adv ? &e->pending : NULL, a);
         if (st) return bfail(b, st, e->err);
+        if (a.wind.d2) return bfail(b, RSD_ERR_STATE, "batched domains take no time-interpolated wind");
         st = fill_pulse(e, p, a.pk);
         if (st) return bfail(b, st, e->err);
         a.pk.row0 = 0;

@@ -29,7 +29,7 @@ def mp_dsd(rain_rate_mm_h=10.0, d_mm=None, n0=8000.0):
 class Config:
     def __init__(self, n_rain=131072, debris=(), concept=DBU, gate=15.0, scan="P:3.0,-12:12:0.01", seed=SEED,
                  prt=1e-3, phase=PHASE_F64, rain_rate=10.0, box=None, name="custom", range_cap=None,
-                 les="suctvort"):
+                 les="suctvort", les_interp=False):
         self.n_rain = n_rain
         self.debris = list(debris)
         self.concept = concept
@@ -42,6 +42,7 @@ class Config:
         self.box = box          # None: RS_suggest_scan_domain; else 8 floats (origin r,a,e,w, size r,a,e,w)
         self.range_cap = range_cap  # None: the reference's 10 km cap on the padded extent
         self.les = les          # LES provider config ("suctvort" or "suctvort:cx,cy,seed")
+        self.les_interp = les_interp  # LES volumes interpolated in time (else switched every tp)
         self.name = name
 
     @property
@@ -111,6 +112,8 @@ def configure(r, cfg):
     r.set_random_seed(cfg.seed)
     r.set_scan_pattern(cfg.scan)
     r.set_vel_data_to_config(cfg.les)
+    if cfg.les_interp:
+        r.set_les_time_interpolation(1)
     for kind, count in cfg.debris:
         r.add_debris(kind, count)
     if cfg.range_cap:

@@ -60,7 +60,34 @@ def run_pair(cfg, pulses, sort=None, fold=None, launch=None, pre=None, setup=Non
     if setup is not None:
         setup(r, o)
     mode = orc.PHASE_F64 if pulse_mode is None else pulse_mode
+    blend = bool(r.info().les_blend)
+    frames = [None]
+
+    def follow_blend():
+        """The oracle's volumes and weight for the coming advance under time
+        interpolation: the active volume vel_idx - 2, the next vel_idx - 1."""
+        inf = r.info()
+        vn = int(inf.vel_idx)
+        assert vn >= 2
+        if frames[0] != vn:
+            o.set_les_frame(r.les_frame(vn - 2), r.les_cpxx(vn - 2) if o.fixed else None)
+            nxt = orc.tex(r.les_frame(vn - 1))
+            nc = orc.tex(r.les_cpxx(vn - 1)) if o.fixed else (None, None)
+            frames[0] = vn
+            frames.append((nxt, nc))
+        (wt, wk), (ct, ck) = frames[-1]
+        orc.set_blend(wt, ct, float(inf.wind_alpha))
     pg, po = [], []
+    try:
+        _steps(r, o, pulses, mode, pg, po, on_pulse, les_switch, follow_blend if blend else None)
+    finally:
+        orc.set_blend(None)
+    r.download()
+    r.check()
+    return r, o, np.array(pg), np.array(po)
+
+
+def _steps(r, o, pulses, mode, pg, po, on_pulse, les_switch, follow_blend):
     for k in range(pulses):
         if k > 0:
             r.make_pulse()
@@ -74,13 +101,12 @@ def run_pair(cfg, pulses, sort=None, fold=None, launch=None, pre=None, setup=Non
             on_pulse(k, r, o)
         vi = int(r.info().vel_idx)
         r.advance_time()
-        if les_switch and int(r.info().vel_idx) != vi:
+        if follow_blend is not None:
+            follow_blend()
+        elif les_switch and int(r.info().vel_idx) != vi:
             o.set_les_frame(r.les_frame(vi), r.les_cpxx(vi) if o.fixed else None)
         o.advance(r.sim_desc())
         r.advance_beam()
-    r.download()
-    r.check()
-    return r, o, np.array(pg), np.array(po)
 
 
 def check_state(r, o, streams=("pos", "vel", "ori", "tum")):

@@ -354,3 +354,25 @@ def test_fixed_position_profiler(gpu_ready):
         errs = [h.rel_err(pg[k][:, half], po[k][:, half]) for k in range(len(pg))]
         assert max(errs) < IQ_TOL, errs
     r.free()
+
+
+@pytest.mark.parametrize("debris", [False, True])
+def test_les_time_interpolation_against_the_oracle(gpu_ready, debris):
+    """RS_set_les_time_interpolation: volumes k and k + 1 resident in the two
+    slots and the wind lerp(sample_k, sample_k+1, (t - k tp) / tp) for rain and
+    debris; across two volume boundaries (PRT 10 ms, tp 2 s) the state stays
+    bit-exact with the oracle blending the same volumes."""
+    cfg = h.Config(n_rain=4096, debris=[(sa.OBJ_LEAF, 256)] if debris else [], scan="P:3.0,-1:1:0.01", prt=0.01,
+                   les_interp=True)
+    alphas = []
+
+    def on_pulse(k, r, o):
+        alphas.append(float(r.info().wind_alpha))
+
+    r, o, pg, po = h.run_pair(cfg, 430, on_pulse=on_pulse)
+    assert r.info().les_blend == 1
+    a = np.array(alphas[1:])
+    assert a.min() < 0.01 and a.max() > 0.99 and np.sum(np.diff(a) < 0) == 2, "two volume boundaries"
+    h.check_state(r, o)
+    _assert_iq(pg, po)
+    r.free()

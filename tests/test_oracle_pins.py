@@ -377,3 +377,36 @@ def test_flux_field_placement_closed_forms(kind):
             assert got[2] == 2.0
     finally:
         orc.set_dff(None, None)
+
+
+# ----------------------------------------------------------------- LES time interpolation
+
+def test_time_interpolated_wind_is_linear_in_the_weight():
+    """Two uniform volumes u0, u1 and weight alpha: bg_atts sets the velocity
+    to u0 + alpha (u1 - u0) (the sampler's lerp, within one rounding)."""
+    u0, u1 = np.array([3.0, -2.0, 0.5]), np.array([-1.0, 4.0, 2.5])
+    t0 = np.tile(np.append(u0, 0.0).astype(np.float32), (3, 3, 3, 1))
+    t1 = np.tile(np.append(u1, 0.0).astype(np.float32), (3, 3, 3, 1))
+    (w0, k0), (w1, k1) = orc.tex(t0), orc.tex(t1)
+    les = np.zeros(16, np.float32)
+    les[0:3] = 1.0
+    sd = np.zeros(16, np.float32)
+    sd[8:11] = [-100.0, -100.0, 0.0]
+    sd[12:15] = [200.0, 200.0, 100.0]
+    sd[11] = 0.01
+    drop = np.zeros((96, 4), np.float32)
+    dd = np.array([20000.0, -5.0, 95.0, 1.0], np.float32)
+    try:
+        for alpha in (0.0, 0.25, 0.5, 0.9, 1.0):
+            orc.set_blend(w1, None, alpha)
+            st = orc.State([[1.0, 2.0, 3.0, 1e-3]], [[0, 0, 0, 0]], [[0, 0, 0, 1]], [[0, 0, 0, 1]], [[0, 0, 0, 0]],
+                           np.ones((1, 4), np.uint32))
+            orc.bg_atts(st, 0, 1, w0, les, drop, dd, sd)
+            np.testing.assert_allclose(st.vel[0, :3], u0 + alpha * (u1 - u0), rtol=1e-6, atol=1e-6)
+        orc.set_blend(None)
+        st = orc.State([[1.0, 2.0, 3.0, 1e-3]], [[0, 0, 0, 0]], [[0, 0, 0, 1]], [[0, 0, 0, 1]], [[0, 0, 0, 0]],
+                       np.ones((1, 4), np.uint32))
+        orc.bg_atts(st, 0, 1, w0, les, drop, dd, sd)
+        np.testing.assert_array_equal(st.vel[0, :3], u0.astype(np.float32))
+    finally:
+        orc.set_blend(None)
