@@ -37,6 +37,11 @@ STATUS_DEBRIS_RCS_UPDATE = 1 << 6
 PHASE_F64 = 0
 PHASE_F32 = 1
 
+FOLD_LANE = 0       # gate-sum strategies of rsd_set_fold (include/rsd.h)
+FOLD_WAVE64 = 1
+FOLD_WAVE32 = 2
+FOLD_WINDOW = 3
+
 OBJ_LEAF = 1
 OBJ_LEAF_BIG = 2
 OBJ_WOODBOARD_2X4 = 3

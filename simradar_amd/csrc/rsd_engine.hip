@@ -78,9 +78,23 @@ __device__ __forceinline__ void st4(float4 *p, float4 x) {
 
 struct F16 { float s[16]; };
 struct F4 { float s[4]; };
-// d2 != NULL: a second volume of the same shape (the next LES frame) and the
-// weight of the linear time interpolation between the two samples.
-struct Tex3 { const float4 *d; int nx, ny, nz; const float4 *d2; float alpha; };
+// A 3-D table held padded: (nx + 3) x (ny + 3) x (nz + 3) texels, the table at
+// offset (1, 1, 1) with its edge texels repeated once before and twice after
+// along every axis (RSD_PAD_*), so that the clamped texel pair of the sampler
+// is always k, k + 1 of the padded volume.  px, pxy: padded row and plane
+// lengths; fx, fy, fz: nx, ny, nz as floats.  d2 != NULL: a second volume of the
+// same shape (the next LES frame) and the weight of the linear time
+// interpolation between the two samples.
+struct Tex3 {
+    const float4 *d;
+    int nx, ny, nz;
+    uint32_t px, pxy;
+    float fx, fy, fz;
+    const float4 *d2;
+    float alpha;
+};
+#define RSD_PAD_LO 1
+#define RSD_PAD_HI 2
 struct Tex2 { const float4 *d; int nx, ny; };
 
 // ---------------------------------------------------------------------------
@@ -108,19 +122,27 @@ __device__ __forceinline__ float4 lerp4(float4 p, float4 q, float t) {
 // in flight.  Texel order t[4 z + 2 y + x].
 struct Gather3 { float4 t[8]; float a, b, c; };
 
+// tex_axis on a padded axis: the padded index of the lower texel (the clamped
+// floor(u - 0.5) + RSD_PAD_LO) and the weight; the pair is (k, k + 1).
+__device__ __forceinline__ uint32_t tex_axis_padded(float u, float nf, float &a) {
+    const float uu = u - 0.5f;
+    const float fl = __builtin_floorf(uu);
+    a = uu - fl;
+    return (uint32_t)((int)__builtin_fminf(__builtin_fmaxf(fl, -1.0f), nf) + RSD_PAD_LO);
+}
+
 __device__ __forceinline__ void gather3d(const Tex3 &t, float u, float v, float w, Gather3 &g) {
-    int i0, i1, j0, j1, k0, k1;
-    tex_axis(u, t.nx, i0, i1, g.a);
-    tex_axis(v, t.ny, j0, j1, g.b);
-    tex_axis(w, t.nz, k0, k1, g.c);
-    const uint32_t nx = (uint32_t)t.nx, nxy = (uint32_t)t.nx * (uint32_t)t.ny;
-    const float4 *D = t.d;
-    const uint32_t r00 = k0 * nxy + j0 * nx, r10 = k0 * nxy + j1 * nx;
-    const uint32_t r01 = k1 * nxy + j0 * nx, r11 = k1 * nxy + j1 * nx;
-    g.t[0] = D[r00 + i0]; g.t[1] = D[r00 + i1];
-    g.t[2] = D[r10 + i0]; g.t[3] = D[r10 + i1];
-    g.t[4] = D[r01 + i0]; g.t[5] = D[r01 + i1];
-    g.t[6] = D[r11 + i0]; g.t[7] = D[r11 + i1];
+    const uint32_t i = tex_axis_padded(u, t.fx, g.a);
+    const uint32_t j = tex_axis_padded(v, t.fy, g.b);
+    const uint32_t k = tex_axis_padded(w, t.fz, g.c);
+    const float4 *b0 = t.d + (k * t.pxy + j * t.px + i);
+    const float4 *b1 = b0 + t.px;
+    const float4 *b2 = b0 + t.pxy;
+    const float4 *b3 = b2 + t.px;
+    g.t[0] = b0[0]; g.t[1] = b0[1];
+    g.t[2] = b1[0]; g.t[3] = b1[1];
+    g.t[4] = b2[0]; g.t[5] = b2[1];
+    g.t[6] = b3[0]; g.t[7] = b3[1];
 }
 
 __device__ __forceinline__ float4 lerp3d(const Gather3 &g) {
@@ -549,6 +571,14 @@ __device__ __forceinline__ void fold_gates(bool on, float r, float4 sa, const Pu
 #define RSD_LDS_DROP_MAX 256      // drop RCS entries staged in LDS by rain pulse launches
 #define RSD_LDS_ANGW_MAX 1024     // angular weight entries staged in LDS
 #define RSD_WIN_EMPTY (-(1 << 24))
+// Gates of the wave-shared window of range-sorted rain: a scatterer's support
+// under the triangle weight spans two gates (three within rounding of a gate
+// range) and a launch's range-ordered scatterers move on through the gates, so
+// eight gates keep nearly every lane in the window (four send too many lanes
+// to the per-lane atomics and run slower, five waves per SIMD included).
+#ifndef RSD_WWIN
+#define RSD_WWIN 8
+#endif
 
 struct GateWin {
     float a[RSD_WIN][4];
@@ -594,16 +624,23 @@ __device__ __forceinline__ void win_lane_flush(const GateWin &w, const PulseK &k
 }
 
 // Range weight of offset x from the table in LDS, held as pairs (W[i],
-// W[min(i + 1, n - 1)] - W[i]): the same clamp / fract / mix as table_lerp,
-// W[i] + (W[i + 1] - W[i]) dec with the difference rounded once when the table
-// is staged.  Only used where the table ends in zeros, so offsets outside the
-// support weigh exactly 0.
+// W[min(i + 1, n - 1)] - W[i]): the same clamp / fract / mix as table_lerp.
+// The upper entry is W[j], j = floor(clamp(fma(x, s0, s1 + 1))), as there.
+// j is i + 1 (i at the clamped top, or with dec = 0, where the difference does
+// not matter) except where x0 rounds to just below a node m and x1 rounds up
+// to m + 1: the table read then mixes W[m - 1] with W[m + 1], and so does this
+// (the rare lane reads W[j]).  Only used where the table ends in zeros, so
+// offsets outside the support weigh exactly 0.
 __device__ __forceinline__ float lds_weight(float x, const float2 *W, const F4 &d) {
-    float x0 = rf_clamp(rf_fma(x, d.s[0], d.s[1]), 0.0f, d.s[2]);
-    float ip = __builtin_floorf(x0);
-    float dec = __builtin_fminf(x0 - ip, 0x1.fffffep-1f);
-    const float2 p = W[(int)ip];
-    return p.x + p.y * dec;
+    const float x0 = rf_clamp(rf_fma(x, d.s[0], d.s[1]), 0.0f, d.s[2]);
+    const float x1 = rf_clamp(rf_fma(x, d.s[0], d.s[1] + 1.0f), 0.0f, d.s[2]);
+    const float ip = __builtin_floorf(x0);
+    const float dec = __builtin_fminf(x0 - ip, 0x1.fffffep-1f);
+    const int i = (int)ip, j = (int)__builtin_floorf(x1);
+    const float2 p = W[i];
+    float dw = p.y;
+    if (j > i + 1) dw = W[j].x - p.x;
+    return p.x + dw * dec;
 }
 
 // The triangle table (0, 1, 0) with max index 2 (RS_set_range_weight_to_triangle,
@@ -743,13 +780,13 @@ __device__ __forceinline__ float wave_transpose_sum(const float (&a)[32]) {
 }
 
 struct WaveWin {
-    float a[4 * RSD_WIN];   // slot 4 s + c: gate kb + s, component c
+    float a[4 * RSD_WWIN];  // slot 4 s + c: gate kb + s, component c
     int kb;                 // wave-uniform first gate; RSD_WIN_EMPTY: no window
 };
 
 __device__ __forceinline__ void wwin_clear(WaveWin &w) {
 #pragma unroll
-    for (int j = 0; j < 4 * RSD_WIN; j++) w.a[j] = 0.0f;
+    for (int j = 0; j < 4 * RSD_WWIN; j++) w.a[j] = 0.0f;
 }
 
 // Terms of one scatterer straight into the LDS gate sums.
@@ -768,7 +805,10 @@ __device__ __forceinline__ void lane_fold(float r, float4 sa, int klo, int khi, 
 
 __device__ __forceinline__ void wwin_flush(const WaveWin &w, const PulseK &k, double *acc) {
     if (w.kb == RSD_WIN_EMPTY) return;
-    const float v = wave_transpose_sum(w.a);
+    float a32[32];      // the transposed sum takes 32 values: the window's, then zeros
+#pragma unroll
+    for (int j = 0; j < 32; j++) a32[j] = j < 4 * RSD_WWIN ? w.a[j] : 0.0f;
+    const float v = wave_transpose_sum(a32);
     const unsigned lane = __lane_id();
     const int g = w.kb + (int)(lane >> 3), c = (int)((lane >> 1) & 3u);
     if (!(lane & 1u) && g >= 0 && g < k.R && v != 0.0f) atomicAdd(&acc[c * k.R + g], (double)v);
@@ -798,11 +838,11 @@ __device__ __forceinline__ void wwin_fold(WaveWin &w, bool on, float r, float4 s
         w.kb = median3(__builtin_amdgcn_readlane(klo, l0), __builtin_amdgcn_readlane(klo, l1),
                        __builtin_amdgcn_readlane(klo, l2)) - 1;
     }
-    const bool fit = on && klo >= w.kb && khi < w.kb + RSD_WIN;
+    const bool fit = on && klo >= w.kb && khi < w.kb + RSD_WWIN;
     if (on && !fit) lane_fold(r, sa, klo, khi, k, acc, wl, grl);
     if (!fit) sa = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
-    for (int s = 0; s < RSD_WIN; s++) {
+    for (int s = 0; s < RSD_WWIN; s++) {
         const int g = w.kb + s;
         if (g < 0 || g >= k.R) continue;
         if (__ballot(fit && klo <= g && g <= khi) == 0) continue;
@@ -1512,8 +1552,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 // first[nt] (a multiple of 8, so the rain blocks' partial rows keep their XCD
 // order).
 
+#ifndef RSD_RAIN_WAVES
+#define RSD_RAIN_WAVES 4    // waves per SIMD the fused rain kernel is compiled for
+#endif
 template <bool ADV, bool PULSE, bool HILO, bool TI = false>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_rain(RainK a, StepDebris d) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RSD_RAIN_WAVES))) void k_rain(RainK a, StepDebris d) {
     extern __shared__ double acc[];
     __shared__ float4 logtab[RF_LOG_TABLE];
     if (ADV) logtab_load(logtab, a.logtab);
@@ -1929,6 +1972,22 @@ static __device__ __align__(256) unsigned char scratch[RSD_SCRATCH_BYTES];
 static void *scratch_dev[RSD_SCRATCH_DEVICES];
 static pthread_mutex_t scratch_lock = PTHREAD_MUTEX_INITIALIZER;
 
+// Padded copy of a 3-D table (see Tex3): texel (i, j, k) of the padded volume is
+// texel (clamp(i - RSD_PAD_LO), clamp(j - RSD_PAD_LO), clamp(k - RSD_PAD_LO)) of
+// the table.
+__global__ void k_pad3d(const float4 *__restrict__ src, float4 *__restrict__ dst, int nx, int ny, int nz) {
+    const uint32_t px = (uint32_t)nx + RSD_PAD_LO + RSD_PAD_HI, py = (uint32_t)ny + RSD_PAD_LO + RSD_PAD_HI;
+    const uint32_t pz = (uint32_t)nz + RSD_PAD_LO + RSD_PAD_HI;
+    const size_t n = (size_t)px * py * pz;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(q % px) - RSD_PAD_LO, j = (int)((q / px) % py) - RSD_PAD_LO, k = (int)(q / ((size_t)px * py)) - RSD_PAD_LO;
+        const int ci = i < 0 ? 0 : (i > nx - 1 ? nx - 1 : i);
+        const int cj = j < 0 ? 0 : (j > ny - 1 ? ny - 1 : j);
+        const int ck = k < 0 ? 0 : (k > nz - 1 ? nz - 1 : k);
+        dst[q] = src[((size_t)ck * ny + cj) * nx + ci];
+    }
+}
+
 // rs.cl:1189 / 1214: gate k's range as start plus k binary32 additions of delta.
 __global__ void k_gate_ranges(float start, float delta, int R, float *out) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1995,8 +2054,10 @@ struct rsd_engine {
     float4 *mom_d;
     int since_sort;
     int permuted;           // stored rain order differs from the reference order
-    float4 *wind[2];
+    float4 *wind[2];        // padded volumes (see Tex3)
     int wnx[2], wny[2], wnz[2];
+    float4 *raw;            // an unpadded table on its way to a padded slot
+    size_t raw_n;
     float4 *cpxx[2];        // (cn2, p, _, _) volume of each LES slot (concept F)
     int cnx[2], cny[2], cnz[2];
     int bg_fixed;           // the background advances as fp_atts (last step's kind)
@@ -2221,6 +2282,7 @@ extern "C" void rsd_close(rsd_engine *e) {
     for (int s = 0; s < 2; s++) {
         dfree(e->wind[s]);
         dfree(e->cpxx[s]);
+        if (s == 0) dfree(e->raw);
         if (e->wind_stage[s]) (void)hipHostFree(e->wind_stage[s]);
         if (e->ev_wind[s]) (void)hipEventDestroy(e->ev_wind[s]);
     }
@@ -2438,16 +2500,54 @@ static int fill_pulse(rsd_engine *e, const rsd_pulse *p, PulseK &k, const Static
     if (p->range_delta == 0.0f) k.dense = 1;
     k.phase_mode = p->phase_mode;
     k.fold = e->fold;
-    // The window needs every scatterer's gates inside RSD_WIN consecutive gates
+    // The windows need every scatterer's gates inside RSD_WWIN consecutive gates
     // (with room to move); wider supports take the wave sums.
     // Its LDS tables (weight pairs and gate ranges) must leave room for the
     // debris kernels' static LDS next to the 4 R binary64 gate sums.
     if (k.fold == RSD_FOLD_WINDOW &&
-        (k.dense || (k.sup_hi - k.sup_lo) * k.gate_inv_delta >= (float)(RSD_WIN - 2) ||
+        (k.dense || (k.sup_hi - k.sup_lo) * k.gate_inv_delta >= (float)(RSD_WWIN - 1) ||
          (size_t)k.R * (4 * sizeof(double) + sizeof(float)) + (size_t)k.rangew_n * sizeof(float2) > 96 * 1024))
         k.fold = RSD_FOLD_WAVE32;
     k.partials = sp ? sp->partials : e->partials;
     k.row0 = 0;
+    return RSD_OK;
+}
+
+static size_t padded_texels(int nx, int ny, int nz) {
+    return (size_t)(nx + RSD_PAD_LO + RSD_PAD_HI) * (ny + RSD_PAD_LO + RSD_PAD_HI) * (nz + RSD_PAD_LO + RSD_PAD_HI);
+}
+
+static Tex3 tex3_of(const float4 *padded, int nx, int ny, int nz) {
+    Tex3 t;
+    memset(&t, 0, sizeof(t));
+    t.d = padded;
+    t.nx = nx; t.ny = ny; t.nz = nz;
+    t.px = (uint32_t)(nx + RSD_PAD_LO + RSD_PAD_HI);
+    t.pxy = t.px * (uint32_t)(ny + RSD_PAD_LO + RSD_PAD_HI);
+    t.fx = (float)nx; t.fy = (float)ny; t.fz = (float)nz;
+    return t;
+}
+
+// The raw upload buffer, at least `texels` long (a larger one replaces it once
+// the pads already queued have read it).
+static int ensure_raw(rsd_engine *e, size_t texels) {
+    if (e->raw && e->raw_n >= texels) return RSD_OK;
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    dfree(e->raw);
+    e->raw = NULL;
+    e->raw_n = 0;
+    HIPCHK(e, hipMalloc((void **)&e->raw, texels * sizeof(float4)));
+    e->raw_n = texels;
+    return RSD_OK;
+}
+
+// Padded copy of the raw buffer's nx x ny x nz table into dst, on the stream.
+static int pad_raw(rsd_engine *e, float4 *dst, int nx, int ny, int nz) {
+    const size_t n = padded_texels(nx, ny, nz);
+    size_t nb = (n + 255) / 256;
+    if (nb > 16384) nb = 16384;
+    hipLaunchKernelGGL(k_pad3d, dim3((unsigned)nb), dim3(256), 0, e->stream, (const float4 *)e->raw, dst, nx, ny, nz);
+    HIPCHK(e, hipGetLastError());
     return RSD_OK;
 }
 
@@ -2468,7 +2568,7 @@ static int fill_rain(rsd_engine *e, const rsd_step *s, RainK &a) {
     if (!e->drop) return fail(e, RSD_ERR_STATE, "drop RCS table not set");
     a.pos = e->pos; a.vel = e->vel; a.rcs = e->rcs; a.rnd = e->rnd;
     a.n = e->n_rain;
-    a.wind.d = e->wind[slot]; a.wind.nx = e->wnx[slot]; a.wind.ny = e->wny[slot]; a.wind.nz = e->wnz[slot];
+    a.wind = tex3_of(e->wind[slot], e->wnx[slot], e->wny[slot], e->wnz[slot]);
     if (s && s->wind_blend) {
         int st = blend_slot(e, slot, &a.wind, s->wind_alpha);
         if (st) return st;
@@ -2507,7 +2607,7 @@ static int fill_debris(rsd_engine *e, const rsd_step *s, const rsd_pulse *p, int
         if (!e->wind_ok[slot]) return fail(e, RSD_ERR_STATE, "wind slot %d not set", slot);
         if (ai < 0 || ai >= RSD_MAX_TABLES || !e->adm_cd[ai]) return fail(e, RSD_ERR_STATE, "ADM table %d not set", ai);
         if (ri < 0 || ri >= RSD_MAX_TABLES || !e->rcs_re[ri]) return fail(e, RSD_ERR_STATE, "RCS table %d not set", ri);
-        a.wind.d = e->wind[slot]; a.wind.nx = e->wnx[slot]; a.wind.ny = e->wny[slot]; a.wind.nz = e->wnz[slot];
+        a.wind = tex3_of(e->wind[slot], e->wnx[slot], e->wny[slot], e->wnz[slot]);
         if (s->wind_blend) {
             int st = blend_slot(e, slot, &a.wind, s->wind_alpha);
             if (st) return st;
@@ -2671,8 +2771,8 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p, con
             return fail(e, RSD_ERR_STATE, "wind and Cn2 volume of slot %d not set", slot);
         a.pos = e->pos; a.vel = e->vel; a.rcs = e->rcs;
         a.n = e->n_rain;
-        a.wind.d = e->wind[slot]; a.wind.nx = e->wnx[slot]; a.wind.ny = e->wny[slot]; a.wind.nz = e->wnz[slot];
-        a.cpxx.d = e->cpxx[slot]; a.cpxx.nx = e->cnx[slot]; a.cpxx.ny = e->cny[slot]; a.cpxx.nz = e->cnz[slot];
+        a.wind = tex3_of(e->wind[slot], e->wnx[slot], e->wny[slot], e->wnz[slot]);
+        a.cpxx = tex3_of(e->cpxx[slot], e->cnx[slot], e->cny[slot], e->cnz[slot]);
         if (s && s->wind_blend) {
             int st = blend_slot(e, slot, &a.wind, s->wind_alpha);
             if (st) return st;
@@ -3118,11 +3218,12 @@ extern "C" int rsd_set_wind(rsd_engine *e, int slot, const float *uvwt, int nx, 
     HIPCHK(e, hipSetDevice(e->device));
     int st = flush_pending(e);
     if (st) return st;
-    const size_t texels = (size_t)nx * ny * nz;
+    const size_t texels = (size_t)nx * ny * nz, pt = padded_texels(nx, ny, nz);
     if (e->wind[slot] && e->wnx[slot] == nx && e->wny[slot] == ny && e->wnz[slot] == nz) {
         // New frame into a slot of the same shape (the LES switch of
-        // RS_advance_time): a stream-ordered copy through pinned staging, so
-        // the queue keeps running; launches already enqueued finish first.
+        // RS_advance_time): a stream-ordered copy through pinned staging and a
+        // pad into the slot, so the queue keeps running; launches already
+        // enqueued finish first.
         join_batch(e);
         if (!e->ev_wind[slot]) HIPCHK(e, hipEventCreateWithFlags(&e->ev_wind[slot], hipEventDisableTiming));
         else HIPCHK(e, hipEventSynchronize(e->ev_wind[slot]));   // the staging buffer's last copy is done
@@ -3133,18 +3234,32 @@ extern "C" int rsd_set_wind(rsd_engine *e, int slot, const float *uvwt, int nx, 
             HIPCHK(e, hipHostMalloc((void **)&e->wind_stage[slot], texels * sizeof(float4), hipHostMallocDefault));
             e->wind_stage_n[slot] = texels;
         }
-        memcpy(e->wind_stage[slot], uvwt, texels * sizeof(float4));
-        HIPCHK(e, hipMemcpyAsync(e->wind[slot], e->wind_stage[slot], texels * sizeof(float4), hipMemcpyHostToDevice,
-                                 e->stream));
-        HIPCHK(e, hipEventRecord(e->ev_wind[slot], e->stream));
-    } else {
-        st = upload_tex(e, &e->wind[slot], uvwt, texels);
+        st = ensure_raw(e, texels);
         if (st) return st;
+        memcpy(e->wind_stage[slot], uvwt, texels * sizeof(float4));
+        HIPCHK(e, hipMemcpyAsync(e->raw, e->wind_stage[slot], texels * sizeof(float4), hipMemcpyHostToDevice, e->stream));
+        HIPCHK(e, hipEventRecord(e->ev_wind[slot], e->stream));
+        st = pad_raw(e, e->wind[slot], nx, ny, nz);
+        if (st) return st;
+    } else {
+        join_batch(e);
+        HIPCHK(e, hipStreamSynchronize(e->stream));
+        dfree(e->wind[slot]);
+        e->wind[slot] = NULL;
+        HIPCHK(e, hipMalloc((void **)&e->wind[slot], pt * sizeof(float4)));
+        st = ensure_raw(e, texels);
+        if (st) return st;
+        HIPCHK(e, hipMemcpyAsync(e->raw, uvwt, texels * sizeof(float4), hipMemcpyHostToDevice, e->stream));
+        st = pad_raw(e, e->wind[slot], nx, ny, nz);
+        if (st) return st;
+        HIPCHK(e, hipStreamSynchronize(e->stream));
         // the other slot takes the next frame of the same shape: allocate it now
         // so that the switch needs no allocation (and no device-wide wait)
         const int o = 1 - slot;
-        if (!e->wind[o]) {
-            HIPCHK(e, hipMalloc((void **)&e->wind[o], texels * sizeof(float4)));
+        if (!e->wind[o] || e->wnx[o] != nx || e->wny[o] != ny || e->wnz[o] != nz) {
+            dfree(e->wind[o]);
+            e->wind[o] = NULL;
+            HIPCHK(e, hipMalloc((void **)&e->wind[o], pt * sizeof(float4)));
             e->wnx[o] = nx; e->wny[o] = ny; e->wnz[o] = nz;
             e->wind_ok[o] = 0;
             if (!e->wind_stage[o]) {
@@ -3167,12 +3282,17 @@ extern "C" int rsd_set_cpxx(rsd_engine *e, int slot, const float *cpxx, int nx, 
     join_batch(e);
     HIPCHK(e, hipStreamSynchronize(e->stream));      // kernels of the slot's previous volume are done
     const size_t texels = (size_t)nx * ny * nz;
-    if (!e->cpxx[slot] || (size_t)e->cnx[slot] * e->cny[slot] * e->cnz[slot] != texels) {
+    if (!e->cpxx[slot] || e->cnx[slot] != nx || e->cny[slot] != ny || e->cnz[slot] != nz) {
         dfree(e->cpxx[slot]);
         e->cpxx[slot] = NULL;
-        HIPCHK(e, hipMalloc((void **)&e->cpxx[slot], texels * sizeof(float4)));
+        HIPCHK(e, hipMalloc((void **)&e->cpxx[slot], padded_texels(nx, ny, nz) * sizeof(float4)));
     }
-    HIPCHK(e, hipMemcpy(e->cpxx[slot], cpxx, texels * sizeof(float4), hipMemcpyHostToDevice));
+    st = ensure_raw(e, texels);
+    if (st) return st;
+    HIPCHK(e, hipMemcpyAsync(e->raw, cpxx, texels * sizeof(float4), hipMemcpyHostToDevice, e->stream));
+    st = pad_raw(e, e->cpxx[slot], nx, ny, nz);
+    if (st) return st;
+    HIPCHK(e, hipStreamSynchronize(e->stream));
     e->cnx[slot] = nx;
     e->cny[slot] = ny;
     e->cnz[slot] = nz;

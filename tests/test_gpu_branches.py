@@ -106,16 +106,20 @@ def test_uniform_grid_wind(gpu_ready, n):
     r.free()
 
 
-@pytest.mark.parametrize("table", ["window", "dense"])
+@pytest.mark.parametrize("table", ["cosine", "bell", "dense"])
 @pytest.mark.parametrize("sort", [0, 32])
 def test_range_weight_tables(gpu_ready, table, sort):
-    """A 9-entry raised-cosine range weight over +-1.5 gates (ends at zero: the
-    wave-shared window folds it through its LDS weight pairs) and a table whose
-    ends do not vanish (every gate weighted: the dense fold)."""
-    if table == "window":
+    """Tables whose ends are zero take the wave-shared window (its LDS weight
+    pairs): a 9-entry raised cosine over +-1.5 gates and a 5-entry bell (0, .7,
+    1, .7, 0) over +-1 gate; a table whose ends do not vanish weighs every gate
+    (the dense fold)."""
+    if table == "cosine":
         w = (0.5 - 0.5 * np.cos(2 * np.pi * np.arange(9) / 8)).astype(np.float32)
         w[0] = w[-1] = 0.0
         half = 22.5
+    elif table == "bell":
+        w = np.array([0.0, 0.7, 1.0, 0.7, 0.0], np.float32)
+        half = 15.0
     else:
         w = np.array([0.2, 0.6, 1.0, 0.6, 0.2], np.float32)
         half = 15.0
@@ -129,6 +133,83 @@ def test_range_weight_tables(gpu_ready, table, sort):
     h.check_state(r, o)
     _assert_iq(pg, po)
     r.free()
+
+
+def _f32_fma(a, b, c):
+    """fl32(a b + c): the binary32 product is exact in binary64 and, for the
+    magnitudes used here, so is the sum."""
+    return np.float32(np.float64(a) * np.float64(b) + np.float64(c))
+
+
+@pytest.mark.parametrize("fold", [sa.FOLD_WINDOW, sa.FOLD_WAVE32])
+def test_range_weight_on_table_nodes(gpu_ready, fold):
+    """Scatterers exactly on the nodes of a 5-entry bell (0, .7, 1, .7, 0) over
+    +-15 m and a few ulps either side.  The table read (rs.cl:1192-1206) takes
+    its upper entry at floor(fma(x, s0, s1 + 1)); at x = -7.5 m, x0 =
+    fma(x, s0, s1) rounds to just below 1 while that index rounds up to 3, so
+    the weight is mix(W[0], W[2], 0.99999994), not W[1].  Both gate folds give
+    the oracle's pulse."""
+    lib = sa.load()
+    R, start, delta = 16, np.float32(1000.0), np.float32(15.0)
+    rw = np.array([0.0, 0.7, 1.0, 0.7, 0.0], np.float32)
+    rwd = orc.table1d_desc(-15.0, np.float32(7.5), 5)
+    gr = orc.gate_ranges(start, delta, R)
+    rr = []
+    for g in range(R):
+        for off in (-15.0, -7.5, 0.0, 7.5, 15.0):
+            r0 = np.float32(gr[g] + np.float32(off))
+            for u in range(-3, 4):
+                rr.append(np.nextafter(r0, np.float32(np.inf if u > 0 else -np.inf)) if u else r0)
+                for _ in range(abs(u) - 1):
+                    rr[-1] = np.nextafter(rr[-1], np.float32(np.inf if u > 0 else -np.inf))
+    rr = np.array(rr, np.float32)
+    n = rr.size
+    # the case above occurs: the upper index lies two entries above the lower
+    x = rr[:, None] - gr[None, :]
+    x0 = np.clip(_f32_fma(x, rwd[0], rwd[1]), 0.0, rwd[2])
+    x1 = np.clip(_f32_fma(x, rwd[0], rwd[1] + np.float32(1.0)), 0.0, rwd[2])
+    assert np.any(np.floor(x1) > np.floor(x0) + 1)
+    pos = np.stack([np.zeros(n), rr, np.zeros(n), np.full(n, 1e-3)], -1).astype(np.float32)
+    rcs = np.tile(np.array([1.0, 0.5, 1.0, 0.5], np.float32), (n, 1))
+    aw = np.array([0.0, 0.7, 1.0, 0.7, 0.0], np.float32)
+    awd = orc.table1d_desc(np.float32(-0.35), np.float32(0.175), 5)
+    drop = np.tile(np.array([1.0, 0.5, 1.0, 0.5], np.float32), (96, 1))
+    ddesc = np.array([20000.0, -5.0, 95.0, 1.0], np.float32)
+    sd = np.zeros(16, np.float32)
+    sd[1] = 1.0
+    sd[4] = np.float32(4.0 * np.pi / 0.1)
+    sd[8:11] = [-2000.0, -2000.0, -2000.0]
+    sd[12:15] = [4000.0, 4000.0, 4000.0]
+    want = orc.make_pulse(pos, rcs, aw, awd, rw, rwd, float(start), float(delta), R, sd, 4.0 * np.pi / 0.1,
+                          orc.PHASE_F32)
+    e = ctypes.c_void_p()
+    assert lib.rsd_open(0, ctypes.byref(e)) == 0
+    try:
+        def ok(rc):
+            assert rc == 0, lib.rsd_last_error(e).decode()
+        origins = (ctypes.c_size_t * 8)(*([0] * 8))
+        counts = (ctypes.c_size_t * 8)(*([n] + [0] * 7))
+        ok(lib.rsd_alloc_state(e, n, origins, counts))
+        z = np.zeros((n, 4), np.float32)
+        seeds = np.ones((n, 4), np.uint32)
+        ok(lib.rsd_upload_state(e, pos.ctypes.data, z.ctypes.data, z.ctypes.data, z.ctypes.data, z.ctypes.data,
+                                rcs.ctypes.data, seeds.ctypes.data))
+        ok(lib.rsd_set_drop_rcs(e, drop.ctypes.data, 96, ddesc.ctypes.data))
+        ok(lib.rsd_set_range_weight(e, rw.ctypes.data, 5, rwd.ctypes.data))
+        ok(lib.rsd_set_angular_weight(e, aw.ctypes.data, 5, awd.ctypes.data))
+        ok(lib.rsd_set_sort(e, 0))
+        ok(lib.rsd_set_fold(e, fold))
+        p = sa.RSDPulse()
+        p.sim_desc[:] = sd.tolist()
+        p.two_over_lambda = 2.0 / 0.1
+        p.range_start, p.range_delta, p.range_count = float(start), float(delta), R
+        p.phase_mode = sa.PHASE_F32
+        ok(lib.rsd_make_pulse(e, ctypes.byref(p)))
+        out = np.zeros(4 * R, np.float64)
+        ok(lib.rsd_read_pulse(e, out.ctypes.data, R))
+    finally:
+        lib.rsd_close(e)
+    assert h.rel_err(out.reshape(R, 4), want) < IQ_TOL, h.rel_err(out.reshape(R, 4), want)
 
 
 def test_binary32_phase(gpu_ready):
