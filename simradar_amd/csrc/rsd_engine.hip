@@ -176,9 +176,21 @@ __device__ __forceinline__ float4 sample2d(const Tex2 &t, float u, float v) {
     return lerp4(y0, y1, b);
 }
 
-// rs.cl:123-176: LES table coordinates of a position.
-__device__ __forceinline__ float4 wind_index(float4 pos, const F16 &d, const F16 &sd, const float4 *logtab) {
+// rs.cl:123-176: LES table coordinates of a position.  STRETCHED: the launch
+// has checked that the table is stretched (slot 7) with non-negative growth
+// factors (slots 4-6), so only that branch is compiled; (cx, cy) is the
+// domain centre sd[8..9] + 0.5 sd[12..13], rounded as below.
+template <bool STRETCHED = false>
+__device__ __forceinline__ float4 wind_index(float4 pos, const F16 &d, const F16 &sd, const float4 *logtab,
+                                             float cx = 0.0f, float cy = 0.0f) {
     float4 c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (STRETCHED) {
+        const float rx = pos.x - cx, ry = pos.y - cy, rz = pos.z - 0.0f;
+        c.x = __builtin_copysignf(d.s[0], rx) * rf_log1p_pos(d.s[4] * __builtin_fabsf(rx), logtab) + d.s[8];
+        c.y = __builtin_copysignf(d.s[1], ry) * rf_log1p_pos(d.s[5] * __builtin_fabsf(ry), logtab) + d.s[9];
+        c.z = __builtin_copysignf(d.s[2], rz) * rf_log1p_pos(d.s[6] * __builtin_fabsf(rz), logtab) + d.s[10];
+        return c;
+    }
     uint32_t spacing = __float_as_uint(d.s[7]);
     if (spacing == 7u) {
         float cx = sd.s[8] + 0.5f * sd.s[12];
@@ -404,8 +416,9 @@ __device__ __forceinline__ float beam_angle(float4 p, const F16 &sd) {
 
 // Signal x angular weight of one scatterer (rs.cl:1050-1110, 1175-1180).
 // With HILO the position is the pair hi + lo (binary32 each) and the range that
-// enters the phase is formed in binary64 from it.
-template <bool HILO = false>
+// enters the phase is formed in binary64 from it.  F64: the launch's phase mode
+// is RSD_PHASE_F64.
+template <bool HILO = false, bool F64 = false>
 __device__ __forceinline__ bool scat_signal(float4 p, float4 rcs, const PulseK &k, const float *angw, float4 &sa,
                                             float &r, float4 lo = make_float4(0.0f, 0.0f, 0.0f, 0.0f)) {
     double r64 = 0.0;
@@ -425,7 +438,7 @@ __device__ __forceinline__ bool scat_signal(float4 p, float4 rcs, const PulseK &
         double turns = r64 * k.two_over_lambda;
         float t = (float)(turns - floor(turns));
         rf_sincos_turns(t, &ss, &cc);
-    } else if (k.phase_mode == RSD_PHASE_F64) {
+    } else if (F64 || k.phase_mode == RSD_PHASE_F64) {
         double turns = (double)r * k.two_over_lambda;
         float t = (float)(turns - floor(turns));
         rf_sincos_turns(t, &ss, &cc);
@@ -922,6 +935,10 @@ struct RainK {
     // the wind texel reads in flight (global reads complete in issue order).
     int tab_lds, drop_n, angw_n;
     unsigned tab_off;
+    // Host-rounded constants of the specialised launch (k_rain FAST): the far
+    // corner sd[8..10] + sd[12..14] of the box and the domain centre sd[8..9] +
+    // 0.5 sd[12..13] of the stretched wind index.
+    float hx, hy, hz, cx, cy;
 };
 
 // Byte offset of the staged rain tables in the dynamic LDS of a pulse launch:
@@ -962,12 +979,14 @@ __device__ __forceinline__ void hilo_step(float &hi, float &lo, float v, float d
 // scatterer that left the box (returns true; vel = 0 then and the step ends);
 // rain_gather reads the wind texels at the new position; rain_drag turns them
 // into the new velocity.  HILO: positions are hi + lo pairs (lo in its own
-// stream); the physics reads hi.
-template <bool HILO = false>
+// stream); the physics reads hi.  FAST: the launch is dragged (concept D), the
+// box's far corner comes rounded from the host.
+template <bool HILO = false, bool FAST = false>
 __device__ __forceinline__ bool rain_move(const RainK &a, size_t i, float4 &pos, float4 &vel, float4 &lo) {
     const F16 &sd = a.sd;
     const float dt = sd.s[11];
-    if (a.dragged) {
+    const bool dragged = FAST || a.dragged;
+    if (dragged) {
         if (HILO) {
             hilo_step(pos.x, lo.x, vel.x, dt);
             hilo_step(pos.y, lo.y, vel.y, dt);
@@ -990,11 +1009,14 @@ __device__ __forceinline__ bool rain_move(const RainK &a, size_t i, float4 &pos,
         }
         pos.w = pos.w + vel.w * 0.0f;
     }
-    if (is_outside(pos, sd)) {
+    const bool out = FAST ? (pos.x <= sd.s[8]) | (pos.y <= sd.s[9]) | (pos.z <= sd.s[10]) | (pos.x >= a.hx) |
+                                (pos.y >= a.hy) | (pos.z >= a.hz)
+                          : is_outside(pos, sd);
+    if (out) {
         uint4 seed = a.rnd[i];
         float4 r = rand4(seed);
         a.rnd[i] = seed;
-        if (a.dragged) {
+        if (dragged) {
             pos.x = rf_fma(r.x, sd.s[12], sd.s[8]);
             pos.y = rf_fma(r.y, sd.s[13], sd.s[9]);
             pos.z = rf_fma(r.z, sd.s[14], sd.s[10]);
@@ -1010,9 +1032,9 @@ __device__ __forceinline__ bool rain_move(const RainK &a, size_t i, float4 &pos,
     return false;
 }
 
-template <bool TI = false>
+template <bool TI = false, bool FAST = false>
 __device__ __forceinline__ void rain_gather(const RainK &a, float4 pos, const float4 *logtab, Gather3 &g, Gather3 *g2 = nullptr) {
-    float4 c = wind_index(pos, a.les, a.sd, logtab);
+    float4 c = wind_index<FAST>(pos, a.les, a.sd, logtab, a.cx, a.cy);
     gather3d(a.wind, c.x, c.y, c.z, g);
     if (TI) {
         Tex3 t2 = a.wind;
@@ -1021,8 +1043,9 @@ __device__ __forceinline__ void rain_gather(const RainK &a, float4 pos, const fl
     }
 }
 
+template <bool FAST = false>
 __device__ __forceinline__ void rain_drag(const RainK &a, float4 pos, float4 &vel, float4 bg) {
-    if (!a.dragged) {
+    if (!FAST && !a.dragged) {
         vel = bg;
         return;
     }
@@ -1062,12 +1085,15 @@ __device__ __forceinline__ void logtab_load(float4 *dst, const float *src) { rf_
 // Scatterers [lo, hi) of one rain population: advance and/or pulse terms into
 // the block's LDS gate sums acc (cleared by the caller).  The hi/lo variant
 // (96 B of state per scatterer, meant for populations far beyond the 256 MB
-// Infinity Cache) streams its state non-temporally.
-template <bool ADV, bool PULSE, bool HILO, bool WIN, bool TRI, bool TI>
+// Infinity Cache) streams its state non-temporally.  FAST: the common launch
+// (dragged, stretched wind index with non-negative growth, drop and angular
+// tables in LDS, binary64 phase) compiled without the branches for the others,
+// which keeps the loop's uniform values in scalar registers.
+template <bool ADV, bool PULSE, bool HILO, bool WIN, bool TRI, bool TI, bool FAST = false>
 __device__ __forceinline__ void rain_range_t(const RainK &a, size_t lo, size_t hi, double *acc, const float4 *logtab) {
     const float2 *wl = win_weights(acc, a.pk);
     const float *grl = win_gates(acc, a.pk);
-    const bool tl = PULSE && a.tab_lds;
+    const bool tl = PULSE && (FAST || a.tab_lds);
     const float4 *dropt = tl ? (const float4 *)((const char *)acc + a.tab_off) : a.drop;
     const float *angt = tl ? (const float *)(dropt + a.drop_n) : a.pk.angw;
     WaveWin win;
@@ -1103,7 +1129,7 @@ __device__ __forceinline__ void rain_range_t(const RainK &a, size_t lo, size_t h
         Gather3 g, g2;
         if (ADV && live) {
             const float4 p_in = p;
-            bool respawned = rain_move<HILO>(a, i, p, v, plo);
+            bool respawned = rain_move<HILO, FAST>(a, i, p, v, plo);
             if (HILO) st4<HILO>(&a.plo[i], plo);
             if (respawned) {
                 if (!stale) {
@@ -1115,7 +1141,7 @@ __device__ __forceinline__ void rain_range_t(const RainK &a, size_t lo, size_t h
             } else {
                 stale = false;
                 moved = true;
-                rain_gather<TI>(a, p, logtab, g, &g2);
+                rain_gather<TI, FAST>(a, p, logtab, g, &g2);
             }
             st4<HILO>(&a.pos[i], make_float4(p.x, p.y, p.z, stale ? -p.w : p.w));
         }
@@ -1125,13 +1151,13 @@ __device__ __forceinline__ void rain_range_t(const RainK &a, size_t lo, size_t h
             bool on = false;
             if (live) {
                 if (!have_rcs) rcs = stale ? a.rcs[i] : ellipsoid_rcs(p, dropt, a.drop_desc);
-                on = scat_signal<HILO>(p, rcs, a.pk, angt, sa, r, plo);
+                on = scat_signal<HILO, FAST>(p, rcs, a.pk, angt, sa, r, plo);
             }
             if (WIN) wwin_fold<TRI>(win, on, r, sa, a.pk, acc, wl, grl);
             else fold_gates(on, r, sa, a.pk, acc);
         }
         if (ADV && live) {
-            if (moved) rain_drag(a, p, v, TI ? lerp4(lerp3d(g), lerp3d(g2), a.wind.alpha) : lerp3d(g));
+            if (moved) rain_drag<FAST>(a, p, v, TI ? lerp4(lerp3d(g), lerp3d(g2), a.wind.alpha) : lerp3d(g));
             st4<HILO>(&a.vel[i], v);
         }
     }
@@ -1141,12 +1167,12 @@ __device__ __forceinline__ void rain_range_t(const RainK &a, size_t lo, size_t h
 // The fold strategy is a template flag so that the loop carrying the window
 // registers has no second fold path (which makes the compiler copy the 32
 // window registers at every step).
-template <bool ADV, bool PULSE, bool HILO, bool TI = false>
+template <bool ADV, bool PULSE, bool HILO, bool TI = false, bool FAST = false>
 __device__ __forceinline__ void rain_range(const RainK &a, size_t lo, size_t hi, double *acc, const float4 *logtab,
                                            bool use_win) {
-    if (use_win && a.pk.tri) rain_range_t<ADV, PULSE, HILO, true, true, TI>(a, lo, hi, acc, logtab);
-    else if (use_win) rain_range_t<ADV, PULSE, HILO, true, false, TI>(a, lo, hi, acc, logtab);
-    else rain_range_t<ADV, PULSE, HILO, false, false, TI>(a, lo, hi, acc, logtab);
+    if (use_win && a.pk.tri) rain_range_t<ADV, PULSE, HILO, true, true, TI, FAST>(a, lo, hi, acc, logtab);
+    else if (use_win) rain_range_t<ADV, PULSE, HILO, true, false, TI, FAST>(a, lo, hi, acc, logtab);
+    else rain_range_t<ADV, PULSE, HILO, false, false, TI, FAST>(a, lo, hi, acc, logtab);
 }
 
 // Many independent rain domains in one persistent launch.  items[t] = (domain,
@@ -1555,7 +1581,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 #ifndef RSD_RAIN_WAVES
 #define RSD_RAIN_WAVES 4    // waves per SIMD the fused rain kernel is compiled for
 #endif
-template <bool ADV, bool PULSE, bool HILO, bool TI = false>
+template <bool ADV, bool PULSE, bool HILO, bool TI = false, bool FAST = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RSD_RAIN_WAVES))) void k_rain(RainK a, StepDebris d) {
     extern __shared__ double acc[];
     __shared__ float4 logtab[RF_LOG_TABLE];
@@ -1581,7 +1607,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RSD_RAIN_WA
     const bool use_win = PULSE && a.pk.fold == RSD_FOLD_WINDOW;
     if (PULSE) stage_rain_tables(acc, a);
     if (use_win) stage_window(acc, a.pk);
-    rain_range<ADV, PULSE, HILO, TI>(a, lo, hi, acc, logtab, use_win);
+    rain_range<ADV, PULSE, HILO, TI, FAST>(a, lo, hi, acc, logtab, use_win);
     if (PULSE) acc_flush_row(acc, a.pk, a.pk.row0 + (int)L);
 }
 
@@ -2369,14 +2395,17 @@ static int blocks_for(const rsd_engine *e, size_t n) {
 // (occupancy of the launch's variant and LDS size, times the compute units).
 // Launches of a multiple of that many equal blocks run in whole rounds (no
 // block waits alone for a free slot at the end).
-static int rain_resident_blocks(rsd_engine *e, bool hilo, size_t lds) {
-    if (!e->rain_per_cu || e->rain_occ_hilo != (int)hilo || e->rain_occ_lds != lds) {
+static int rain_resident_blocks(rsd_engine *e, bool hilo, bool fast, size_t lds) {
+    const int key = (int)hilo | ((int)fast << 1);
+    if (!e->rain_per_cu || e->rain_occ_hilo != key || e->rain_occ_lds != lds) {
         int nb = 0;
-        const void *k = hilo ? reinterpret_cast<const void *>(&k_rain<true, true, true>)
-                             : reinterpret_cast<const void *>(&k_rain<true, true, false>);
+        const void *k = hilo ? (fast ? reinterpret_cast<const void *>(&k_rain<true, true, true, false, true>)
+                                     : reinterpret_cast<const void *>(&k_rain<true, true, true>))
+                             : (fast ? reinterpret_cast<const void *>(&k_rain<true, true, false, false, true>)
+                                     : reinterpret_cast<const void *>(&k_rain<true, true, false>));
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, e->block, lds) != hipSuccess || nb < 1) nb = 1;
         e->rain_per_cu = nb;
-        e->rain_occ_hilo = (int)hilo;
+        e->rain_occ_hilo = key;
         e->rain_occ_lds = lds;
     }
     return e->rain_per_cu * e->cus;
@@ -2579,7 +2608,27 @@ static int fill_rain(rsd_engine *e, const rsd_step *s, RainK &a) {
     a.drop_desc = e->drop_desc;
     a.dragged = s ? (s->background_kind == RSD_ADVANCE_DRAGGED) : 1u;
     a.logtab = e->logtab;
+    a.hx = a.sd.s[8] + a.sd.s[12];
+    a.hy = a.sd.s[9] + a.sd.s[13];
+    a.hz = a.sd.s[10] + a.sd.s[14];
+    a.cx = a.sd.s[8] + 0.5f * a.sd.s[12];
+    a.cy = a.sd.s[9] + 0.5f * a.sd.s[13];
     return RSD_OK;
+}
+
+// The fused advance + pulse launches that the specialised rain kernel (k_rain
+// FAST) covers: dragged rain, a stretched wind table with non-negative growth
+// factors, drop and angular tables that fit in LDS and a binary64 phase (or
+// hi/lo positions).
+static bool rain_fast(const rsd_engine *e, const rsd_step *s, const rsd_pulse *p) {
+    if (!s || !p || s->background_kind != RSD_ADVANCE_DRAGGED) return false;
+    const F16 &l = e->les[s->wind_slot];
+    uint32_t spacing;
+    memcpy(&spacing, &l.s[7], sizeof(spacing));
+    if (spacing != 7u || !(l.s[4] >= 0.0f && l.s[5] >= 0.0f && l.s[6] >= 0.0f)) return false;
+    const int dn = (int)e->drop_desc.s[2] + 1, an = (int)e->angw_desc.s[2] + 1;
+    if (dn < 1 || dn > RSD_LDS_DROP_MAX || an < 1 || an > RSD_LDS_ANGW_MAX) return false;
+    return e->plo != NULL || p->phase_mode == RSD_PHASE_F64;
 }
 
 // Stage the drop RCS and angular weight tables of a rain pulse launch in LDS
@@ -2686,7 +2735,7 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p, con
         const size_t lds_est = (size_t)4 * R * sizeof(double) + (size_t)e->rangew_n * sizeof(float2) +
                                (size_t)R * sizeof(float) + 16 + (size_t)RSD_LDS_DROP_MAX * sizeof(float4) +
                                (size_t)RSD_LDS_ANGW_MAX * sizeof(float);
-        const int res = rain_resident_blocks(e, e->plo != NULL, lds_est);
+        const int res = rain_resident_blocks(e, e->plo != NULL, rain_fast(e, s, p), lds_est);
         // riders taking a round or more: the rain keeps the default grid
         const int round = ride_blocks < res ? 2 * res - ride_blocks : rain_blocks;
         const size_t need = (e->n_rain + 63) / 64;   // at least a wave per block
@@ -2820,7 +2869,13 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p, con
         // queue's processing of separate event packets.
         hipEvent_t t0 = timed ? e->prof_ev[2 * e->prof_n] : nullptr, t1 = timed ? e->prof_ev[2 * e->prof_n + 1] : nullptr;
         const bool ti = a.wind.d2 != nullptr;
-        if (e->plo && ti) {
+        const bool fast = rain_fast(e, s, p) && a.tab_lds;
+        if (fast) {
+            if (e->plo && ti) hipExtLaunchKernelGGL((k_rain<true, true, true, true, true>), grid, dim3(e->block), lds_rain, e->stream, t0, t1, 0, a, d);
+            else if (ti) hipExtLaunchKernelGGL((k_rain<true, true, false, true, true>), grid, dim3(e->block), lds_rain, e->stream, t0, t1, 0, a, d);
+            else if (e->plo) hipExtLaunchKernelGGL((k_rain<true, true, true, false, true>), grid, dim3(e->block), lds_rain, e->stream, t0, t1, 0, a, d);
+            else hipExtLaunchKernelGGL((k_rain<true, true, false, false, true>), grid, dim3(e->block), lds_rain, e->stream, t0, t1, 0, a, d);
+        } else if (e->plo && ti) {
             if (s && p) hipExtLaunchKernelGGL((k_rain<true, true, true, true>), grid, dim3(e->block), lds_rain, e->stream, t0, t1, 0, a, d);
             else if (s) hipExtLaunchKernelGGL((k_rain<true, false, true, true>), grid, dim3(e->block), 0, e->stream, t0, t1, 0, a, d);
             else hipExtLaunchKernelGGL((k_rain<false, true, true>), grid, dim3(e->block), lds_rain, e->stream, t0, t1, 0, a, d);
