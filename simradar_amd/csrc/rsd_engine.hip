@@ -797,6 +797,23 @@ struct WaveWin {
     int kb;                 // wave-uniform first gate; RSD_WIN_EMPTY: no window
 };
 
+// Bitwise OR over the wave (row_shr 1/2/4/8 within each row of 16 lanes, then
+// row_bcast 15 / 31 into lane 63), returned wave-uniform.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint32_t dpp_or(uint32_t v) {
+    return v | (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xf, false);
+}
+
+__device__ __forceinline__ uint32_t wave_or(uint32_t v) {
+    v = dpp_or<0x111, 0xf>(v);
+    v = dpp_or<0x112, 0xf>(v);
+    v = dpp_or<0x114, 0xf>(v);
+    v = dpp_or<0x118, 0xf>(v);
+    v = dpp_or<0x142, 0xa>(v);
+    v = dpp_or<0x143, 0xc>(v);
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
 __device__ __forceinline__ void wwin_clear(WaveWin &w) {
 #pragma unroll
     for (int j = 0; j < 4 * RSD_WWIN; j++) w.a[j] = 0.0f;
@@ -834,7 +851,10 @@ __device__ __forceinline__ void wwin_flush(const WaveWin &w, const PulseK &k, do
 // launch's range-ordered scatterers then stay inside it, and those that do not
 // fold directly.  The slots a lane touches need no mask: outside a scatterer's
 // support the table weight is exactly 0 (the window is only used for tables
-// that end in zeros).
+// that end in zeros), and lanes outside the window carry a zero signal; so the
+// wave visits the union of its window lanes' slots (one OR over the wave of
+// per-lane slot bits) and every lane adds its term there.  A visited slot lies
+// inside some lane's [klo, khi], hence inside [0, R).
 __device__ __forceinline__ int median3(int a, int b, int c) { return max(min(a, b), min(max(a, b), c)); }
 
 template <bool TRI>
@@ -854,11 +874,11 @@ __device__ __forceinline__ void wwin_fold(WaveWin &w, bool on, float r, float4 s
     const bool fit = on && klo >= w.kb && khi < w.kb + RSD_WWIN;
     if (on && !fit) lane_fold(r, sa, klo, khi, k, acc, wl, grl);
     if (!fit) sa = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const uint32_t used = wave_or(fit ? ((2u << (khi - klo)) - 1u) << (klo - w.kb) : 0u);
 #pragma unroll
     for (int s = 0; s < RSD_WWIN; s++) {
+        if (!(used & (1u << s))) continue;
         const int g = w.kb + s;
-        if (g < 0 || g >= k.R) continue;
-        if (__ballot(fit && klo <= g && g <= khi) == 0) continue;
         const float x = r - grl[g];
         const float wr = TRI ? tri_weight(x, k) : lds_weight(x, wl, k.rangew_desc);
         w.a[4 * s + 0] = __builtin_fmaf(wr, sa.x, w.a[4 * s + 0]);
