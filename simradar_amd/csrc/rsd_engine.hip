@@ -1184,15 +1184,115 @@ __device__ __forceinline__ void rain_range_t(const RainK &a, size_t lo, size_t h
     if (WIN) wwin_flush(win, a.pk, acc);
 }
 
+// Position and velocity streams through buffer descriptors (gfx9 raw buffers:
+// 32-bit byte offsets from a scalar base, range-checked against the stream's
+// size; word 3 selects 32-bit data).  A lane past the end passes
+// RSD_BUF_OOB: its load returns zeros and its store is dropped, so these
+// accesses need no branch around them.
+typedef unsigned int rsd_u4 __attribute__((ext_vector_type(4)));
+#define RSD_BUF_OOB 0x80000000u
+#define RSD_BUF_MAX_BYTES 0x7ffffff0ull    // streams the descriptors cover (with RSD_BUF_OOB beyond)
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t stream_rsrc(const float4 *p, unsigned long long n) {
+    return __builtin_amdgcn_make_buffer_rsrc((void *)p, 0, (int)(n * sizeof(float4)), 0x00020000);
+}
+
+__device__ __forceinline__ float4 bld4(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    const rsd_u4 v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
+    return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w));
+}
+
+__device__ __forceinline__ void bst4(__amdgpu_buffer_rsrc_t r, uint32_t off, float4 x) {
+    const rsd_u4 v = {__float_as_uint(x.x), __float_as_uint(x.y), __float_as_uint(x.z), __float_as_uint(x.w)};
+    __builtin_amdgcn_raw_buffer_store_b128(v, r, (int)off, 0, 0);
+}
+
+// The specialised fused advance + pulse over binary32 positions: the same
+// steps and results as rain_range_t<true, true, false, WIN, TRI, TI, true>,
+// with the next scatterer's pos / vel reads, the wind texel reads and the pos /
+// vel writes issued by every lane (a dead lane's go out of range, a respawned
+// lane's texels are read and not used), so no branch surrounds them and each
+// wait on a read waits for that read alone, not for the writes after it.
+// Stream indices fit 32 bits (the launch checks RSD_BUF_MAX_BYTES).
+template <bool WIN, bool TRI, bool TI>
+__device__ __forceinline__ void rain_range_buf(const RainK &a, size_t lo, size_t hi, double *acc, const float4 *logtab) {
+    const float2 *wl = win_weights(acc, a.pk);
+    const float *grl = win_gates(acc, a.pk);
+    const float4 *dropt = (const float4 *)((const char *)acc + a.tab_off);
+    const float *angt = (const float *)(dropt + a.drop_n);
+    const __amdgpu_buffer_rsrc_t rp = stream_rsrc(a.pos, a.n), rv = stream_rsrc(a.vel, a.n);
+    const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    WaveWin win;
+    win.kb = RSD_WIN_EMPTY;
+    wwin_clear(win);
+    const uint32_t lo32 = (uint32_t)lo, hi32 = (uint32_t)hi, bd = blockDim.x;
+    const uint32_t i0 = lo32 + threadIdx.x;
+    float4 pn = bld4(rp, i0 < hi32 ? i0 * 16u : RSD_BUF_OOB);
+    float4 vn = bld4(rv, i0 < hi32 ? i0 * 16u : RSD_BUF_OOB);
+    for (uint32_t base = lo32; base < hi32; base += bd) {
+        const uint32_t i = base + threadIdx.x;
+        const bool live = i < hi32;
+        const uint32_t off = live ? i * 16u : RSD_BUF_OOB;
+        float4 p = pn, v = vn;
+        const uint32_t nx = i + bd;
+        const uint32_t noff = nx < hi32 ? nx * 16u : RSD_BUF_OOB;
+        pn = bld4(rp, noff);
+        vn = bld4(rv, noff);
+        bool stale = __builtin_signbit(p.w);
+        p.w = __builtin_fabsf(p.w);
+        float4 rcs = zero, plo = zero;
+        bool have_rcs = false, moved = false;
+        if (live) {
+            const float4 p_in = p;
+            if (rain_move<false, true>(a, (size_t)i, p, v, plo)) {
+                // respawned: the pulse takes the rcs of the position it left
+                // (stored now, or kept from a respawn the step before)
+                if (!stale) {
+                    rcs = ellipsoid_rcs(p_in, dropt, a.drop_desc);
+                    a.rcs[i] = rcs;
+                    stale = true;
+                } else {
+                    rcs = a.rcs[i];
+                }
+                have_rcs = true;
+            } else {
+                stale = false;
+                moved = true;
+            }
+        }
+        Gather3 g, g2;
+        rain_gather<TI, true>(a, p, logtab, g, &g2);
+        bst4(rp, off, make_float4(p.x, p.y, p.z, stale ? -p.w : p.w));
+        float4 sa = zero;
+        float r = 0.0f;
+        bool on = false;
+        if (live) {
+            if (!have_rcs) rcs = ellipsoid_rcs(p, dropt, a.drop_desc);     // moved: not stale
+            on = scat_signal<false, true>(p, rcs, a.pk, angt, sa, r);
+        }
+        if (WIN) wwin_fold<TRI>(win, on, r, sa, a.pk, acc, wl, grl);
+        else fold_gates(on, r, sa, a.pk, acc);
+        if (moved) rain_drag<true>(a, p, v, TI ? lerp4(lerp3d(g), lerp3d(g2), a.wind.alpha) : lerp3d(g));
+        bst4(rv, off, v);
+    }
+    if (WIN) wwin_flush(win, a.pk, acc);
+}
+
 // The fold strategy is a template flag so that the loop carrying the window
 // registers has no second fold path (which makes the compiler copy the 32
 // window registers at every step).
 template <bool ADV, bool PULSE, bool HILO, bool TI = false, bool FAST = false>
 __device__ __forceinline__ void rain_range(const RainK &a, size_t lo, size_t hi, double *acc, const float4 *logtab,
                                            bool use_win) {
-    if (use_win && a.pk.tri) rain_range_t<ADV, PULSE, HILO, true, true, TI, FAST>(a, lo, hi, acc, logtab);
-    else if (use_win) rain_range_t<ADV, PULSE, HILO, true, false, TI, FAST>(a, lo, hi, acc, logtab);
-    else rain_range_t<ADV, PULSE, HILO, false, false, TI, FAST>(a, lo, hi, acc, logtab);
+    if constexpr (FAST && ADV && PULSE && !HILO) {
+        if (use_win && a.pk.tri) rain_range_buf<true, true, TI>(a, lo, hi, acc, logtab);
+        else if (use_win) rain_range_buf<true, false, TI>(a, lo, hi, acc, logtab);
+        else rain_range_buf<false, false, TI>(a, lo, hi, acc, logtab);
+    } else {
+        if (use_win && a.pk.tri) rain_range_t<ADV, PULSE, HILO, true, true, TI, FAST>(a, lo, hi, acc, logtab);
+        else if (use_win) rain_range_t<ADV, PULSE, HILO, true, false, TI, FAST>(a, lo, hi, acc, logtab);
+        else rain_range_t<ADV, PULSE, HILO, false, false, TI, FAST>(a, lo, hi, acc, logtab);
+    }
 }
 
 // Many independent rain domains in one persistent launch.  items[t] = (domain,
@@ -2648,6 +2748,7 @@ static bool rain_fast(const rsd_engine *e, const rsd_step *s, const rsd_pulse *p
     if (spacing != 7u || !(l.s[4] >= 0.0f && l.s[5] >= 0.0f && l.s[6] >= 0.0f)) return false;
     const int dn = (int)e->drop_desc.s[2] + 1, an = (int)e->angw_desc.s[2] + 1;
     if (dn < 1 || dn > RSD_LDS_DROP_MAX || an < 1 || an > RSD_LDS_ANGW_MAX) return false;
+    if (!e->plo && (unsigned long long)e->n_rain * sizeof(float4) > RSD_BUF_MAX_BYTES) return false;
     return e->plo != NULL || p->phase_mode == RSD_PHASE_F64;
 }
 

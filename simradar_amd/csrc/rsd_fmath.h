@@ -116,9 +116,13 @@ __device__ __forceinline__ float rf_log1pf(float x, const float4 *t4) {
 // rf_log1pf for finite x >= 0 (the stretched wind index): the same series
 // below 2^-7 and the same table form above, without the branches for the
 // cases such an argument cannot take (equal results there: the series returns
-// +0 for +0).
+// +0 for +0).  Both forms are evaluated and one selected (the table row read
+// is in range for every such x), so the three axes of an index interleave
+// instead of each waiting on its own table read inside a branch.  The row's
+// fourth word (+0) is added to the series, which leaves it unchanged for x >=
+// 0 and gives the row read a use on both sides of the select, so the compiler
+// keeps the read out of a branch.
 __device__ __forceinline__ float rf_log1p_pos(float x, const float4 *t4) {
-    if (x < 0x1p-7f) return rf_fma(x * x, rf_fma(x, rf_fma(x, -0.25f, 0x1.555556p-2f), -0.5f), x);
     float u = 1.0f + x;
     float tail = (u < 0x1p+24f) ? x - (u - 1.0f) : 0.0f;
     uint32_t iu = __float_as_uint(u);
@@ -133,7 +137,8 @@ __device__ __forceinline__ float rf_log1p_pos(float x, const float4 *t4) {
     float scale = __uint_as_float((uint32_t)(127 - k) << 23);
     float lo = rf_fma(kf, RF_LN2_LO, row.z) + (tail * g) * scale;
     float hi = rf_fma(kf, RF_LN2_HI, row.y);
-    return hi + (p + lo);
+    const float series = rf_fma(x * x, rf_fma(x, rf_fma(x, -0.25f, 0x1.555556p-2f), -0.5f), x) + row.w;
+    return x < 0x1p-7f ? series : hi + (p + lo);
 }
 
 // Rows (g_j, head_j, tail_j, 0) of the log1p table for rf_log1p_pos, copied by
