@@ -85,6 +85,8 @@ def main():
     ap.add_argument("--e2e-steps", type=int, default=200)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--max-blocks", type=int, default=0)
+    ap.add_argument("--sort", type=int, default=-1,
+                    help="pulses between range reorders of the rain streams (-1: the engine default)")
     ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"])
     args = ap.parse_args()
     if args.gpus != 1:
@@ -106,6 +108,8 @@ def main():
     r = wl.make(cfg)
     if args.block or args.max_blocks:
         r.set_launch(args.block, args.max_blocks)
+    if args.sort >= 0:
+        r.set_sort(args.sort)
     R = int(r.params().range_count)
 
     cpu = None

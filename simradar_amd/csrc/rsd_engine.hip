@@ -936,6 +936,14 @@ __device__ __forceinline__ void acc_flush_row(const double *acc, const PulseK &k
 // Rain (type 0): el_atts (rs.cl:653-739) or bg_atts (rs.cl:533-580), fused with
 // the pulse.  Template flags select advance and/or pulse.
 // ---------------------------------------------------------------------------
+// A pulse's partial-sum columns (rows binary64 values each, one per block of
+// its launch) and the output slot(s) of their sums.
+struct FinK {
+    const double *partials;
+    double *out, *out2;
+    int rows, n4r;
+};
+
 struct RainK {
     float4 *pos, *vel, *rcs;
     float4 *plo;            // low parts of the positions (hi/lo mode), else NULL
@@ -959,6 +967,10 @@ struct RainK {
     // corner sd[8..10] + sd[12..14] of the box and the domain centre sd[8..9] +
     // 0.5 sd[12..13] of the stretched wind index.
     float hx, hy, hz, cx, cy;
+    // The previous pulse's finalize, run by the last fin_blocks blocks of this
+    // launch (after the rain blocks); fin_blocks = 0: none.
+    FinK fin;
+    unsigned fin_blocks;
 };
 
 // Byte offset of the staged rain tables in the dynamic LDS of a pulse launch:
@@ -1347,23 +1359,21 @@ __device__ __forceinline__ double column_sum(const double *col, int rows) {
     return ((red[0] + red[1]) + red[2]) + red[3];
 }
 
-struct FinK {
-    const double *partials;
-    double *out, *out2;
-    int rows, n4r;
-};
-
-__global__ __launch_bounds__(256) void k_finalize_batch(const FinK *f, const int *first, int nd) {
-    int d = 0;
-    while (d + 1 < nd && first[d + 1] <= (int)blockIdx.x) d++;
-    const FinK a = f[d];
-    const int j = (int)blockIdx.x - first[d];
+// Column j of a pulse's partial sums into its output slot(s): (Ih, Qh, Iv, Qv)
+// of gate j % R, component j / R.
+__device__ __forceinline__ void finalize_column(const FinK &a, int j) {
     const int R = a.n4r / 4;
     const double s = column_sum(a.partials + (size_t)j * a.rows, a.rows);
     if (threadIdx.x == 0) {
         a.out[4 * (j % R) + j / R] = s;
         if (a.out2) a.out2[4 * (j % R) + j / R] = s;
     }
+}
+
+__global__ __launch_bounds__(256) void k_finalize_batch(const FinK *f, const int *first, int nd) {
+    int d = 0;
+    while (d + 1 < nd && first[d + 1] <= (int)blockIdx.x) d++;
+    finalize_column(f[d], (int)blockIdx.x - first[d]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1705,8 +1715,14 @@ template <bool ADV, bool PULSE, bool HILO, bool TI = false, bool FAST = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RSD_RAIN_WAVES))) void k_rain(RainK a, StepDebris d) {
     extern __shared__ double acc[];
     __shared__ float4 logtab[RF_LOG_TABLE];
-    if (ADV) logtab_load(logtab, a.logtab);
     const unsigned nd = (unsigned)d.first[d.nt];
+    const unsigned nr = gridDim.x - a.fin_blocks;      // end of the rain blocks
+    if (blockIdx.x >= nr) {     // the previous pulse's finalize, in the slots the last rain blocks free
+        const int j = (int)(blockIdx.x - nr);
+        if (j < a.fin.n4r) finalize_column(a.fin, j);
+        return;
+    }
+    if (ADV) logtab_load(logtab, a.logtab);
     if (blockIdx.x < nd) {
         const int j = step_type(d);
         const DebrisK &da = d.t[j];
@@ -1723,7 +1739,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RSD_RAIN_WA
     if (PULSE) acc_clear(acc, 4 * a.pk.R);
     else __syncthreads();
     size_t lo, hi;
-    const unsigned L = block_range_at(0, a.n, blockIdx.x - nd, gridDim.x - nd, lo, hi, true);
+    const unsigned L = block_range_at(0, a.n, blockIdx.x - nd, nr - nd, lo, hi, true);
     const bool use_win = PULSE && a.pk.fold == RSD_FOLD_WINDOW;
     if (PULSE) stage_rain_tables(acc, a);
     if (use_win) stage_window(acc, a.pk);
@@ -1823,13 +1839,13 @@ __global__ __launch_bounds__(256) void k_fixed(FixedK a) {
 // reading its contiguous column.  Columns are [component][gate]; the pulse
 // buffer is [gate][component].
 __global__ __launch_bounds__(256) void k_finalize(const double *partials, int rows, int n4r, double *out, double *out2) {
-    const int j = blockIdx.x;
-    const int R = n4r / 4;
-    const double s = column_sum(partials + (size_t)j * rows, rows);
-    if (threadIdx.x == 0) {
-        out[4 * (j % R) + j / R] = s;
-        if (out2) out2[4 * (j % R) + j / R] = s;
-    }
+    FinK a;
+    a.partials = partials;
+    a.out = out;
+    a.out2 = out2;
+    a.rows = rows;
+    a.n4r = n4r;
+    finalize_column(a, (int)blockIdx.x);
 }
 
 // Pulse-pair moments of one gate over the last M pulses of the IQ ring
@@ -2231,6 +2247,15 @@ struct rsd_engine {
     double gate_drift;      // max |gate_r[k] - (start + k delta)| of the binary32 sums
     double *partials;
     size_t partial_cap;
+    // Rain pulses alternate between partials and partials2 (part_cur: the one the
+    // next writes), so that the finalize of pulse k, run by the first blocks of
+    // pulse k + 1's launch, reads the other buffer.  fin_on: a rain pulse whose
+    // finalize has not been launched yet (fin holds its arguments).
+    double *partials2;
+    size_t partial_cap2;
+    int part_cur;
+    int fin_on;
+    FinK fin;
     double *pulse_d;         // device address of pulse_h: the finalize writes the
     double *pulse_h;         // pulse straight into pinned host memory
     int have_pending;
@@ -2285,6 +2310,7 @@ struct rsd_batch {
 };
 
 static int flush_pending(rsd_engine *e);
+static int flush_finalize(rsd_engine *e);
 static void join_batch(rsd_engine *e);
 extern "C" int rsd_batch_launch(struct rsd_batch *b);
 
@@ -2435,7 +2461,7 @@ extern "C" void rsd_close(rsd_engine *e) {
     for (int t = 0; t < RSD_MAX_TABLES; t++) {
         dfree(e->adm_cd[t]); dfree(e->adm_cm[t]); dfree(e->rcs_re[t]); dfree(e->rcs_im[t]);
     }
-    dfree(e->drop); dfree(e->dff); dfree(e->rangew); dfree(e->angw); dfree(e->gate_r); dfree(e->partials);
+    dfree(e->drop); dfree(e->dff); dfree(e->rangew); dfree(e->angw); dfree(e->gate_r); dfree(e->partials); dfree(e->partials2);
     dfree(e->logtab);
     dfree(e->ring);
     dfree(e->mom_d);
@@ -2476,6 +2502,10 @@ extern "C" int rsd_alloc_state(rsd_engine *e, size_t num_scats, const size_t *or
     }
     if (total != num_scats || origins[0] != 0) return fail(e, RSD_ERR_ARG, "type counts (%zu) != num_scats (%zu)", total, num_scats);
     HIPCHK(e, hipSetDevice(e->device));
+    {
+        int st = flush_finalize(e);
+        if (st) return st;
+    }
     HIPCHK(e, hipStreamSynchronize(e->stream));
     free_state(e);
     e->n = num_scats;
@@ -2531,14 +2561,29 @@ static int rain_resident_blocks(rsd_engine *e, bool hilo, bool fast, size_t lds)
     return e->rain_per_cu * e->cus;
 }
 
-static int ensure_partials(rsd_engine *e, size_t rows, int R) {
+// Partial-sum buffer `which` (0: partials, 1: partials2) for rows x 4 R values.
+static int ensure_partials(rsd_engine *e, size_t rows, int R, int which = 0) {
+    double **buf = which ? &e->partials2 : &e->partials;
+    size_t *cap = which ? &e->partial_cap2 : &e->partial_cap;
     size_t need = rows * 4 * (size_t)R;
-    if (need > e->partial_cap) {
-        dfree(e->partials);
-        e->partials = NULL;
-        HIPCHK(e, hipMalloc((void **)&e->partials, need * sizeof(double)));
-        e->partial_cap = need;
+    if (need > *cap) {
+        dfree(*buf);
+        *buf = NULL;
+        HIPCHK(e, hipMalloc((void **)buf, need * sizeof(double)));
+        *cap = need;
     }
+    return RSD_OK;
+}
+
+// Launch the finalize of the last rain pulse if its launch has not run yet
+// (the pulse's output and IQ ring slot are complete once the stream reaches
+// this point).
+static int flush_finalize(rsd_engine *e) {
+    if (!e->fin_on) return RSD_OK;
+    e->fin_on = 0;
+    hipLaunchKernelGGL(k_finalize, dim3(e->fin.n4r), dim3(256), 0, e->stream, e->fin.partials, e->fin.rows, e->fin.n4r,
+                       e->fin.out, e->fin.out2);
+    HIPCHK(e, hipGetLastError());
     return RSD_OK;
 }
 
@@ -2827,6 +2872,15 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p, con
     join_batch(e);
     const int R = p ? (int)p->range_count : 0;
     if (p && (R < 1 || R > RSD_MAX_GATES)) return fail(e, RSD_ERR_ARG, "range_count %d out of range", R);
+    // A rain pulse (k_rain) runs the previous rain pulse's finalize in its first
+    // blocks and leaves its own to the next one; every other launch finalizes
+    // what is pending first and its own pulse at once.
+    const bool fixed_bg = s ? s->background_kind == RSD_ADVANCE_FIXED : e->bg_fixed != 0;
+    const bool defer = p && !sp && e->n_rain && !fixed_bg;
+    if (!defer) {
+        int st = flush_finalize(e);
+        if (st) return st;
+    }
     // Rows of the partial-sum buffer: one per block of every launch, rain
     // first, then each debris type in type order.
     int deb_blocks[RSD_MAX_TYPES] = {0};
@@ -2880,10 +2934,11 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p, con
     PulseK pk;
     memset(&pk, 0, sizeof(pk));
     if (p) {
-        int st = sp ? RSD_OK : ensure_partials(e, (size_t)rows, R);
+        int st = sp ? RSD_OK : ensure_partials(e, (size_t)rows, R, defer ? e->part_cur : 0);
         if (st) return st;
         st = fill_pulse(e, p, pk, sp);
         if (st) return st;
+        if (defer && e->part_cur) pk.partials = e->partials2;
         pk.rows = rows;
         if (sp) {
             hipLaunchKernelGGL(k_gate_ranges, dim3(1), dim3(64), 0, e->stream, p->range_start, p->range_delta, R,
@@ -2983,7 +3038,12 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p, con
             d.nt++;
         }
         const size_t lds_rain = p ? set_rain_tables(e, a, R, lds) : lds;
-        const dim3 grid(ride_blocks + rain_blocks);
+        if (e->fin_on) {        // defer holds here (only rain pulses leave a finalize pending)
+            a.fin = e->fin;
+            a.fin_blocks = (unsigned)e->fin.n4r;
+            e->fin_on = 0;
+        }
+        const dim3 grid(ride_blocks + a.fin_blocks + rain_blocks);
         const bool timed = e->prof_on && e->prof_n < RSD_PROF_CAP && (e->prof_tick++ % e->prof_stride) == 0;
         // A timed launch carries its events in the dispatch itself
         // (hipExtLaunchKernelGGL): they bracket the kernel's execution, not the
@@ -3034,9 +3094,19 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p, con
             e->ring_next = (e->ring_next + 1) % e->ring_m;
             if (e->ring_count < e->ring_m) e->ring_count++;
         }
-        hipLaunchKernelGGL(k_finalize, dim3(4 * R), dim3(256), 0, e->stream, (const double *)e->partials, rows, 4 * R,
-                           e->pulse_d, slot);
-        HIPCHK(e, hipGetLastError());
+        if (defer) {
+            e->fin.partials = pk.partials;
+            e->fin.rows = rows;
+            e->fin.n4r = 4 * R;
+            e->fin.out = e->pulse_d;
+            e->fin.out2 = slot;
+            e->fin_on = 1;
+            e->part_cur ^= 1;
+        } else {
+            hipLaunchKernelGGL(k_finalize, dim3(4 * R), dim3(256), 0, e->stream, (const double *)e->partials, rows,
+                               4 * R, e->pulse_d, slot);
+            HIPCHK(e, hipGetLastError());
+        }
         e->last_pulse = *p;
         e->have_pulse = 1;
         e->sigaux_saved = 0;
@@ -3194,6 +3264,8 @@ extern "C" int rsd_set_iq_ring(rsd_engine *e, int pulses, int range_count) {
     int st0 = flush_pending(e);
     if (st0) return st0;
     join_batch(e);
+    st0 = flush_finalize(e);        // its ring slot is in the ring freed below
+    if (st0) return st0;
     HIPCHK(e, hipStreamSynchronize(e->stream));
     dfree(e->ring);
     dfree(e->mom_d);
@@ -3218,6 +3290,8 @@ extern "C" int rsd_moments(rsd_engine *e, int pulses, const rsd_moment_params *m
     if (e->ring_count < pulses) return fail(e, RSD_ERR_STATE, "only %d pulses recorded", e->ring_count);
     HIPCHK(e, hipSetDevice(e->device));
     join_batch(e);
+    st = flush_finalize(e);
+    if (st) return st;
     MomK a;
     a.ring = e->ring;
     a.ring_m = e->ring_m;
@@ -3639,6 +3713,8 @@ extern "C" int rsd_read_pulse(rsd_engine *e, double *out, uint32_t R) {
     if (!e->have_pulse) return fail(e, RSD_ERR_STATE, "no pulse made yet");
     if (R != e->last_pulse.range_count) return fail(e, RSD_ERR_ARG, "range_count %u != %u of the last pulse", R, e->last_pulse.range_count);
     HIPCHK(e, hipSetDevice(e->device));
+    int st = flush_finalize(e);
+    if (st) return st;
     HIPCHK(e, hipStreamSynchronize(e->stream));   // the finalize wrote pulse_h itself
     memcpy(out, e->pulse_h, (size_t)4 * R * sizeof(double));
     return RSD_OK;
@@ -3650,6 +3726,8 @@ extern "C" int rsd_sync(rsd_engine *e) {
     int st = flush_pending(e);
     if (st) return st;
     join_batch(e);
+    st = flush_finalize(e);
+    if (st) return st;
     HIPCHK(e, hipStreamSynchronize(e->stream));
     return RSD_OK;
 }
@@ -3797,6 +3875,9 @@ extern "C" int rsd_batch_create(rsd_engine *const *engines, int count, rsd_batch
         for (int j = 0; j < i; j++)
             if (engines[j] == engines[i]) return RSD_ERR_ARG;
     }
+    if (hipSetDevice(dev) != hipSuccess) return RSD_ERR_HIP;
+    for (int i = 0; i < count; i++)
+        if (flush_finalize(engines[i]) != RSD_OK) return RSD_ERR_HIP;
     rsd_batch *b = (rsd_batch *)calloc(1, sizeof(rsd_batch));
     if (!b) return RSD_ERR_ALLOC;
     b->eng = (rsd_engine **)calloc((size_t)count, sizeof(rsd_engine *));
@@ -3828,8 +3909,9 @@ extern "C" int rsd_batch_create(rsd_engine *const *engines, int count, rsd_batch
     (void)hipEventRecord(b->ev_launch, b->stream);
     for (int i = 0; i < count; i++) {
         rsd_engine *e = engines[i];
-        // wait for whatever the engine's stream still holds, then stagger the
-        // domains' range reorders over the sort interval
+        // wait for whatever the engine's stream still holds (its last pulse's
+        // finalize included), then stagger the domains' range reorders over the
+        // sort interval
         (void)hipEventRecord(e->ev_fork, e->stream);
         (void)hipStreamWaitEvent(b->stream, e->ev_fork, 0);
         if (e->sort_interval > 0 && e->permuted) e->since_sort = (int)((long)i * e->sort_interval / count);
