@@ -172,7 +172,7 @@ int rsd_io_test(rsd_engine *e, size_t n, int reps, double *gbps);
  * of the riding debris types); other launches take at most 2048 blocks. */
 int rsd_set_launch(rsd_engine *e, int block_size, int max_blocks);
 
-/* Spatial order of the rain streams: every `interval` pulses (default 32,
+/* Spatial order of the rain streams: every `interval` pulses (default 64,
  * 0 = never) the engine reorders the stored rain scatterers by range shell
  * (1/8 gate) and box cell.  Invisible to callers: downloads return the
  * reference order and uploads restore it.  No counterpart in the reference

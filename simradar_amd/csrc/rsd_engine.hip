@@ -2383,7 +2383,7 @@ extern "C" int rsd_open(int device_index, rsd_engine **out) {
     e->block = 256;
     e->max_blocks = 0;
     if (rsd_device_props(device_index, &e->cus, NULL, NULL) != RSD_OK || e->cus < 1) e->cus = 1;
-    e->sort_interval = 32;
+    e->sort_interval = 64;
     e->fold = RSD_FOLD_WINDOW;
     if (hipSetDevice(device_index) != hipSuccess || hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking) != hipSuccess ||

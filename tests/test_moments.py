@@ -119,6 +119,35 @@ def test_gpu_moments_match_oracle_estimator_and_oracle_iq(gpu_ready):
 
 
 @pytest.mark.gpu
+def test_unread_pulses_fill_the_ring_as_read_ones(gpu_ready):
+    """A rain pulse's gate sums are finished by the next rain launch (or by the
+    first read): pulses made back to back without a read between them give
+    the same IQ ring, moments and last pulse, bit for bit, as pulses read one
+    by one."""
+    cfg = h.Config(n_rain=16384, debris=[(sa.OBJ_LEAF, 512)], scan="P:3.0,-1:1:0.01")
+    M = 12
+    runs = []
+    for read_each in (True, False):
+        r = sa.Radar(verb=0)
+        h.configure(r, cfg)
+        r.populate()
+        r.set_moment_pulses(M)
+        for k in range(M):
+            r.make_pulse()
+            if read_each:
+                r.download_pulse_only()
+            r.advance_time()
+            r.advance_beam()
+        r.download_pulse_only()
+        r.check()
+        runs.append((r.pulse_f64().copy(), r.moments(M).copy()))
+        r.free()
+    (p_a, m_a), (p_b, m_b) = runs
+    assert np.array_equal(h.bits(p_a), h.bits(p_b))
+    assert np.array_equal(m_a.view(np.uint32), m_b.view(np.uint32))
+
+
+@pytest.mark.gpu
 def test_gpu_moments_need_enough_pulses(gpu_ready):
     cfg = h.Config(n_rain=4096, scan="P:3.0,-1:1:0.01")
     r = sa.Radar(verb=0)
