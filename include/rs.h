@@ -113,6 +113,8 @@ typedef struct _rs_worker {
     cl_float16        les_desc;
     unsigned int      les_id;
     cl_float16        dff_desc;
+    float             *dff_icdf[2];   /* host copies of the flux field of each LES slot */
+    int               dff_count[2];
     size_t            mem_size;
     size_t            mem_usage;
 } RSWorker;
@@ -387,6 +389,10 @@ const float *RS_get_les_cpxx(RSHandle *H, int frame, uint32_t *dims3);
 int RS_get_debris_table(RSHandle *H, int t, int which, float *out4, uint32_t *dims2);
 /* addition: DSD bins (radius, pdf, cdf; count floats each); returns count. */
 int RS_get_dsd(RSHandle *H, float *r, float *pdf, float *cdf, int cap);
+/* addition: the debris flux field of LES slot `slot` (0 or 1) as uploaded:
+ * inverse CDF (up to cap entries; icdf may be NULL to query) and the shared
+ * descriptor (desc16 may be NULL); returns the entry count, 0 when none. */
+int RS_get_debris_flux_field(RSHandle *H, int slot, float *icdf, int cap, float *desc16);
 
 #ifdef __cplusplus
 }

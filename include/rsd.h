@@ -15,6 +15,7 @@
  *   rsd_set_rcs                     RS_set_rcs_data (rs.c:3852-3972)
  *   rsd_set_drop_rcs                RS_set_rcs_ellipsoid_table (rs.c:2472-2545)
  *   rsd_set_dff                     RS_set_debris_flux_field_by_icdf (rs.c:3331-3404)
+ *   rsd_set_dff_slot                its per-buffer write, dff_icdf[les_id] (rs.c:3364-3366)
  *   rsd_set_range_weight            RS_set_range_weight (rs.c:2548-2618)
  *   rsd_set_angular_weight          RS_set_angular_weight (rs.c:2626-2698)
  *   rsd_advance                     enqueue block of RS_advance_time (rs.c:5327-5388)
@@ -117,8 +118,16 @@ int rsd_set_angular_weight(rsd_engine *e, const float *w, int n, const float *de
 /* Debris flux field (concept C): inverse CDF of `count` entries and its
  * descriptor (RS_set_debris_flux_field_by_icdf, rs.c:3331-3404, read by
  * db_atts at rs.cl:795-836); count 0 removes it.  Used on debris respawn
- * while the step's concept has bit 7 set. */
+ * while the step's concept has bit 7 set.  The engine keeps one table per
+ * wind slot and a step reads its wind_slot's (the reference's dff_icdf[les_id],
+ * rs.c:3351-3373, 5358); rsd_set_dff sets both. */
 int rsd_set_dff(rsd_engine *e, const float *icdf, int count, const float *desc16);
+/* The table of one wind slot (0 or 1), e.g. the field of the LES frame just
+ * loaded into that slot: written in stream order through pinned staging, so
+ * steps already enqueued read the previous table.  The first table, a removal
+ * (count 0) or a change of size sets both slots, as rsd_set_dff.  Replaces the
+ * per-buffer write of RS_set_debris_flux_field_by_icdf (rs.c:3364-3366). */
+int rsd_set_dff_slot(rsd_engine *e, int slot, const float *icdf, int count, const float *desc16);
 
 /* One PRT of every scatterer type.  May be held back and fused with the next
  * rsd_make_pulse; any transfer or rsd_sync performs it first. */

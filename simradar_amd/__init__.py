@@ -78,11 +78,11 @@ EXPORTED_RS = [
     "RS_init_host_only", "RS_set_population", "RS_set_phase_precision", "RS_set_les_time_interpolation", "RS_get_pulse_f64",
     "RS_set_moment_pulses", "RS_get_moments", "RS_set_range_cap", "RS_batch_round",
     "RS_get_descriptor", "RS_radar_gain", "RS_engine", "RS_get_info", "RS_get_host_stream", "RS_get_params",
-    "RS_get_dsd", "RS_suggest_scan_domain_values", "RS_set_scan_box_values", "RS_get_les_frame", "RS_get_les_cpxx", "RS_get_debris_table",
+    "RS_get_dsd", "RS_suggest_scan_domain_values", "RS_set_scan_box_values", "RS_get_les_frame", "RS_get_les_cpxx", "RS_get_debris_table", "RS_get_debris_flux_field",
 ]
 EXPORTED_RSD = [
     "rsd_device_count", "rsd_device_props", "rsd_open", "rsd_close", "rsd_last_error", "rsd_alloc_state",
-    "rsd_upload_state", "rsd_download_state", "rsd_set_wind", "rsd_set_cpxx", "rsd_set_adm", "rsd_set_rcs", "rsd_set_drop_rcs", "rsd_set_dff",
+    "rsd_upload_state", "rsd_download_state", "rsd_set_wind", "rsd_set_cpxx", "rsd_set_adm", "rsd_set_rcs", "rsd_set_drop_rcs", "rsd_set_dff", "rsd_set_dff_slot",
     "rsd_set_range_weight", "rsd_set_angular_weight", "rsd_advance", "rsd_make_pulse", "rsd_make_pulse_noalloc", "rsd_read_pulse",
     "rsd_sync", "rsd_timer_start", "rsd_timer_stop", "rsd_set_launch", "rsd_math_probe", "rsd_profile",
     "rsd_profile_read", "rsd_io_test", "rsd_set_sort", "rsd_get_sort", "rsd_set_fold", "rsd_set_iq_ring", "rsd_moments",
@@ -227,6 +227,7 @@ def _declare(lib):
         "RS_get_host_stream": (P, [P, I]),
         "RS_get_params": (None, [P, ctypes.POINTER(RSParams)]),
         "RS_get_dsd": (I, [P, P, P, P, I]),
+        "RS_get_debris_flux_field": (I, [P, I, P, I, P]),
         "RS_suggest_scan_domain_values": (None, [P, P]),
         "RS_get_les_frame": (P, [P, I, P]),
         "RS_get_les_cpxx": (P, [P, I, P]),
@@ -246,6 +247,7 @@ def _declare(lib):
         "rsd_set_rcs": (I, [P, I, P, P, I, I, P]),
         "rsd_set_drop_rcs": (I, [P, P, I, P]),
         "rsd_set_dff": (I, [P, P, I, P]),
+        "rsd_set_dff_slot": (I, [P, I, P, I, P]),
         "rsd_set_range_weight": (I, [P, P, I, P]),
         "rsd_set_angular_weight": (I, [P, P, I, P]),
         "rsd_advance": (I, [P, ctypes.POINTER(RSDStep)]),
@@ -456,6 +458,16 @@ class Radar:
         nx, ny, nz = (int(v) for v in dims)
         buf = (ctypes.c_char * (nx * ny * nz * 16)).from_address(p)
         return np.frombuffer(buf, dtype=np.float32).reshape(nz, ny, nx, 4).copy()
+
+    def debris_flux_field(self, slot):
+        """(inverse CDF, descriptor16) of the debris flux field of an LES slot, or None."""
+        n = self.lib.RS_get_debris_flux_field(self.H, slot, None, 0, None)
+        if n <= 0:
+            return None
+        icdf = np.zeros(n, np.float32)
+        desc = np.zeros(16, np.float32)
+        self.lib.RS_get_debris_flux_field(self.H, slot, _ptr(icdf), n, _ptr(desc))
+        return icdf, desc
 
     def debris_table(self, t, which):
         """(ny, nx, 4) float32 texels of debris object t (0 cd, 1 cm, 2 rcs real, 3 rcs imag)."""

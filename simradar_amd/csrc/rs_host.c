@@ -182,6 +182,8 @@ void RS_free(RSHandle *H) {
     if (H->P && H->owns_pattern) POS_free((POSPattern *)H->P);
     for (int i = 0; i < RS_MAX_GPU_DEVICE; i++) {
         if (H->workers[i].engine) rsd_close(H->workers[i].engine);
+        free(H->workers[i].dff_icdf[0]);
+        free(H->workers[i].dff_icdf[1]);
     }
     free_scat_memory(H);
     free(H->anchor_pos);
@@ -805,9 +807,8 @@ void RS_set_vel_data_to_LES_table(RSHandle *H, const LESTable *leslie) {
     t.tr = leslie->tr;
     t.uvwt = (cl_float4 *)leslie->uvwt;
     t.cpxx = (cl_float4 *)leslie->cpxx;
-    if (H->sim_concept & RSSimulationConceptDebrisFluxFromVelocity) {
-        rsprint("WARNING: Debris flux field (concept C) is not part of this engine; uniform respawn is used.");
-    }
+    /* rs.c:3106-3111: concept C rebuilds the debris flux field from every frame */
+    if (H->sim_concept & RSSimulationConceptDebrisFluxFromVelocity) RS_set_debris_flux_field_from_LES(H, leslie);
     RS_set_vel_data(H, t);
     upload_cpxx(H, leslie);
     H->vel_desc = *leslie;
@@ -999,7 +1000,32 @@ void RS_set_debris_flux_field_by_icdf(RSHandle *H, RSTable2D *map, const float *
     d.s[14] = (float)map->y_;
     d.s[15] = (float)map->x_ - 1.0f;
     C->dff_desc = d;
-    if (has_engine(H)) engine_ok(H, rsd_set_dff(C->engine, icdf, (int)map->x_, d.s), "debris flux field upload");
+    /* the slot of the LES frame being set (dff_icdf[les_id], rs.c:3364-3366);
+     * the first field, or one of another size, fills both (rs.c:3342-3348) */
+    const int n = (int)map->x_;
+    const int slot = (int)(C->les_id & 1u);
+    const int both = !C->dff_icdf[0] || !C->dff_icdf[1] || C->dff_count[0] != n || C->dff_count[1] != n;
+    for (int k = 0; k < 2; k++) {
+        if (!both && k != slot) continue;
+        float *h = (float *)realloc(C->dff_icdf[k], (size_t)(n > 0 ? n : 1) * sizeof(float));
+        if (!h) {
+            rsprint("ERROR: Unable to keep the debris flux field.");
+            return;
+        }
+        memcpy(h, icdf, (size_t)n * sizeof(float));
+        C->dff_icdf[k] = h;
+        C->dff_count[k] = n;
+    }
+    if (has_engine(H)) engine_ok(H, rsd_set_dff_slot(C->engine, slot, icdf, n, d.s), "debris flux field upload");
+}
+
+int RS_get_debris_flux_field(RSHandle *H, int slot, float *icdf, int cap, float *desc16) {
+    if (!H || slot < 0 || slot > 1) return 0;
+    const RSWorker *C = &H->workers[0];
+    const int n = C->dff_icdf[slot] ? C->dff_count[slot] : 0;
+    if (icdf && n > 0) memcpy(icdf, C->dff_icdf[slot], (size_t)(n < cap ? n : cap) * sizeof(float));
+    if (desc16) memcpy(desc16, C->dff_desc.s, 16 * sizeof(float));
+    return n;
 }
 
 /* rs.c:3431-3459: all flux in the middle cell (4) of a 3 x 3 field. */
@@ -1092,17 +1118,23 @@ void RS_set_debris_flux_field_to_checker_board_stretched(RSHandle *H, const LEST
 
 /* rs.c:3565-3626: flux of the second LES level, |u|^2 per cell times the cell
  * area on a stretched grid, normalised to a PDF (also left in l->flux when the
- * table has that buffer). */
+ * table has that buffer).  The velocity comes from the separate u, v, w arrays
+ * or, for frames that only hold the interleaved texels, from uvwt, which holds
+ * the same values (les.c:395-398). */
 void RS_set_debris_flux_field_from_LES(RSHandle *H, const LESTable *l) {
     const int count = (int)(l->nx * l->ny);
     const float mx = 0.5f * (float)(l->nx - 1), my = 0.5f * (float)(l->ny - 1);
-    if (count < 1 || !l->data.u || !l->data.v || !l->data.w) return;
+    const int split = l->data.u && l->data.v && l->data.w;
+    if (count < 1 || l->nz < 2 || (!split && !l->uvwt)) return;
     float *flux = l->flux ? l->flux : (float *)malloc((size_t)count * sizeof(float));
     if (!flux) return;
     float sum = 0.0f;
     for (int k = 0; k < count; k++) {
         const int i = count + k;
-        float v = l->data.u[i] * l->data.u[i] + l->data.v[i] * l->data.v[i] + l->data.w[i] * l->data.w[i];
+        const float u = split ? l->data.u[i] : l->uvwt[i][0];
+        const float vv = split ? l->data.v[i] : l->uvwt[i][1];
+        const float w = split ? l->data.w[i] : l->uvwt[i][2];
+        float v = u * u + vv * vv + w * w;
         if (l->is_stretched) {
             const int iy = k / (int)l->nx, ix = k % (int)l->nx;
             const float dx = l->ax * powf(l->rx, fabsf((float)ix - mx));

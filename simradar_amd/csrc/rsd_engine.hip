@@ -1377,8 +1377,9 @@ __global__ __launch_bounds__(256) void k_finalize_batch(const FinK *f, const int
 }
 
 // ---------------------------------------------------------------------------
-// Debris (types >= 1): db_atts (rs.cl:744-907, uniform respawn placement) and
-// db_rcs (rs.cl:909-919), fused with the pulse.
+// Debris (types >= 1): db_atts (rs.cl:744-907; respawns placed uniformly in
+// the box, or through the flux field of the step's wind slot under concept C)
+// and db_rcs (rs.cl:909-919), fused with the pulse.
 // ---------------------------------------------------------------------------
 struct DebrisK {
     float4 *pos, *ori, *vel, *tum, *rcs;
@@ -2232,7 +2233,14 @@ struct rsd_engine {
     F16 rcs_desc[RSD_MAX_TABLES];
     float4 *drop;
     F4 drop_desc;
-    float *dff;             // debris flux field: inverse CDF and descriptor
+    // Debris flux field: one inverse CDF per wind slot (the step's wind_slot
+    // selects it, as the reference pairs dff_icdf[les_id] with the LES frame,
+    // rs.c:3351-3373, 5358), one descriptor; a slot's later uploads go through
+    // pinned staging in stream order.
+    float *dff[2];
+    int dff_n[2];
+    float *dff_stage[2];
+    hipEvent_t ev_dff[2];
     F16 dff_desc;
     float *rangew;
     float rangew_h[1024];
@@ -2456,12 +2464,14 @@ extern "C" void rsd_close(rsd_engine *e) {
         dfree(e->cpxx[s]);
         if (s == 0) dfree(e->raw);
         if (e->wind_stage[s]) (void)hipHostFree(e->wind_stage[s]);
+        if (e->dff_stage[s]) (void)hipHostFree(e->dff_stage[s]);
+        if (e->ev_dff[s]) (void)hipEventDestroy(e->ev_dff[s]);
         if (e->ev_wind[s]) (void)hipEventDestroy(e->ev_wind[s]);
     }
     for (int t = 0; t < RSD_MAX_TABLES; t++) {
         dfree(e->adm_cd[t]); dfree(e->adm_cm[t]); dfree(e->rcs_re[t]); dfree(e->rcs_im[t]);
     }
-    dfree(e->drop); dfree(e->dff); dfree(e->rangew); dfree(e->angw); dfree(e->gate_r); dfree(e->partials); dfree(e->partials2);
+    dfree(e->drop); dfree(e->dff[0]); dfree(e->dff[1]); dfree(e->rangew); dfree(e->angw); dfree(e->gate_r); dfree(e->partials); dfree(e->partials2);
     dfree(e->logtab);
     dfree(e->ring);
     dfree(e->mom_d);
@@ -2815,7 +2825,7 @@ static int fill_debris(rsd_engine *e, const rsd_step *s, const rsd_pulse *p, int
     a.begin = e->origins[type];
     a.end = e->origins[type] + e->counts[type];
     a.logtab = e->logtab;
-    a.dff = e->dff;
+    a.dff = e->dff[s ? s->wind_slot : 0];
     a.dff_desc = e->dff_desc;
     if (s) {
         int slot = s->wind_slot, ai = s->adm_index[type], ri = s->rcs_index[type];
@@ -3589,22 +3599,68 @@ extern "C" int rsd_set_drop_rcs(rsd_engine *e, const float *table, int n, const 
     return RSD_OK;
 }
 
-extern "C" int rsd_set_dff(rsd_engine *e, const float *icdf, int count, const float *desc16) {
-    if (!e || count < 0 || (count > 0 && (!icdf || !desc16))) return fail(e, RSD_ERR_ARG, "bad debris flux field");
+static int check_dff(rsd_engine *e, int slot, const float *icdf, int count, const float *desc16) {
+    if (!e || slot < -1 || slot > 1 || count < 0 || (count > 0 && (!icdf || !desc16)))
+        return fail(e, RSD_ERR_ARG, "bad debris flux field");
     if (count > 0 && !(desc16[15] >= 0.0f && desc16[15] <= (float)(count - 1)))
         return fail(e, RSD_ERR_ARG, "flux field max index %g beyond %d entries", desc16[15], count);
     if (count > 0 && !(desc16[13] > 0.0f)) return fail(e, RSD_ERR_ARG, "flux field row length %g", desc16[13]);
-    HIPCHK(e, hipSetDevice(e->device));
-    int st = flush_pending(e);
-    if (st) return st;
+    return RSD_OK;
+}
+
+// Both slots (re)allocated and filled with one table, after the queue drains.
+static int dff_both(rsd_engine *e, const float *icdf, int count, const float *desc16) {
     HIPCHK(e, hipStreamSynchronize(e->stream));
-    dfree(e->dff);
-    e->dff = NULL;
+    for (int k = 0; k < 2; k++) {
+        if (e->ev_dff[k]) HIPCHK(e, hipEventSynchronize(e->ev_dff[k]));
+        dfree(e->dff[k]);
+        e->dff[k] = NULL;
+        e->dff_n[k] = 0;
+        if (e->dff_stage[k]) HIPCHK(e, hipHostFree(e->dff_stage[k]));      // sized for the old count
+        e->dff_stage[k] = NULL;
+    }
     memset(&e->dff_desc, 0, sizeof(e->dff_desc));
     if (count == 0) return RSD_OK;
-    HIPCHK(e, hipMalloc((void **)&e->dff, (size_t)count * sizeof(float)));
-    HIPCHK(e, hipMemcpyAsync(e->dff, icdf, (size_t)count * sizeof(float), hipMemcpyHostToDevice, e->stream));
+    for (int k = 0; k < 2; k++) {
+        HIPCHK(e, hipMalloc((void **)&e->dff[k], (size_t)count * sizeof(float)));
+        HIPCHK(e, hipMemcpyAsync(e->dff[k], icdf, (size_t)count * sizeof(float), hipMemcpyHostToDevice, e->stream));
+        e->dff_n[k] = count;
+    }
     HIPCHK(e, hipStreamSynchronize(e->stream));
+    memcpy(e->dff_desc.s, desc16, sizeof(e->dff_desc.s));
+    return RSD_OK;
+}
+
+extern "C" int rsd_set_dff(rsd_engine *e, const float *icdf, int count, const float *desc16) {
+    int st = check_dff(e, -1, icdf, count, desc16);
+    if (st) return st;
+    HIPCHK(e, hipSetDevice(e->device));
+    st = flush_pending(e);
+    if (st) return st;
+    join_batch(e);
+    return dff_both(e, icdf, count, desc16);
+}
+
+extern "C" int rsd_set_dff_slot(rsd_engine *e, int slot, const float *icdf, int count, const float *desc16) {
+    int st = check_dff(e, slot, icdf, count, desc16);
+    if (st) return st;
+    if (slot < 0) return fail(e, RSD_ERR_ARG, "wind slot %d", slot);
+    HIPCHK(e, hipSetDevice(e->device));
+    st = flush_pending(e);
+    if (st) return st;
+    join_batch(e);
+    // first field, a removal or a new size: both slots at once, as the first
+    // upload of the reference fills both buffers
+    if (count == 0 || !e->dff[0] || !e->dff[1] || e->dff_n[0] != count || e->dff_n[1] != count)
+        return dff_both(e, icdf, count, desc16);
+    // a new field for one slot (the LES frame loaded into it): stream-ordered,
+    // so launches already enqueued read the old table and the queue keeps running
+    if (!e->ev_dff[slot]) HIPCHK(e, hipEventCreateWithFlags(&e->ev_dff[slot], hipEventDisableTiming));
+    else HIPCHK(e, hipEventSynchronize(e->ev_dff[slot]));     // the staging buffer's last copy is done
+    if (!e->dff_stage[slot]) HIPCHK(e, hipHostMalloc((void **)&e->dff_stage[slot], (size_t)count * sizeof(float), hipHostMallocDefault));
+    memcpy(e->dff_stage[slot], icdf, (size_t)count * sizeof(float));
+    HIPCHK(e, hipMemcpyAsync(e->dff[slot], e->dff_stage[slot], (size_t)count * sizeof(float), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(e, hipEventRecord(e->ev_dff[slot], e->stream));
     memcpy(e->dff_desc.s, desc16, sizeof(e->dff_desc.s));
     return RSD_OK;
 }

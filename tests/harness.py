@@ -32,13 +32,15 @@ def bits(a):
 
 
 def run_pair(cfg, pulses, sort=None, fold=None, launch=None, pre=None, setup=None, oracle_kw=None,
-             pulse_mode=None, les_switch=False, on_pulse=None):
+             pulse_mode=None, les_switch=False, on_pulse=None, on_switch=None):
     """Populate an engine run of cfg and step it `pulses` times next to the
     oracle from the same populated state (the driver order of
     simradar.c:1011-1074).  pre(r) runs before RS_populate, setup(r, o) after
     the oracle is made; oracle_kw go to OracleRun; pulse_mode selects the
     oracle's phase precision; les_switch: the oracle follows the host's LES
-    volume switches; on_pulse(k, r, o) runs after pulse k is compared.
+    volume switches; on_pulse(k, r, o) runs after pulse k is compared;
+    on_switch(r, o) after the host loads a new LES volume (before the oracle's
+    advance).
     Returns (radar, oracle, engine pulses, oracle pulses)."""
     r = sa.Radar(verb=0)
     configure(r, cfg)
@@ -79,7 +81,7 @@ def run_pair(cfg, pulses, sort=None, fold=None, launch=None, pre=None, setup=Non
         orc.set_blend(wt, ct, float(inf.wind_alpha))
     pg, po = [], []
     try:
-        _steps(r, o, pulses, mode, pg, po, on_pulse, les_switch, follow_blend if blend else None)
+        _steps(r, o, pulses, mode, pg, po, on_pulse, les_switch, follow_blend if blend else None, on_switch)
     finally:
         orc.set_blend(None)
     r.download()
@@ -87,7 +89,7 @@ def run_pair(cfg, pulses, sort=None, fold=None, launch=None, pre=None, setup=Non
     return r, o, np.array(pg), np.array(po)
 
 
-def _steps(r, o, pulses, mode, pg, po, on_pulse, les_switch, follow_blend):
+def _steps(r, o, pulses, mode, pg, po, on_pulse, les_switch, follow_blend, on_switch=None):
     for k in range(pulses):
         if k > 0:
             r.make_pulse()
@@ -105,6 +107,8 @@ def _steps(r, o, pulses, mode, pg, po, on_pulse, les_switch, follow_blend):
             follow_blend()
         elif les_switch and int(r.info().vel_idx) != vi:
             o.set_les_frame(r.les_frame(vi), r.les_cpxx(vi) if o.fixed else None)
+            if on_switch is not None:
+                on_switch(r, o)
         o.advance(r.sim_desc())
         r.advance_beam()
 

@@ -99,3 +99,44 @@ def test_product_table_equals_oracle(kind):
     a, b = _product_icdf(pdf), orc.dff_icdf_from_pdf(pdf)
     assert a is not None and b is not None
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_concept_c_builds_the_field_from_the_second_les_level():
+    """With concept C, loading an LES volume (RS_set_vel_data_to_config here)
+    builds the flux field from the volume (rs.c:3106-3111): |u|^2 of its second
+    level (cell nx ny + k) times the stretched cell area a r^|i - m| per axis,
+    normalised (rs.c:3565-3626).  The first field fills both LES slots.  The
+    table inverts that PDF's CDF (binary64 here) within the table's resolution,
+    and not the first level's."""
+    import harness as h
+    r = sa.Radar(verb=0, host_only=True)
+    h.configure(r, h.Config(n_rain=4096, scan="P:3.0,-1:1:0.01", concept=h.Config().concept | sa.CONCEPT_DFF))
+    f0, f1 = r.debris_flux_field(0), r.debris_flux_field(1)
+    assert f0 is not None and f1 is not None
+    assert np.array_equal(f0[0], f1[0]) and np.array_equal(f0[1], f1[1])
+    icdf, desc = f0
+    assert desc[14] == 1.0          # stretched grid: scale a / (1 - r) in slot 0, ratio r in slot 4
+    frame = r.les_frame(0).astype(np.float64)      # (nz, ny, nx, 4)
+    nz, ny, nx, _ = frame.shape
+    assert icdf.size == N and desc[13] == nx and desc[8] == nx - 1 and desc[9] == ny - 1
+    rx, ry = float(desc[4]), float(desc[5])
+    ax, ay = float(desc[0]) * (1.0 - rx), float(desc[1]) * (1.0 - ry)
+    ix, iy = np.meshgrid(np.arange(nx), np.arange(ny))
+    area = (ax * rx ** np.abs(ix - 0.5 * (nx - 1))) * (ay * ry ** np.abs(iy - 0.5 * (ny - 1)))
+
+    def pdf(level):
+        u = frame[level, :, :, :3]
+        w = (u * u).sum(-1) * area
+        return (w / w.sum()).ravel()
+
+    def quantile_error(p):
+        # entry j of the table inverts the CDF (linear inside each cell) at ~ j / N
+        c = np.concatenate([[0.0], np.cumsum(p)])
+        cell = np.minimum(np.floor(icdf).astype(np.int64), p.size - 1)
+        at = c[cell] + (icdf - cell) * p[cell]
+        return np.max(np.abs(at - np.arange(N) / float(N)))
+
+    e1, e0 = quantile_error(pdf(1)), quantile_error(pdf(0))
+    assert e1 <= 1.5 / N, e1        # one entry: the table's sampling of the CDF
+    assert e0 > 3.0 / N, e0         # the first level's field is another table
+    r.free()

@@ -235,6 +235,47 @@ def test_debris_flux_field_respawn(gpu_ready, mode):
         orc.set_dff(None, None)
 
 
+def test_concept_c_flux_field_follows_the_les_volumes(gpu_ready):
+    """Concept C with the LES velocity (rs.c:3106-3111, 3565-3626): every LES
+    volume the host loads rebuilds the debris flux field from the volume's
+    second level and uploads it to that volume's slot, and respawned debris are
+    placed through the active slot's field (dff_icdf[les_id], rs.c:5358).  Over
+    a volume switch (every 200 pulses at a PRT of 10 ms), with debris lifted
+    above the box on alternate pulses around it so that they respawn under
+    both fields, the state stays bit-exact with the oracle taking each field
+    from the host."""
+    cfg = h.Config(n_rain=4096, debris=[(sa.OBJ_LEAF, 1024)], scan="P:3.0,-1:1:0.01", prt=0.01,
+                   concept=h.Config().concept | sa.CONCEPT_DFF)
+    fields = []
+
+    def take_field(r):
+        f = r.debris_flux_field(int(r.info().les_id))
+        assert f is not None, "concept C left no flux field"
+        orc.set_dff(*f)
+        fields.append(f[0].copy())
+
+    def setup(r, o):
+        take_field(r)
+
+    def on_pulse(k, r, o):
+        if 190 <= k <= 210 and k % 2 == 0:
+            _lift_debris(r, o)
+
+    def on_switch(r, o):
+        take_field(r)
+
+    try:
+        r, o, pg, po = h.run_pair(cfg, 215, setup=setup, on_pulse=on_pulse, les_switch=True, on_switch=on_switch)
+        _check_state(r, o)
+        assert len(fields) == 2, "expected one volume switch"
+        assert not np.array_equal(fields[0], fields[1]), "the new volume's field equals the old one"
+        errs = [h.rel_err(pg[k], po[k]) for k in range(len(pg))]
+        assert max(errs) < IQ_TOL, errs
+        r.free()
+    finally:
+        orc.set_dff(None, None)
+
+
 def test_rain_sorted_every_pulse(gpu_ready):
     """Spatial reorder before every pulse: streams still come back bit-exact in
     the reference order and the gate sums are unchanged."""
