@@ -1311,8 +1311,12 @@ __device__ __forceinline__ void rain_range(const RainK &a, size_t lo, size_t hi,
 // chunk): chunk c of domain d covers scatterers [c chunk, (c + 1) chunk) and owns
 // partial row c of that domain.  Block b takes the items of logical block
 // L = (b mod 8)(G / 8) + b / 8, L + G, L + 2G, ...: each XCD walks contiguous
-// items, so one domain's wind table stays in one L2.
-template <bool ADV, bool PULSE>
+// items, so one domain's wind table stays in one L2 (equal items walked in
+// stripes keep each XCD on one or two domains at a time; equal shares of one
+// flat range over all domains measured slower: each XCD then holds the tables
+// of several domains at once).  FAST: every domain is a launch the specialised
+// rain loop covers (rain_fast).
+template <bool ADV, bool PULSE, bool FAST = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_rain_batch(
     const RainK *doms, const int2 *items, int n_items, int chunk, const float *logtab_g) {
     extern __shared__ double acc[];
@@ -1329,8 +1333,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         if (PULSE) stage_rain_tables(acc, a);
         if (use_win) stage_window(acc, a.pk);
         const size_t lo = (size_t)it.y * (size_t)chunk;
-        const size_t hi = lo + (size_t)chunk < a.n ? lo + (size_t)chunk : a.n;
-        rain_range<ADV, PULSE, false>(a, lo, hi, acc, logtab, use_win);
+        const size_t hi = lo + (size_t)chunk < a.n ? lo + (size_t)chunk : a.n;    // a.n = 0: the domain sits this round out
+        rain_range<ADV, PULSE, false, false, FAST>(a, lo, hi > lo ? hi : lo, acc, logtab, use_win);
         if (PULSE) acc_flush_row(acc, a.pk, it.y);
         __syncthreads();
     }
@@ -1362,6 +1366,7 @@ __device__ __forceinline__ double column_sum(const double *col, int rows) {
 // Column j of a pulse's partial sums into its output slot(s): (Ih, Qh, Iv, Qv)
 // of gate j % R, component j / R.
 __device__ __forceinline__ void finalize_column(const FinK &a, int j) {
+    if (!a.out) return;     // a batched domain launched on its own this round
     const int R = a.n4r / 4;
     const double s = column_sum(a.partials + (size_t)j * a.rows, a.rows);
     if (threadIdx.x == 0) {
@@ -2207,6 +2212,7 @@ struct rsd_engine {
     int have_pending_pulse;
     rsd_pulse pending_pulse;
     int batch_wait;         // the engine stream must wait for the batch's last launch
+    int batch_rejoin;       // launched on its own stream in the last round: the batch waits for it
     void *sort_tmp;
     size_t sort_tmp_bytes;
     int sort_interval;      // pulses between sorts (0 = never)
@@ -2886,7 +2892,11 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p, con
     // blocks and leaves its own to the next one; every other launch finalizes
     // what is pending first and its own pulse at once.
     const bool fixed_bg = s ? s->background_kind == RSD_ADVANCE_FIXED : e->bg_fixed != 0;
-    const bool defer = p && !sp && e->n_rain && !fixed_bg;
+    // An engine inside a batch (a launch of its own beside the batched ones)
+    // finalizes at once, from the second partial buffer: the batch kernel may
+    // write the first meanwhile and finalizes the engine's batched pulses.
+    const bool defer = p && !sp && e->n_rain && !fixed_bg && !e->batch;
+    const int part_buf = defer ? e->part_cur : (e->batch ? 1 : 0);
     if (!defer) {
         int st = flush_finalize(e);
         if (st) return st;
@@ -2944,11 +2954,11 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p, con
     PulseK pk;
     memset(&pk, 0, sizeof(pk));
     if (p) {
-        int st = sp ? RSD_OK : ensure_partials(e, (size_t)rows, R, defer ? e->part_cur : 0);
+        int st = sp ? RSD_OK : ensure_partials(e, (size_t)rows, R, part_buf);
         if (st) return st;
         st = fill_pulse(e, p, pk, sp);
         if (st) return st;
-        if (defer && e->part_cur) pk.partials = e->partials2;
+        if (!sp && part_buf) pk.partials = e->partials2;
         pk.rows = rows;
         if (sp) {
             hipLaunchKernelGGL(k_gate_ranges, dim3(1), dim3(64), 0, e->stream, p->range_start, p->range_delta, R,
@@ -3113,7 +3123,7 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p, con
             e->fin_on = 1;
             e->part_cur ^= 1;
         } else {
-            hipLaunchKernelGGL(k_finalize, dim3(4 * R), dim3(256), 0, e->stream, (const double *)e->partials, rows,
+            hipLaunchKernelGGL(k_finalize, dim3(4 * R), dim3(256), 0, e->stream, (const double *)pk.partials, rows,
                                4 * R, e->pulse_d, slot);
             HIPCHK(e, hipGetLastError());
         }
@@ -4104,19 +4114,35 @@ extern "C" int rsd_batch_launch(rsd_batch *b) {
     }
     (void)mixed;
     if (ns == 0) return RSD_OK;
-    // Range reorder of each domain that is due, on its own stream; only those
-    // streams have new work the batch must wait for (engine transfers are
-    // synchronous and fallback launches are not in the batch).
+    // A domain due for its range reorder sits the batched launch out: the sort
+    // and the domain's own fused launch run on its stream beside the batch
+    // kernel, whose work items for it find no scatterers and whose finalize
+    // leaves its pulse alone (the item list, and so the graph, stay as they
+    // are); the next round's batch waits for that stream.  Only those streams
+    // have new work the batch must wait for (engine transfers are synchronous
+    // and fallback launches are not in the batch).
+    unsigned char sat_out[1024];
     for (int k = 0; k < ns; k++) {
         rsd_engine *e = sel[k];
+        sat_out[k] = 0;
+        if (e->batch_rejoin) {
+            if (hipEventRecord(e->ev_fork, e->stream) != hipSuccess ||
+                hipStreamWaitEvent(b->stream, e->ev_fork, 0) != hipSuccess)
+                return bfail(b, RSD_ERR_HIP, "stream join");
+            e->batch_rejoin = 0;
+        }
         if (e->sort_interval > 0 && e->n_rain > 1 && (!e->permuted || e->since_sort >= e->sort_interval)) {
             join_batch(e);
             int st = sort_rain(e, e->have_pending ? e->pending.sim_desc : e->pending_pulse.sim_desc,
                                e->pending_pulse.range_delta);
             if (st) return bfail(b, st, e->err);
-            if (hipEventRecord(e->ev_fork, e->stream) != hipSuccess ||
-                hipStreamWaitEvent(b->stream, e->ev_fork, 0) != hipSuccess)
-                return bfail(b, RSD_ERR_HIP, "stream join");
+            sat_out[k] = 1;
+            e->have_pending_pulse = 0;
+            const bool adv = e->have_pending != 0;
+            e->have_pending = 0;
+            st = launch_step(e, adv ? &e->pending : NULL, &e->pending_pulse);
+            if (st) return bfail(b, st, e->err);
+            e->batch_rejoin = 1;
         }
         e->since_sort++;
     }
@@ -4180,11 +4206,13 @@ extern "C" int rsd_batch_launch(rsd_batch *b) {
     int *first = (int *)(b->desc_h[buf] + rk_bytes + fk_bytes);
     size_t lds = 0;
     int total4r = 0;
+    bool fast = any_adv == 1;
     for (int k = 0; k < ns; k++) {
         rsd_engine *e = sel[k];
         const rsd_pulse *p = &e->pending_pulse;
         const int R = (int)p->range_count;
         const int rows = (int)((e->n_rain + chunk - 1) / chunk);
+        if (fast && !(rain_fast(e, &e->pending, p) && !e->plo)) fast = false;
         int st = ensure_partials(e, (size_t)rows, R);
         if (st) return bfail(b, st, e->err);
         RainK a;
@@ -4197,14 +4225,15 @@ extern "C" int rsd_batch_launch(rsd_batch *b) {
         a.pk.row0 = 0;
         a.pk.rows = rows;
         a.plo = NULL;
+        if (sat_out[k]) a.n = 0;
         rk[k] = a;
         FinK f;
         f.partials = e->partials;
         f.rows = rows;
         f.n4r = 4 * R;
-        f.out = e->pulse_d;
+        f.out = sat_out[k] ? NULL : e->pulse_d;
         f.out2 = NULL;
-        if (e->ring && e->ring_R == R) {
+        if (!sat_out[k] && e->ring && e->ring_R == R) {
             f.out2 = e->ring + (size_t)e->ring_next * 4 * R;
             e->ring_next = (e->ring_next + 1) % e->ring_m;
             if (e->ring_count < e->ring_m) e->ring_count++;
@@ -4214,6 +4243,7 @@ extern "C" int rsd_batch_launch(rsd_batch *b) {
         total4r += 4 * R;
         size_t l = (size_t)4 * R * sizeof(double) + (a.pk.fold == RSD_FOLD_WINDOW ? (size_t)e->rangew_n * sizeof(float2) + (size_t)R * sizeof(float) : 0);
         l = set_rain_tables(e, rk[k], R, l);
+        if (!rk[k].tab_lds) fast = false;
         if (l > lds) lds = l;
     }
     first[ns] = total4r;
@@ -4224,12 +4254,15 @@ extern "C" int rsd_batch_launch(rsd_batch *b) {
     const bool timed = b->tuning;
     // everything the graph depends on besides descriptor contents
     const size_t gsig = (((((size_t)n_items * 131u + (size_t)grid) * 131u + lds) * 131u + (size_t)chunk) * 131u +
-                         (size_t)total4r) * 4u + (size_t)any_adv * 2u + 1u + (desc_bytes << 20) +
+                         (size_t)total4r) * 8u + (size_t)fast * 4u + (size_t)any_adv * 2u + 1u + (desc_bytes << 20) +
                         (size_t)(uintptr_t)logtab * 7u;
     auto enqueue = [&](hipStream_t st) -> hipError_t {
         hipError_t r = hipMemcpyAsync(b->desc_d, b->desc_h[buf], desc_bytes, hipMemcpyHostToDevice, st);
         if (r != hipSuccess) return r;
-        if (any_adv)
+        if (fast)
+            hipLaunchKernelGGL((k_rain_batch<true, true, true>), dim3(grid), dim3(256), lds, st, rk_d,
+                               (const int2 *)b->items_d, n_items, chunk, logtab);
+        else if (any_adv)
             hipLaunchKernelGGL((k_rain_batch<true, true>), dim3(grid), dim3(256), lds, st, rk_d,
                                (const int2 *)b->items_d, n_items, chunk, logtab);
         else
