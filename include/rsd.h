@@ -106,6 +106,18 @@ int rsd_download_state(rsd_engine *e, float *pos, float *vel, float *ori, float 
 
 /* Tables.  Texels are float4, x fastest.  Descriptors use the reference slots. */
 int rsd_set_wind(rsd_engine *e, int slot, const float *uvwt, int nx, int ny, int nz, const float *les_desc16);
+/* Index tables of a stretched LES descriptor (slot 7 = 7) for the box of
+ * sim_desc16 (slots 8-10 origin, 12-14 size), as every advance evaluates the
+ * index c = s8 + copysign(s0 log1p(s4 |r|), r) of rs.cl:130-151: per axis
+ * (x, y, z) 256 cubic pieces of P(x) = s0 log1p(s4 x) over x in [0, X], X the
+ * box's extent from the axis origin (box centre in x and y, 0 in z) times
+ * 1 + 2^-10; rows[4 (256 a + j) + 0..3] = binary32 monomial coefficients of
+ * piece j in f, x = (j + f) / ih[a], interpolating binary64 P at the Chebyshev
+ * nodes 0.5 + 0.5 cos((2k + 1) pi / 8); jm[a] = 255.  The engine builds them
+ * with this function on the host (no device needed) whenever a wind slot's
+ * descriptor or the box changes.  No counterpart in the reference (which
+ * evaluates log1p per scatterer). */
+int rsd_wind_index_table(const float *les_desc16, const float *sim_desc16, float *rows, float *ih, float *jm);
 /* The (cn2, p, _, _) volume of the LES slot (the second image RS_set_vel_data
  * uploads, rs.c:2925), read with the slot's wind descriptor by the concept-F
  * advance; same shape as the slot's wind table.  Waits for the device. */
@@ -306,12 +318,11 @@ int rsd_debris_samples(rsd_engine *e, int adm_table, int rcs_table, size_t n, ui
 int rsd_set_debris_surrogate(rsd_engine *e, int type, const rsd_mlp *m);
 
 /* Self-test: evaluate one of the device's binary32 functions on n inputs
- * (fn: 0 log1p(a), 1 log(a), 2 sin(a), 3 cos(a), 4 atan2(a, b), 5 acos(a),
- * 6 sqrt(a), 7 a / b, 8 sin(2 pi a), 9 cos(2 pi a), 10 the pulse path's fast
- * acos(a), 11 the gate window's transposed wave sum of a + j b, j = 0..31 (n a
- * multiple of 64), 12 the wind index's log1p(a) for a >= 0).  No counterpart in
- * the reference; it lets a caller check the device arithmetic the advection
- * uses. */
+ * (fn: 1 log(a), 2 sin(a), 3 cos(a), 4 atan2(a, b), 5 acos(a), 6 sqrt(a),
+ * 7 a / b, 8 sin(2 pi a), 9 cos(2 pi a), 10 the pulse path's fast acos(a),
+ * 11 the gate window's transposed wave sum of a + j b, j = 0..31 (n a multiple
+ * of 64); other values return 0).  No counterpart in the reference; it lets a
+ * caller check the device arithmetic the advection uses. */
 int rsd_math_probe(int device_index, int fn, const float *a, const float *b, float *out, size_t n);
 
 #ifdef __cplusplus

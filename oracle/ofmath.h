@@ -4,18 +4,18 @@
  * TEST INFRASTRUCTURE: this file belongs to the oracle.  Only tests/, the smoke
  * check and bench.py's cpu_baseline leg may build or call it.
  *
- * The reference evaluates log1p/atan2/acos/sincos/cos/log with the OpenCL
- * device's built-ins (rs.cl:151, 218-219, 275-276, 292-305, 326-344, 848-857,
- * 894, 1066), whose accuracy is implementation-defined.  The oracle fixes one
- * documented algorithm per function so that the GPU kernels, which implement the
- * same operation sequence, reproduce positions, respawn decisions and gate
- * indices bit for bit:
+ * The reference evaluates atan2/acos/sincos/cos/log with the OpenCL device's
+ * built-ins (rs.cl:218-219, 275-276, 292-305, 326-344, 848-857, 894, 1066), whose
+ * accuracy is implementation-defined.  The oracle fixes one documented
+ * algorithm per function so that the GPU kernels, which implement the same
+ * operation sequence, reproduce positions, respawn decisions and gate indices
+ * bit for bit.  (The stretched wind index's log1p, rs.cl:151, goes through the
+ * index tables of rs_oracle.c instead.)
  *
- *   log / log1p : fdlibm-style reduction u = m 2^k, m in [sqrt(1/2), sqrt(2)),
+ *   log         : fdlibm-style reduction u = m 2^k, m in [sqrt(1/2), sqrt(2)),
  *                 s = f/(2+f), log(1+f) = f - (hfsq - s (hfsq + R(s^2))) with R the
  *                 atanh series 2/3 z + 2/5 z^2 + 2/7 z^3 + 2/9 z^4 (truncation error
- *                 below 3e-9 relative), ln2 split in a 16-bit head and a tail;
- *                 log1p adds the rounding correction (x - (u - 1)) / u of u = 1 + x.
+ *                 below 3e-9 relative), ln2 split in a 16-bit head and a tail.
  *   sincos      : Cody-Waite reduction by pi/2 in three parts (each part the binary32
  *                 rounding of what is left of pi/2), Taylor polynomials of sin to r^9
  *                 and cos to r^10 on |r| <= pi/4.
@@ -83,59 +83,6 @@ static inline float of_logf(float x) {
     float m = of_split(x, &k);
     float r = of_log_kernel(m - 1.0f, k, 0.0f);
     return (scale != 0.0f) ? r - scale * 0x1.62e430p-1f : r;
-}
-
-/* Division-free log1p used by the wind index: u = 1 + x = 2^k m, m in [1, 2);
- * the top 7 mantissa bits select c_j = 1 + (j + 1/2) / 128 with the binary32
- * reciprocal g_j = fl(1 / c_j) and L_j = -log(g_j) split into binary32 head and
- * tail (binary64 log of the host C library); r = fma(m, g_j, -1) (|r| < 2^-7),
- * log(m) = L_j + log1p(r), log1p(r) = r + r^2 (-1/2 + r (1/3 - r / 4)); the
- * rounding of u = 1 + x adds (x - (u - 1)) g_j 2^-k. */
-#define OF_LOG_TABLE 128
-static float of_log_g[OF_LOG_TABLE], of_log_hi[OF_LOG_TABLE], of_log_lo[OF_LOG_TABLE];
-static int of_log_ready = 0;
-
-static inline void of_log_table_init(void) {
-    if (of_log_ready) return;
-    for (int j = 0; j < OF_LOG_TABLE; j++) {
-        double c = 1.0 + ((double)j + 0.5) / (double)OF_LOG_TABLE;
-        float g = (float)(1.0 / c);
-        double L = -log((double)g);
-        float hi = (float)L;
-        of_log_g[j] = g;
-        of_log_hi[j] = hi;
-        of_log_lo[j] = (float)(L - (double)hi);
-    }
-    of_log_ready = 1;
-}
-
-static inline float of_log1pf(float x) {
-    if (x == 0.0f) return x;
-    if (!(x > -1.0f)) return (x == -1.0f) ? -INFINITY : NAN;
-    if (isinf(x)) return x;
-    if (x < 0.0f) {
-        /* not on the wind-index path (its argument is s |p| >= 0) */
-        float u = 1.0f + x;
-        int k;
-        float m = of_split(u, &k);
-        return of_log_kernel(m - 1.0f, k, (x - (u - 1.0f)) / u);
-    }
-    if (x < 0x1p-7f) return fmaf(x * x, fmaf(x, fmaf(x, -0.25f, 0x1.555556p-2f), -0.5f), x);
-    of_log_table_init();
-    float u = 1.0f + x;
-    float tail = (u < 0x1p+24f) ? x - (u - 1.0f) : 0.0f;
-    uint32_t iu = of_f2bits(u);
-    int k = (int)((iu >> 23) & 0xffu) - 127;
-    int j = (int)((iu >> 16) & 0x7fu);
-    float m = of_bits2f((iu & 0x007fffffu) | 0x3f800000u);
-    float g = of_log_g[j];
-    float r = fmaf(m, g, -1.0f);
-    float p = fmaf(r * r, fmaf(r, fmaf(r, -0.25f, 0x1.555556p-2f), -0.5f), r);
-    float kf = (float)k;
-    float scale = of_bits2f((uint32_t)(127 - k) << 23);
-    float lo = fmaf(kf, OF_LN2_LO, of_log_lo[j]) + (tail * g) * scale;
-    float hi = fmaf(kf, OF_LN2_HI, of_log_hi[j]);
-    return hi + (p + lo);
 }
 
 /* sin and cos of x; exact reduction for |x| below about 2^20. */

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
 
 PHASE_F32, PHASE_F64, FULL_F64 = 0, 1, 2
-MATH = {"log1p": 0, "log": 1, "sin": 2, "cos": 3, "atan2": 4, "acos": 5, "pow": 6, "exp": 7}
+MATH = {"log": 1, "sin": 2, "cos": 3, "atan2": 4, "acos": 5, "pow": 6, "exp": 7}
 
 _lib = None
 
@@ -68,6 +68,7 @@ def lib():
             "orc_math": (None, [I, P, P, P, S]),
             "orc_moments": (None, [P, I, I, D, D, D, D, D, P]),
             "orc_wind_index": (F4, [F4, P, P]),
+            "orc_wind_index_table": (None, [P, P, P, P, P]),
             "orc_sample3d": (F4, [T, F, F, F]),
             "orc_sample2d": (F4, [T, F, F]),
             "orc_ellipsoid_rcs": (F4, [F4, P, P]),
@@ -121,6 +122,15 @@ def moments(iq, lam, prt, gain, range_start, range_delta):
 
 def wind_index(pos, les_desc, sim_desc):
     return lib().orc_wind_index(_f4(pos), _p(f32(les_desc)), _p(f32(sim_desc))).np()
+
+
+def wind_index_table(les_desc, sim_desc):
+    """(rows[768, 4], ih[3], jm[3]) of the stretched-grid index."""
+    rows = np.zeros((768, 4), np.float32)
+    ih = np.zeros(3, np.float32)
+    jm = np.zeros(3, np.float32)
+    lib().orc_wind_index_table(_p(f32(les_desc)), _p(f32(sim_desc)), _p(rows), _p(ih), _p(jm))
+    return rows, ih, jm
 
 
 def sample3d(t, u, v, w):

@@ -6,12 +6,11 @@
 #include <stddef.h>
 #include "ofmath.h"
 
-enum { OF_LOG1P = 0, OF_LOG, OF_SIN, OF_COS, OF_ATAN2, OF_ACOS, OF_POW, OF_EXP };
+enum { OF_LOG = 1, OF_SIN, OF_COS, OF_ATAN2, OF_ACOS, OF_POW, OF_EXP };
 
 void orc_math(int fn, const float *a, const float *b, float *out, size_t n) {
     for (size_t i = 0; i < n; i++) {
         switch (fn) {
-            case OF_LOG1P: out[i] = of_log1pf(a[i]); break;
             case OF_LOG: out[i] = of_logf(a[i]); break;
             case OF_SIN: out[i] = of_sinf(a[i]); break;
             case OF_COS: out[i] = of_cosf(a[i]); break;

@@ -45,17 +45,85 @@ void orc_rand4(ou4 *s, of4 *r) {
     r->w = (float)s->w * n;
 }
 
+/* Stretched-grid index of rs.cl:130-151, c = s8 + copysign(s0 log1p(s4 |r|), r),
+ * as the engine documents its evaluation (include/rsd.h rsd_wind_index_table):
+ * P(x) = s0 log1p(s4 x) from ORC_ITAB_N cubic pieces per axis over [0, X], X the
+ * box's extent from the axis origin times 1 + 2^-10, each piece the binary64
+ * cubic through P at the four Chebyshev nodes u_k = 1/2 + cos((2k + 1) pi / 8) / 2
+ * of its local coordinate f (x = (j + f) / ih), coefficients in binary32. */
+#define ORC_ITAB_N 256
+
+static void itab_piece_axis(double s0, double s4, float ihf, float *rows) {
+    const double k1 = 0x1.d906bcf328d46p-1, k3 = 0x1.87de2a6aea964p-2;   /* cos(pi/8), cos(3 pi/8) */
+    const double u[4] = {0.5 + 0.5 * k1, 0.5 + 0.5 * k3, 0.5 - 0.5 * k3, 0.5 - 0.5 * k1};
+    const double ih = (double)ihf;
+    for (int j = 0; j < ORC_ITAB_N; j++) {
+        double y[4];
+        for (int k = 0; k < 4; k++) y[k] = s0 * log1p(s4 * (((double)j + u[k]) / ih));
+        /* Newton divided differences, then the Newton form expanded in powers of f */
+        const double d01 = (y[1] - y[0]) / (u[1] - u[0]);
+        const double d12 = (y[2] - y[1]) / (u[2] - u[1]);
+        const double d23 = (y[3] - y[2]) / (u[3] - u[2]);
+        const double d012 = (d12 - d01) / (u[2] - u[0]);
+        const double d123 = (d23 - d12) / (u[3] - u[1]);
+        const double d0123 = (d123 - d012) / (u[3] - u[0]);
+        float *r = rows + 4 * j;
+        r[0] = (float)(((y[0] - d01 * u[0]) + d012 * (u[0] * u[1])) - d0123 * ((u[0] * u[1]) * u[2]));
+        r[1] = (float)((d01 - d012 * (u[0] + u[1])) + d0123 * ((u[0] * u[1] + u[0] * u[2]) + u[1] * u[2]));
+        r[2] = (float)(d012 - d0123 * ((u[0] + u[1]) + u[2]));
+        r[3] = (float)d0123;
+    }
+}
+
+void orc_wind_index_table(const float *d, const float *sd, float *rows, float *ih, float *jm) {
+    const float org[3] = {sd[8] + 0.5f * sd[12], sd[9] + 0.5f * sd[13], 0.0f};
+    for (int a = 0; a < 3; a++) {
+        const float lo = sd[8 + a];
+        const float hi = sd[8 + a] + sd[12 + a];
+        double X = fabs((double)lo - (double)org[a]);
+        const double Xh = fabs((double)hi - (double)org[a]);
+        if (Xh > X) X = Xh;
+        X = X * (1.0 + 0x1p-10);
+        if (X < 1.0e-3) X = 1.0e-3;
+        ih[a] = (float)((double)ORC_ITAB_N / X);
+        jm[a] = (float)(ORC_ITAB_N - 1);
+        itab_piece_axis((double)d[a], (double)d[4 + a], ih[a], rows + 4 * ORC_ITAB_N * a);
+    }
+}
+
+static inline float itab_eval(float r, const float *T, float ih, float jm, float base) {
+    const float t = fabsf(r) * ih;
+    const float jf = fminf(floorf(t), jm);
+    const float f = t - jf;
+    const float *c = T + 4 * (int)jf;
+    const float p = fmaf(fmaf(fmaf(c[3], f, c[2]), f, c[1]), f, c[0]);
+    return copysignf(p, r) + base;
+}
+
+/* The tables of the last (descriptor, box) seen by this thread. */
+static __thread float itab_key[22];
+static __thread int itab_have;
+static __thread float itab_rows[3 * ORC_ITAB_N * 4], itab_ih[3], itab_jm[3];
+
 of4 orc_wind_index(of4 pos, const float *d, const float *sd) {
     of4 c = {0.0f, 0.0f, 0.0f, 0.0f};
     uint32_t spacing = fbits(d[7]);
     if (spacing == 7u) {
-        /* rs.cl:130-151: relative to the box centre in x, y; stretched log1p map */
+        /* rs.cl:130-151: relative to the box centre in x, y and to 0 in z */
+        float key[22];
+        memcpy(key, d, 16 * sizeof(float));
+        memcpy(key + 16, sd + 8, 3 * sizeof(float));
+        memcpy(key + 19, sd + 12, 3 * sizeof(float));
+        if (!itab_have || memcmp(key, itab_key, sizeof(key)) != 0) {
+            orc_wind_index_table(d, sd, itab_rows, itab_ih, itab_jm);
+            memcpy(itab_key, key, sizeof(key));
+            itab_have = 1;
+        }
         float cx = sd[8] + 0.5f * sd[12];
         float cy = sd[9] + 0.5f * sd[13];
-        float rx = pos.x - cx, ry = pos.y - cy, rz = pos.z - 0.0f;
-        c.x = copysignf(d[0], rx) * of_log1pf(d[4] * fabsf(rx)) + d[8];
-        c.y = copysignf(d[1], ry) * of_log1pf(d[5] * fabsf(ry)) + d[9];
-        c.z = copysignf(d[2], rz) * of_log1pf(d[6] * fabsf(rz)) + d[10];
+        c.x = itab_eval(pos.x - cx, itab_rows, itab_ih[0], itab_jm[0], d[8]);
+        c.y = itab_eval(pos.y - cy, itab_rows + 4 * ORC_ITAB_N, itab_ih[1], itab_jm[1], d[9]);
+        c.z = itab_eval(pos.z - 0.0f, itab_rows + 8 * ORC_ITAB_N, itab_ih[2], itab_jm[2], d[10]);
     } else if (spacing == 0u) {
         /* rs.cl:173 */
         c.x = fmaf(pos.x, d[0], d[4]);

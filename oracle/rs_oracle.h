@@ -32,8 +32,12 @@ enum {
 /* rs.cl:35-44 -- four 31-bit multiplicative LCG lanes, one step each. */
 void orc_rand4(ou4 *seed, of4 *r);
 
-/* rs.cl:123-176 -- table coordinates of a position. */
+/* rs.cl:123-176 -- table coordinates of a position (stretched axes through the
+ * index tables below). */
 of4 orc_wind_index(of4 pos, const float *les_desc, const float *sim_desc);
+/* The stretched-grid index tables for a descriptor and the box of sim_desc:
+ * rows[3 * 256 * 4] (x, y, z pieces), ih[3], jm[3]. */
+void orc_wind_index_table(const float *les_desc, const float *sim_desc, float *rows, float *ih, float *jm);
 
 /* Time interpolation of the advances' wind (and Cn^2) towards the next LES
  * volume with weight alpha; wind_next NULL switches it off (the reference's

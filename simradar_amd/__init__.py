@@ -91,7 +91,7 @@ EXPORTED_RSD = [
     "rsd_batch_timer_start", "rsd_batch_timer_stop", "rsd_batch_last_error",
     "rsd_mlp_create", "rsd_mlp_destroy", "rsd_mlp_set_params", "rsd_mlp_get_params", "rsd_mlp_set_output_scaling",
     "rsd_mlp_forward", "rsd_mlp_gradient", "rsd_mlp_set_data", "rsd_mlp_train", "rsd_mlp_sync", "rsd_mlp_last_error",
-    "rsd_debris_samples", "rsd_set_debris_surrogate",
+    "rsd_debris_samples", "rsd_set_debris_surrogate", "rsd_wind_index_table",
 ]
 
 
@@ -289,6 +289,7 @@ def _declare(lib):
         "rsd_debris_samples": (I, [P, I, I, S, ctypes.c_uint64, P, P]),
         "rsd_set_debris_surrogate": (I, [P, I, P]),
         "rsd_math_probe": (I, [I, I, P, P, P, S]),
+        "rsd_wind_index_table": (I, [P, P, P, P, P]),
         "rsd_profile": (I, [P, I]),
         "rsd_io_test": (I, [P, S, I, ctypes.POINTER(ctypes.c_double)]),
         "rsd_profile_read": (I, [P, I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]),
@@ -313,6 +314,21 @@ def load():
 
 def gpu_count():
     return int(load().rsd_device_count())
+
+
+def wind_index_table(les_desc, sim_desc):
+    """The engine's stretched-grid index tables for an LES descriptor and a box
+    (rsd_wind_index_table; host only): rows[768, 4] (x, y, z: 256 cubic pieces
+    each), ih[3] pieces per metre, jm[3] last piece index."""
+    les = np.ascontiguousarray(les_desc, np.float32)
+    sd = np.ascontiguousarray(sim_desc, np.float32)
+    rows = np.zeros((768, 4), np.float32)
+    ih = np.zeros(3, np.float32)
+    jm = np.zeros(3, np.float32)
+    rc = load().rsd_wind_index_table(_ptr(les), _ptr(sd), _ptr(rows), _ptr(ih), _ptr(jm))
+    if rc:
+        raise RuntimeError("rsd_wind_index_table failed (%d)" % rc)
+    return rows, ih, jm
 
 
 def math_probe(fn, a, b=None, device=0):

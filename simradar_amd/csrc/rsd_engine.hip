@@ -176,35 +176,56 @@ __device__ __forceinline__ float4 sample2d(const Tex2 &t, float u, float v) {
     return lerp4(y0, y1, b);
 }
 
-// rs.cl:123-176: LES table coordinates of a position.  STRETCHED: the launch
-// has checked that the table is stretched (slot 7) with non-negative growth
-// factors (slots 4-6), so only that branch is compiled; (cx, cy) is the
-// domain centre sd[8..9] + 0.5 sd[12..13], rounded as below.
+// Stretched-grid index tables.  The index of a stretched LES axis is
+// c = s8 + copysign(s0 log1p(s4 |r|), r) (rs.cl:130-151), r the position
+// relative to the box centre in x and y and to z = 0 in z.  P(x) = s0 log1p(s4 x)
+// is evaluated from RSD_ITAB_N cubic pieces per axis over x in [0, X], X the
+// box's extent from the origin on that axis (with a margin): piece j covers x =
+// (j + f) / ih, f in [0, 1], and holds the binary32 monomial coefficients
+// (a0, a1, a2, a3) of the cubic interpolating binary64 P at the four Chebyshev
+// nodes of f (rsd_wind_index_table builds them on the host).  Evaluation:
+// t = |r| ih, j = min(floor(t), N - 1), f = t - j, P = a0 + f (a1 + f (a2 + f a3))
+// by fma, c = copysign(P, r) + s8.  Rows: x at 0, y at N, z at 2 N.
+#define RSD_ITAB_N 256
+#define RSD_ITAB_ROWS (3 * RSD_ITAB_N)
+
+struct WIdx {
+    const float4 *t;        // RSD_ITAB_ROWS rows in device memory
+    float ih[3];            // pieces per metre
+    float jm[3];            // last piece index (N - 1)
+};
+
+__device__ __forceinline__ float itab_eval(float r, const float4 *T, float ih, float jm, float base) {
+    const float t = __builtin_fabsf(r) * ih;
+    const float jf = __builtin_fminf(__builtin_floorf(t), jm);
+    const float f = t - jf;
+    const float4 a = T[(int)jf];
+    const float p = rf_fma(rf_fma(rf_fma(a.w, f, a.z), f, a.y), f, a.x);
+    return __builtin_copysignf(p, r) + base;
+}
+
+__device__ __forceinline__ void itab_stage(float4 *dst, const float4 *src) {
+    for (int j = threadIdx.x; j < RSD_ITAB_ROWS; j += blockDim.x) dst[j] = src[j];
+}
+
+// rs.cl:123-176: LES table coordinates of a position.  Stretched tables (slot
+// 7 = 7) go through the index tables T (LDS or global) with the scales of w;
+// STRETCHED: the launch has checked that the table is stretched, so only that
+// branch is compiled, and (cx, cy) is the domain centre sd[8..9] + 0.5
+// sd[12..13] rounded as below.
 template <bool STRETCHED = false>
-__device__ __forceinline__ float4 wind_index(float4 pos, const F16 &d, const F16 &sd, const float4 *logtab,
+__device__ __forceinline__ float4 wind_index(float4 pos, const F16 &d, const F16 &sd, const float4 *T, const WIdx &w,
                                              float cx = 0.0f, float cy = 0.0f) {
     float4 c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (STRETCHED) {
-        const float rx = pos.x - cx, ry = pos.y - cy, rz = pos.z - 0.0f;
-        c.x = __builtin_copysignf(d.s[0], rx) * rf_log1p_pos(d.s[4] * __builtin_fabsf(rx), logtab) + d.s[8];
-        c.y = __builtin_copysignf(d.s[1], ry) * rf_log1p_pos(d.s[5] * __builtin_fabsf(ry), logtab) + d.s[9];
-        c.z = __builtin_copysignf(d.s[2], rz) * rf_log1p_pos(d.s[6] * __builtin_fabsf(rz), logtab) + d.s[10];
-        return c;
-    }
     uint32_t spacing = __float_as_uint(d.s[7]);
-    if (spacing == 7u) {
-        float cx = sd.s[8] + 0.5f * sd.s[12];
-        float cy = sd.s[9] + 0.5f * sd.s[13];
-        float rx = pos.x - cx, ry = pos.y - cy, rz = pos.z - 0.0f;
-        if (d.s[4] >= 0.0f && d.s[5] >= 0.0f && d.s[6] >= 0.0f) {   // growing spacing: arguments >= 0
-            c.x = __builtin_copysignf(d.s[0], rx) * rf_log1p_pos(d.s[4] * __builtin_fabsf(rx), logtab) + d.s[8];
-            c.y = __builtin_copysignf(d.s[1], ry) * rf_log1p_pos(d.s[5] * __builtin_fabsf(ry), logtab) + d.s[9];
-            c.z = __builtin_copysignf(d.s[2], rz) * rf_log1p_pos(d.s[6] * __builtin_fabsf(rz), logtab) + d.s[10];
-        } else {
-            c.x = __builtin_copysignf(d.s[0], rx) * rf_log1pf(d.s[4] * __builtin_fabsf(rx), logtab) + d.s[8];
-            c.y = __builtin_copysignf(d.s[1], ry) * rf_log1pf(d.s[5] * __builtin_fabsf(ry), logtab) + d.s[9];
-            c.z = __builtin_copysignf(d.s[2], rz) * rf_log1pf(d.s[6] * __builtin_fabsf(rz), logtab) + d.s[10];
+    if (STRETCHED || spacing == 7u) {
+        if (!STRETCHED) {
+            cx = sd.s[8] + 0.5f * sd.s[12];
+            cy = sd.s[9] + 0.5f * sd.s[13];
         }
+        c.x = itab_eval(pos.x - cx, T, w.ih[0], w.jm[0], d.s[8]);
+        c.y = itab_eval(pos.y - cy, T + RSD_ITAB_N, w.ih[1], w.jm[1], d.s[9]);
+        c.z = itab_eval(pos.z - 0.0f, T + 2 * RSD_ITAB_N, w.ih[2], w.jm[2], d.s[10]);
     } else if (spacing == 0u) {
         c.x = rf_fma(pos.x, d.s[0], d.s[4]);
         c.y = rf_fma(pos.y, d.s[1], d.s[5]);
@@ -955,7 +976,7 @@ struct RainK {
     const float4 *drop;
     F4 drop_desc;
     uint32_t dragged;
-    const float *logtab;    // global copy of the log1p table (3 x 128 floats)
+    WIdx widx;              // stretched-grid index tables of the wind slot
     PulseK pk;
     // Pulse launches stage the drop RCS table (drop_n entries) and the angular
     // weight table (angw_n entries) in LDS at byte offset tab_off; tab_lds = 0:
@@ -1065,8 +1086,8 @@ __device__ __forceinline__ bool rain_move(const RainK &a, size_t i, float4 &pos,
 }
 
 template <bool TI = false, bool FAST = false>
-__device__ __forceinline__ void rain_gather(const RainK &a, float4 pos, const float4 *logtab, Gather3 &g, Gather3 *g2 = nullptr) {
-    float4 c = wind_index<FAST>(pos, a.les, a.sd, logtab, a.cx, a.cy);
+__device__ __forceinline__ void rain_gather(const RainK &a, float4 pos, const float4 *itab, Gather3 &g, Gather3 *g2 = nullptr) {
+    float4 c = wind_index<FAST>(pos, a.les, a.sd, itab, a.widx, a.cx, a.cy);
     gather3d(a.wind, c.x, c.y, c.z, g);
     if (TI) {
         Tex3 t2 = a.wind;
@@ -1112,8 +1133,6 @@ __device__ __forceinline__ void rain_drag(const RainK &a, float4 pos, float4 &ve
     }
 }
 
-__device__ __forceinline__ void logtab_load(float4 *dst, const float *src) { rf_log_rows(dst, src); }
-
 // Scatterers [lo, hi) of one rain population: advance and/or pulse terms into
 // the block's LDS gate sums acc (cleared by the caller).  The hi/lo variant
 // (96 B of state per scatterer, meant for populations far beyond the 256 MB
@@ -1122,7 +1141,7 @@ __device__ __forceinline__ void logtab_load(float4 *dst, const float *src) { rf_
 // tables in LDS, binary64 phase) compiled without the branches for the others,
 // which keeps the loop's uniform values in scalar registers.
 template <bool ADV, bool PULSE, bool HILO, bool WIN, bool TRI, bool TI, bool FAST = false>
-__device__ __forceinline__ void rain_range_t(const RainK &a, size_t lo, size_t hi, double *acc, const float4 *logtab) {
+__device__ __forceinline__ void rain_range_t(const RainK &a, size_t lo, size_t hi, double *acc, const float4 *itab) {
     const float2 *wl = win_weights(acc, a.pk);
     const float *grl = win_gates(acc, a.pk);
     const bool tl = PULSE && (FAST || a.tab_lds);
@@ -1173,7 +1192,7 @@ __device__ __forceinline__ void rain_range_t(const RainK &a, size_t lo, size_t h
             } else {
                 stale = false;
                 moved = true;
-                rain_gather<TI, FAST>(a, p, logtab, g, &g2);
+                rain_gather<TI, FAST>(a, p, itab, g, &g2);
             }
             st4<HILO>(&a.pos[i], make_float4(p.x, p.y, p.z, stale ? -p.w : p.w));
         }
@@ -1227,7 +1246,7 @@ __device__ __forceinline__ void bst4(__amdgpu_buffer_rsrc_t r, uint32_t off, flo
 // wait on a read waits for that read alone, not for the writes after it.
 // Stream indices fit 32 bits (the launch checks RSD_BUF_MAX_BYTES).
 template <bool WIN, bool TRI, bool TI>
-__device__ __forceinline__ void rain_range_buf(const RainK &a, size_t lo, size_t hi, double *acc, const float4 *logtab) {
+__device__ __forceinline__ void rain_range_buf(const RainK &a, size_t lo, size_t hi, double *acc, const float4 *itab) {
     const float2 *wl = win_weights(acc, a.pk);
     const float *grl = win_gates(acc, a.pk);
     const float4 *dropt = (const float4 *)((const char *)acc + a.tab_off);
@@ -1273,7 +1292,7 @@ __device__ __forceinline__ void rain_range_buf(const RainK &a, size_t lo, size_t
             }
         }
         Gather3 g, g2;
-        rain_gather<TI, true>(a, p, logtab, g, &g2);
+        rain_gather<TI, true>(a, p, itab, g, &g2);
         bst4(rp, off, make_float4(p.x, p.y, p.z, stale ? -p.w : p.w));
         float4 sa = zero;
         float r = 0.0f;
@@ -1294,16 +1313,16 @@ __device__ __forceinline__ void rain_range_buf(const RainK &a, size_t lo, size_t
 // registers has no second fold path (which makes the compiler copy the 32
 // window registers at every step).
 template <bool ADV, bool PULSE, bool HILO, bool TI = false, bool FAST = false>
-__device__ __forceinline__ void rain_range(const RainK &a, size_t lo, size_t hi, double *acc, const float4 *logtab,
+__device__ __forceinline__ void rain_range(const RainK &a, size_t lo, size_t hi, double *acc, const float4 *itab,
                                            bool use_win) {
     if constexpr (FAST && ADV && PULSE && !HILO) {
-        if (use_win && a.pk.tri) rain_range_buf<true, true, TI>(a, lo, hi, acc, logtab);
-        else if (use_win) rain_range_buf<true, false, TI>(a, lo, hi, acc, logtab);
-        else rain_range_buf<false, false, TI>(a, lo, hi, acc, logtab);
+        if (use_win && a.pk.tri) rain_range_buf<true, true, TI>(a, lo, hi, acc, itab);
+        else if (use_win) rain_range_buf<true, false, TI>(a, lo, hi, acc, itab);
+        else rain_range_buf<false, false, TI>(a, lo, hi, acc, itab);
     } else {
-        if (use_win && a.pk.tri) rain_range_t<ADV, PULSE, HILO, true, true, TI, FAST>(a, lo, hi, acc, logtab);
-        else if (use_win) rain_range_t<ADV, PULSE, HILO, true, false, TI, FAST>(a, lo, hi, acc, logtab);
-        else rain_range_t<ADV, PULSE, HILO, false, false, TI, FAST>(a, lo, hi, acc, logtab);
+        if (use_win && a.pk.tri) rain_range_t<ADV, PULSE, HILO, true, true, TI, FAST>(a, lo, hi, acc, itab);
+        else if (use_win) rain_range_t<ADV, PULSE, HILO, true, false, TI, FAST>(a, lo, hi, acc, itab);
+        else rain_range_t<ADV, PULSE, HILO, false, false, TI, FAST>(a, lo, hi, acc, itab);
     }
 }
 
@@ -1320,12 +1339,8 @@ __device__ __forceinline__ void rain_range(const RainK &a, size_t lo, size_t hi,
 // specialised rain loop covers (rain_fast).
 template <bool ADV, bool PULSE, bool FAST = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_rain_batch(
-    const RainK *__restrict__ doms, const int2 *__restrict__ items, int n_items, int chunk,
-    const float *__restrict__ logtab_g) {
+    const RainK *__restrict__ doms, const int2 *__restrict__ items, int n_items, int chunk) {
     extern __shared__ double acc[];
-    __shared__ float4 logtab[RF_LOG_TABLE];
-    if (ADV) logtab_load(logtab, logtab_g);
-    __syncthreads();
     const unsigned G = gridDim.x, b = blockIdx.x;
     for (int t = (int)b; t < n_items; t += (int)G) {
         const int2 it = items[t];
@@ -1336,7 +1351,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         if (use_win) stage_window(acc, a.pk);
         const size_t lo = (size_t)it.y * (size_t)chunk;
         const size_t hi = lo + (size_t)chunk < a.n ? lo + (size_t)chunk : a.n;    // a.n = 0: the domain sits this round out
-        rain_range<ADV, PULSE, false, false, FAST>(a, lo, hi > lo ? hi : lo, acc, logtab, use_win);
+        rain_range<ADV, PULSE, false, false, FAST>(a, lo, hi > lo ? hi : lo, acc, a.widx.t, use_win);
         if (PULSE) acc_flush_row(acc, a.pk, it.y);
         __syncthreads();
     }
@@ -1406,7 +1421,7 @@ struct DebrisK {
     uint32_t refresh;
     uint32_t pulse_concept;
     uint32_t reuse_rcs;     // advance and refresh use the same table and concept
-    const float *logtab;
+    WIdx widx;              // stretched-grid index tables of the wind slot
     MlpDev net;             // surrogate network of this type (k_debris<.., true>)
     PulseK pk;
 };
@@ -1538,11 +1553,11 @@ __device__ __forceinline__ void debris_forces(const DebrisK &a, float4 ori, floa
     tum = norm4(tq);
 }
 
-__device__ __forceinline__ bool debris_advance(const DebrisK &a, size_t i, float4 &pos, float4 &ori, float4 &vel, float4 &tum, float4 &rcs, const float4 *logtab) {
+__device__ __forceinline__ bool debris_advance(const DebrisK &a, size_t i, float4 &pos, float4 &ori, float4 &vel, float4 &tum, float4 &rcs, const float4 *itab) {
     const F16 &sd = a.sd;
     const uint32_t concept = __float_as_uint(sd.s[5]);
     if (debris_move(a, i, pos, ori, vel, tum, rcs)) return true;
-    float4 c = wind_index(pos, a.les, sd, logtab);
+    float4 c = wind_index(pos, a.les, sd, itab, a.widx);
     float4 bg = sample3d(a.wind, c.x, c.y, c.z);
     float4 ur = make_float4(bg.x - vel.x, bg.y - vel.y, bg.z - vel.z, bg.w - vel.w);
     float4 uh = qrot(norm4(ur), qconj(ori));
@@ -1568,7 +1583,7 @@ __device__ __forceinline__ bool debris_advance(const DebrisK &a, size_t i, float
 // block's LDS gate sums (cleared, window tables staged by the caller).  SURR:
 // the surrogate network in LDS (nw, nb) with per-wave scratch nscr.
 template <bool ADV, bool PULSE, bool SURR>
-__device__ __forceinline__ void debris_range(const DebrisK &a, size_t lo, size_t hi, double *acc, const float4 *logtab,
+__device__ __forceinline__ void debris_range(const DebrisK &a, size_t lo, size_t hi, double *acc, const float4 *itab,
                                              const uint16_t *nw, const float *nb, float *nscr) {
     const bool use_win = PULSE && a.pk.fold == RSD_FOLD_WINDOW;
     const float2 *wl = win_weights(acc, a.pk);
@@ -1593,7 +1608,7 @@ __device__ __forceinline__ void debris_range(const DebrisK &a, size_t lo, size_t
                 t = a.tum[i];
                 respawned = debris_move(a, i, p, o, v, t, rcs);
                 if (!respawned) {
-                    float4 c = wind_index(p, a.les, a.sd, logtab);
+                    float4 c = wind_index(p, a.les, a.sd, itab, a.widx);
                     bg = sample3d(a.wind, c.x, c.y, c.z);
                     ur = make_float4(bg.x - v.x, bg.y - v.y, bg.z - v.z, bg.w - v.w);
                 }
@@ -1636,7 +1651,7 @@ __device__ __forceinline__ void debris_range(const DebrisK &a, size_t lo, size_t
         } else {
         if (ADV && live) {
             float4 v = a.vel[i], t = a.tum[i];
-            respawned = debris_advance(a, i, p, o, v, t, rcs, logtab);
+            respawned = debris_advance(a, i, p, o, v, t, rcs, itab);
             a.pos[i] = p;
             a.ori[i] = o;
             a.vel[i] = v;
@@ -1689,7 +1704,6 @@ __device__ __forceinline__ int step_type(const StepDebris &d) {
 template <bool ADV, bool PULSE, bool SURR>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_debris(StepDebris d) {
     extern __shared__ double acc[];
-    __shared__ float4 logtab[RF_LOG_TABLE];
     __shared__ uint16_t nw[SURR ? MLP_WN : 2];
     __shared__ float nb[SURR ? MLP_BN : 2];
     __shared__ float nscr[SURR ? 4 * 64 * MLP_N4 : 2];
@@ -1699,14 +1713,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         for (int q = threadIdx.x; q < MLP_WN; q += blockDim.x) nw[q] = a.net.w[q];
         for (int q = threadIdx.x; q < MLP_BN; q += blockDim.x) nb[q] = a.net.b[q];
     }
-    if (ADV) logtab_load(logtab, a.logtab);
     if (PULSE) acc_clear(acc, 4 * a.pk.R);
     else __syncthreads();
     size_t lo, hi;
     const unsigned L = block_range_at(a.begin, a.end, blockIdx.x - (unsigned)d.first[j],
                                       (unsigned)(d.first[j + 1] - d.first[j]), lo, hi);
     if (PULSE && a.pk.fold == RSD_FOLD_WINDOW) stage_window(acc, a.pk);
-    debris_range<ADV, PULSE, SURR>(a, lo, hi, acc, logtab, nw, nb, nscr);
+    // index tables read from global memory (L1 / L2): this kernel's LDS holds
+    // the network
+    debris_range<ADV, PULSE, SURR>(a, lo, hi, acc, a.widx.t, nw, nb, nscr);
     if (PULSE) acc_flush_row(acc, a.pk, a.pk.row0 + (int)L);
 }
 
@@ -1722,7 +1737,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 template <bool ADV, bool PULSE, bool HILO, bool TI = false, bool FAST = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RSD_RAIN_WAVES))) void k_rain(RainK a, StepDebris d) {
     extern __shared__ double acc[];
-    __shared__ float4 logtab[RF_LOG_TABLE];
+    __shared__ float4 itab[RSD_ITAB_ROWS];
     const unsigned nd = (unsigned)d.first[d.nt];
     const unsigned nr = gridDim.x - a.fin_blocks;      // end of the rain blocks
     if (blockIdx.x >= nr) {     // the previous pulse's finalize, in the slots the last rain blocks free
@@ -1730,7 +1745,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RSD_RAIN_WA
         if (j < a.fin.n4r) finalize_column(a.fin, j);
         return;
     }
-    if (ADV) logtab_load(logtab, a.logtab);
+    if (ADV) itab_stage(itab, a.widx.t);     // riding debris share the rain's wind slot
     if (blockIdx.x < nd) {
         const int j = step_type(d);
         const DebrisK &da = d.t[j];
@@ -1740,7 +1755,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RSD_RAIN_WA
         size_t lo, hi;
         const unsigned L = block_range_at(da.begin, da.end, blockIdx.x - (unsigned)d.first[j],
                                           (unsigned)(d.first[j + 1] - d.first[j]), lo, hi);
-        debris_range<ADV, PULSE, false>(da, lo, hi, acc, logtab, nullptr, nullptr, nullptr);
+        debris_range<ADV, PULSE, false>(da, lo, hi, acc, itab, nullptr, nullptr, nullptr);
         if (PULSE) acc_flush_row(acc, da.pk, da.pk.row0 + (int)L);
         return;
     }
@@ -1751,7 +1766,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RSD_RAIN_WA
     const bool use_win = PULSE && a.pk.fold == RSD_FOLD_WINDOW;
     if (PULSE) stage_rain_tables(acc, a);
     if (use_win) stage_window(acc, a.pk);
-    rain_range<ADV, PULSE, HILO, TI, FAST>(a, lo, hi, acc, logtab, use_win);
+    rain_range<ADV, PULSE, HILO, TI, FAST>(a, lo, hi, acc, itab, use_win);
     if (PULSE) acc_flush_row(acc, a.pk, a.pk.row0 + (int)L);
 }
 
@@ -1771,7 +1786,7 @@ struct FixedK {
     Tex3 wind, cpxx;
     F16 les;
     F16 sd;                 // sim_desc of the advance
-    const float *logtab;
+    WIdx widx;
     PulseK pk;
 };
 
@@ -1789,7 +1804,7 @@ __device__ __forceinline__ void fixed_step(const F16 &sd, float4 p, float4 uvwt,
 }
 
 template <bool ADV, bool PULSE, bool WIN, bool TRI>
-__device__ __forceinline__ void fixed_range(const FixedK &a, size_t lo, size_t hi, double *acc, const float4 *logtab) {
+__device__ __forceinline__ void fixed_range(const FixedK &a, size_t lo, size_t hi, double *acc, const float4 *itab) {
     const float2 *wl = win_weights(acc, a.pk);
     const float *grl = win_gates(acc, a.pk);
     WaveWin win;
@@ -1804,7 +1819,7 @@ __device__ __forceinline__ void fixed_range(const FixedK &a, size_t lo, size_t h
             rcs = a.rcs[i];
         }
         if (ADV && live) {
-            const float4 c = wind_index(p, a.les, a.sd, logtab);
+            const float4 c = wind_index(p, a.les, a.sd, itab, a.widx);
             const float4 uvwt = sample3d(a.wind, c.x, c.y, c.z);
             const float4 cp = sample3d(a.cpxx, c.x, c.y, c.z);
             fixed_step(a.sd, p, uvwt, cp, rcs);
@@ -1829,17 +1844,17 @@ __device__ __forceinline__ void fixed_range(const FixedK &a, size_t lo, size_t h
 template <bool ADV, bool PULSE>
 __global__ __launch_bounds__(256) void k_fixed(FixedK a) {
     extern __shared__ double acc[];
-    __shared__ float4 logtab[RF_LOG_TABLE];
-    if (ADV) logtab_load(logtab, a.logtab);
+    __shared__ float4 itab[RSD_ITAB_ROWS];
+    if (ADV) itab_stage(itab, a.widx.t);
     if (PULSE) acc_clear(acc, 4 * a.pk.R);
     else __syncthreads();
     size_t lo, hi;
     const unsigned L = block_range_at(0, a.n, blockIdx.x, gridDim.x, lo, hi, true);
     const bool use_win = PULSE && a.pk.fold == RSD_FOLD_WINDOW;
     if (use_win) stage_window(acc, a.pk);
-    if (use_win && a.pk.tri) fixed_range<ADV, PULSE, true, true>(a, lo, hi, acc, logtab);
-    else if (use_win) fixed_range<ADV, PULSE, true, false>(a, lo, hi, acc, logtab);
-    else fixed_range<ADV, PULSE, false, false>(a, lo, hi, acc, logtab);
+    if (use_win && a.pk.tri) fixed_range<ADV, PULSE, true, true>(a, lo, hi, acc, itab);
+    else if (use_win) fixed_range<ADV, PULSE, true, false>(a, lo, hi, acc, itab);
+    else fixed_range<ADV, PULSE, false, false>(a, lo, hi, acc, itab);
     if (PULSE) acc_flush_row(acc, a.pk, a.pk.row0 + (int)L);
 }
 
@@ -1925,17 +1940,12 @@ __global__ __launch_bounds__(256) void k_io(const float4 *__restrict__ in, float
 }
 
 // Device evaluation of the binary32 functions the kernels use (self-test).
-__global__ void k_math_probe(int fn, const float *a, const float *b, float *out, unsigned long long n, const float *logtab_g) {
-    __shared__ float4 logtab[RF_LOG_TABLE];
-    rf_log_rows(logtab, logtab_g);
-    __syncthreads();
+__global__ void k_math_probe(int fn, const float *a, const float *b, float *out, unsigned long long n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float x = a[i], y = b[i], s, c;
     float r = 0.0f;
     switch (fn) {
-        case 0: r = rf_log1pf(x, logtab); break;
-        case 12: r = rf_log1p_pos(x, logtab); break;
         case 10: r = rf_acos_fast(x); break;
         case 1: r = rf_logf(x); break;
         case 2: rf_sincosf(x, &s, &c); r = s; break;
@@ -2284,7 +2294,15 @@ struct rsd_engine {
     int rain_occ_hilo;       // variant and LDS size rain_per_cu was computed for
     size_t rain_occ_lds;
     hipEvent_t ev0, ev1;
-    float *logtab;           // 3 x 128 floats of the wind-index log1p
+    // Stretched-grid index tables per wind slot, built by
+    // rsd_wind_index_table for the slot's LES descriptor and the step's box
+    // (rebuilt when either changes) and copied stream-ordered from pinned
+    // staging.
+    float4 *itab_d[2], *itab_h[2];
+    hipEvent_t ev_itab[2];
+    float itab_key[2][22];
+    int itab_ok[2];
+    WIdx itab_w[2];
     int prof_on, prof_n;
     int prof_stride, prof_tick;  // time every prof_stride-th rain launch
     hipEvent_t *prof_ev;     // start/stop pairs around the rain kernel
@@ -2350,26 +2368,45 @@ static void dfree(void *p) {
     if (p) (void)hipFree(p);
 }
 
-// Table of the division-free log1p (rsd_fmath.h): g_j = fl(1 / c_j),
-// c_j = 1 + (j + 1/2) / 128, and -log(g_j) split into binary32 head and tail.
-static void make_logtab(float *t) {
-    for (int j = 0; j < RF_LOG_TABLE; j++) {
-        double c = 1.0 + ((double)j + 0.5) / (double)RF_LOG_TABLE;
-        float g = (float)(1.0 / c);
-        double L = -log((double)g);
-        float hi = (float)L;
-        t[j] = g;
-        t[RF_LOG_TABLE + j] = hi;
-        t[2 * RF_LOG_TABLE + j] = (float)(L - (double)hi);
+// Chebyshev nodes of [0, 1]: 0.5 + 0.5 cos((2k + 1) pi / 8), k = 0..3.
+static const double itab_cos1 = 0x1.d906bcf328d46p-1, itab_cos3 = 0x1.87de2a6aea964p-2;
+
+// Pieces of one axis: P(x) = s0 log1p(s4 x) on x = (j + f) / ih, interpolated
+// at the Chebyshev nodes u0..u3 of f in binary64 (Newton divided differences,
+// expanded into monomials of f), coefficients rounded to binary32.
+static void itab_axis(double s0, double s4, float ihf, float4 *rows) {
+    const double u0 = 0.5 + 0.5 * itab_cos1, u1 = 0.5 + 0.5 * itab_cos3;
+    const double u2 = 0.5 - 0.5 * itab_cos3, u3 = 0.5 - 0.5 * itab_cos1;
+    const double ih = (double)ihf;
+    for (int j = 0; j < RSD_ITAB_N; j++) {
+        const double y0 = s0 * log1p(s4 * (((double)j + u0) / ih));
+        const double y1 = s0 * log1p(s4 * (((double)j + u1) / ih));
+        const double y2 = s0 * log1p(s4 * (((double)j + u2) / ih));
+        const double y3 = s0 * log1p(s4 * (((double)j + u3) / ih));
+        const double d01 = (y1 - y0) / (u1 - u0), d12 = (y2 - y1) / (u2 - u1), d23 = (y3 - y2) / (u3 - u2);
+        const double d012 = (d12 - d01) / (u2 - u0), d123 = (d23 - d12) / (u3 - u1);
+        const double A = (d123 - d012) / (u3 - u0), B = d012, C = d01;
+        const double c3 = A;
+        const double c2 = B - A * ((u0 + u1) + u2);
+        const double c1 = (C - B * (u0 + u1)) + A * ((u0 * u1 + u0 * u2) + u1 * u2);
+        const double c0 = ((y0 - C * u0) + B * (u0 * u1)) - A * ((u0 * u1) * u2);
+        rows[j] = make_float4((float)c0, (float)c1, (float)c2, (float)c3);
     }
 }
 
-static hipError_t upload_logtab(float **dst) {
-    float h[3 * RF_LOG_TABLE];
-    make_logtab(h);
-    hipError_t r = hipMalloc((void **)dst, sizeof(h));
-    if (r == hipSuccess) r = hipMemcpy(*dst, h, sizeof(h), hipMemcpyHostToDevice);
-    return r;
+extern "C" int rsd_wind_index_table(const float *les, const float *sd, float *rows, float *ih, float *jm) {
+    if (!les || !sd || !rows || !ih || !jm) return RSD_ERR_ARG;
+    // origins as the kernels round them: the box centre in x and y, 0 in z
+    const float o[3] = {sd[8] + 0.5f * sd[12], sd[9] + 0.5f * sd[13], 0.0f};
+    for (int a = 0; a < 3; a++) {
+        const float lo = sd[8 + a], hi = sd[8 + a] + sd[12 + a];
+        double X = fmax(fabs((double)lo - (double)o[a]), fabs((double)hi - (double)o[a]));
+        X = fmax(X * (1.0 + 0x1p-10), 1.0e-3);
+        ih[a] = (float)((double)RSD_ITAB_N / X);
+        jm[a] = (float)(RSD_ITAB_N - 1);
+        itab_axis((double)les[a], (double)les[4 + a], ih[a], (float4 *)rows + a * RSD_ITAB_N);
+    }
+    return RSD_OK;
 }
 
 extern "C" int rsd_device_count(void) {
@@ -2408,7 +2445,12 @@ extern "C" int rsd_open(int device_index, rsd_engine **out) {
         hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess ||
         hipHostMalloc((void **)&e->pulse_h, 4 * RSD_MAX_GATES * sizeof(double), hipHostMallocMapped) != hipSuccess ||
         hipHostGetDevicePointer((void **)&e->pulse_d, e->pulse_h, 0) != hipSuccess ||
-        upload_logtab(&e->logtab) != hipSuccess) {
+        hipMalloc((void **)&e->itab_d[0], RSD_ITAB_ROWS * sizeof(float4)) != hipSuccess ||
+        hipMalloc((void **)&e->itab_d[1], RSD_ITAB_ROWS * sizeof(float4)) != hipSuccess ||
+        hipHostMalloc((void **)&e->itab_h[0], RSD_ITAB_ROWS * sizeof(float4), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&e->itab_h[1], RSD_ITAB_ROWS * sizeof(float4), hipHostMallocDefault) != hipSuccess ||
+        hipMemset(e->itab_d[0], 0, RSD_ITAB_ROWS * sizeof(float4)) != hipSuccess ||
+        hipMemset(e->itab_d[1], 0, RSD_ITAB_ROWS * sizeof(float4)) != hipSuccess) {
         free(e);
         return RSD_ERR_HIP;
     }
@@ -2475,12 +2517,14 @@ extern "C" void rsd_close(rsd_engine *e) {
         if (e->dff_stage[s]) (void)hipHostFree(e->dff_stage[s]);
         if (e->ev_dff[s]) (void)hipEventDestroy(e->ev_dff[s]);
         if (e->ev_wind[s]) (void)hipEventDestroy(e->ev_wind[s]);
+        if (e->ev_itab[s]) (void)hipEventDestroy(e->ev_itab[s]);
+        dfree(e->itab_d[s]);
+        if (e->itab_h[s]) (void)hipHostFree(e->itab_h[s]);
     }
     for (int t = 0; t < RSD_MAX_TABLES; t++) {
         dfree(e->adm_cd[t]); dfree(e->adm_cm[t]); dfree(e->rcs_re[t]); dfree(e->rcs_im[t]);
     }
     dfree(e->drop); dfree(e->dff[0]); dfree(e->dff[1]); dfree(e->rangew); dfree(e->angw); dfree(e->gate_r); dfree(e->partials); dfree(e->partials2);
-    dfree(e->logtab);
     dfree(e->ring);
     dfree(e->mom_d);
     if (e->pulse_h) (void)hipHostFree(e->pulse_h);
@@ -2774,9 +2818,42 @@ static int blend_slot(rsd_engine *e, int slot, Tex3 *t, float alpha) {
     return RSD_OK;
 }
 
+// The index tables of a wind slot for the box of sim_desc sd: rebuilt and
+// copied (stream-ordered) when the slot's LES descriptor or the box changed;
+// *rebuilt tells the caller (a launch on another stream waits for the copy).
+static int wind_itab(rsd_engine *e, int slot, const float *sd, WIdx *w, int *rebuilt) {
+    float key[22];
+    memcpy(key, e->les[slot].s, 16 * sizeof(float));
+    memcpy(key + 16, sd + 8, 3 * sizeof(float));
+    memcpy(key + 19, sd + 12, 3 * sizeof(float));
+    if (rebuilt) *rebuilt = 0;
+    if (!e->itab_ok[slot] || memcmp(key, e->itab_key[slot], sizeof(key)) != 0) {
+        if (!e->ev_itab[slot]) HIPCHK(e, hipEventCreateWithFlags(&e->ev_itab[slot], hipEventDisableTiming));
+        else HIPCHK(e, hipEventSynchronize(e->ev_itab[slot]));     // the staging's last copy is done
+        WIdx &t = e->itab_w[slot];
+        int st = rsd_wind_index_table(e->les[slot].s, sd, (float *)e->itab_h[slot], t.ih, t.jm);
+        if (st) return fail(e, st, "wind index table");
+        t.t = e->itab_d[slot];
+        HIPCHK(e, hipMemcpyAsync(e->itab_d[slot], e->itab_h[slot], RSD_ITAB_ROWS * sizeof(float4), hipMemcpyHostToDevice,
+                                 e->stream));
+        HIPCHK(e, hipEventRecord(e->ev_itab[slot], e->stream));
+        memcpy(e->itab_key[slot], key, sizeof(key));
+        e->itab_ok[slot] = 1;
+        if (rebuilt) *rebuilt = 1;
+    }
+    *w = e->itab_w[slot];
+    w->t = e->itab_d[slot];
+    return RSD_OK;
+}
+
 static int fill_rain(rsd_engine *e, const rsd_step *s, RainK &a) {
     int slot = s ? s->wind_slot : 0;
     if (s && (!e->wind_ok[slot])) return fail(e, RSD_ERR_STATE, "wind slot %d not set", slot);
+    a.widx.t = e->itab_d[slot];     // read only by advances
+    if (s) {
+        int st = wind_itab(e, slot, s->sim_desc, &a.widx, NULL);
+        if (st) return st;
+    }
     if (!e->drop) return fail(e, RSD_ERR_STATE, "drop RCS table not set");
     a.pos = e->pos; a.vel = e->vel; a.rcs = e->rcs; a.rnd = e->rnd;
     a.n = e->n_rain;
@@ -2790,7 +2867,6 @@ static int fill_rain(rsd_engine *e, const rsd_step *s, RainK &a) {
     a.drop = e->drop;
     a.drop_desc = e->drop_desc;
     a.dragged = s ? (s->background_kind == RSD_ADVANCE_DRAGGED) : 1u;
-    a.logtab = e->logtab;
     a.hx = a.sd.s[8] + a.sd.s[12];
     a.hy = a.sd.s[9] + a.sd.s[13];
     a.hz = a.sd.s[10] + a.sd.s[14];
@@ -2800,15 +2876,14 @@ static int fill_rain(rsd_engine *e, const rsd_step *s, RainK &a) {
 }
 
 // The fused advance + pulse launches that the specialised rain kernel (k_rain
-// FAST) covers: dragged rain, a stretched wind table with non-negative growth
-// factors, drop and angular tables that fit in LDS and a binary64 phase (or
+// FAST) covers: dragged rain, a stretched wind table, drop and angular tables that fit in LDS and a binary64 phase (or
 // hi/lo positions).
 static bool rain_fast(const rsd_engine *e, const rsd_step *s, const rsd_pulse *p) {
     if (!s || !p || s->background_kind != RSD_ADVANCE_DRAGGED) return false;
     const F16 &l = e->les[s->wind_slot];
     uint32_t spacing;
     memcpy(&spacing, &l.s[7], sizeof(spacing));
-    if (spacing != 7u || !(l.s[4] >= 0.0f && l.s[5] >= 0.0f && l.s[6] >= 0.0f)) return false;
+    if (spacing != 7u) return false;
     const int dn = (int)e->drop_desc.s[2] + 1, an = (int)e->angw_desc.s[2] + 1;
     if (dn < 1 || dn > RSD_LDS_DROP_MAX || an < 1 || an > RSD_LDS_ANGW_MAX) return false;
     if (!e->plo && (unsigned long long)e->n_rain * sizeof(float4) > RSD_BUF_MAX_BYTES) return false;
@@ -2832,7 +2907,7 @@ static int fill_debris(rsd_engine *e, const rsd_step *s, const rsd_pulse *p, int
     a.pos = e->pos; a.ori = e->ori; a.vel = e->vel; a.tum = e->tum; a.rcs = e->rcs; a.rnd = e->rnd;
     a.begin = e->origins[type];
     a.end = e->origins[type] + e->counts[type];
-    a.logtab = e->logtab;
+    a.widx.t = e->itab_d[s ? s->wind_slot : 0];
     a.dff = e->dff[s ? s->wind_slot : 0];
     a.dff_desc = e->dff_desc;
     if (s) {
@@ -2847,6 +2922,8 @@ static int fill_debris(rsd_engine *e, const rsd_step *s, const rsd_pulse *p, int
         }
         a.les = e->les[slot];
         memcpy(a.sd.s, s->sim_desc, sizeof(a.sd.s));
+        int st = wind_itab(e, slot, s->sim_desc, &a.widx, NULL);
+        if (st) return st;
         a.adm_cd.d = e->adm_cd[ai]; a.adm_cm.d = e->adm_cm[ai];
         a.adm_cd.nx = a.adm_cm.nx = e->adm_nx[ai]; a.adm_cd.ny = a.adm_cm.ny = e->adm_ny[ai];
         a.adm_desc = e->adm_desc[ai];
@@ -2901,6 +2978,13 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p, con
     const int part_buf = defer ? e->part_cur : (e->batch ? 1 : 0);
     if (!defer) {
         int st = flush_finalize(e);
+        if (st) return st;
+    }
+    if (s && e->wind_ok[s->wind_slot]) {
+        // the index tables of the step's wind slot, copied (if rebuilt) on the
+        // main stream before the side stream forks from it
+        WIdx w;
+        int st = wind_itab(e, s->wind_slot, s->sim_desc, &w, NULL);
         if (st) return st;
     }
     // Rows of the partial-sum buffer: one per block of every launch, rain
@@ -3030,8 +3114,12 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p, con
             a.cpxx.alpha = s->wind_alpha;
         }
         a.les = e->les[slot];
-        if (s) memcpy(a.sd.s, s->sim_desc, sizeof(a.sd.s));
-        a.logtab = e->logtab;
+        a.widx.t = e->itab_d[slot];
+        if (s) {
+            memcpy(a.sd.s, s->sim_desc, sizeof(a.sd.s));
+            int st = wind_itab(e, slot, s->sim_desc, &a.widx, NULL);
+            if (st) return st;
+        }
         a.pk = pk;
         a.pk.row0 = 0;
         const dim3 grid(rain_blocks);
@@ -3805,23 +3893,18 @@ extern "C" int rsd_math_probe(int device, int fn, const float *a, const float *b
     if (fn == 11 && n % 64 != 0) return RSD_ERR_ARG;   // whole waves only
     if (device < 0 || device >= rsd_device_count()) return RSD_ERR_NO_DEVICE;
     if (hipSetDevice(device) != hipSuccess) return RSD_ERR_HIP;
-    float *d = NULL, *lt = NULL;
-    if (upload_logtab(&lt) != hipSuccess) return RSD_ERR_HIP;
-    if (hipMalloc((void **)&d, 3 * n * sizeof(float)) != hipSuccess) {
-        (void)hipFree(lt);
-        return RSD_ERR_ALLOC;
-    }
+    float *d = NULL;
+    if (hipMalloc((void **)&d, 3 * n * sizeof(float)) != hipSuccess) return RSD_ERR_ALLOC;
     int rc = RSD_OK;
     if (hipMemcpy(d, a, n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(d + n, b, n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) rc = RSD_ERR_HIP;
     if (rc == RSD_OK) {
         hipLaunchKernelGGL(k_math_probe, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, fn, (const float *)d,
-                           (const float *)(d + n), d + 2 * n, (unsigned long long)n, (const float *)lt);
+                           (const float *)(d + n), d + 2 * n, (unsigned long long)n);
         if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = RSD_ERR_HIP;
     }
     if (rc == RSD_OK && hipMemcpy(out, d + 2 * n, n * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) rc = RSD_ERR_HIP;
     (void)hipFree(d);
-    (void)hipFree(lt);
     return rc;
 }
 
@@ -3865,7 +3948,11 @@ extern "C" int rsd_profile(rsd_engine *e, int enable) {
         e->prof_ev = (hipEvent_t *)calloc(2 * RSD_PROF_CAP, sizeof(hipEvent_t));
         e->prof_kind = (int *)calloc(RSD_PROF_CAP, sizeof(int));
         if (!e->prof_ev || !e->prof_kind) return fail(e, RSD_ERR_ALLOC, "profile events");
-        for (int i = 0; i < 2 * RSD_PROF_CAP; i++) HIPCHK(e, hipEventCreate(&e->prof_ev[i]));
+        // timing only: no system-scope release when the dispatch completes (that
+        // release writes the L2's dirty lines back and would lengthen the timed
+        // launch beyond an untimed one)
+        for (int i = 0; i < 2 * RSD_PROF_CAP; i++)
+            HIPCHK(e, hipEventCreateWithFlags(&e->prof_ev[i], hipEventDisableSystemFence));
     }
     if (enable < 0) return fail(e, RSD_ERR_ARG, "profile stride %d", enable);
     e->prof_on = enable ? 1 : 0;
@@ -4217,6 +4304,16 @@ extern "C" int rsd_batch_launch(rsd_batch *b) {
         if (fast && !(rain_fast(e, &e->pending, p) && !e->plo)) fast = false;
         int st = ensure_partials(e, (size_t)rows, R);
         if (st) return bfail(b, st, e->err);
+        if (any_adv && e->wind_ok[e->pending.wind_slot]) {
+            // index tables copied on the engine's stream: wait for a rebuilt
+            // one here, the batch runs on its own stream
+            WIdx w;
+            int rebuilt = 0;
+            st = wind_itab(e, e->pending.wind_slot, e->pending.sim_desc, &w, &rebuilt);
+            if (st) return bfail(b, st, e->err);
+            if (rebuilt && hipEventSynchronize(e->ev_itab[e->pending.wind_slot]) != hipSuccess)
+                return bfail(b, RSD_ERR_HIP, "index table copy");
+        }
         RainK a;
         memset(&a, 0, sizeof(a));
         st = fill_rain(e, any_adv ? &e->pending : NULL, a);
@@ -4249,27 +4346,25 @@ extern "C" int rsd_batch_launch(rsd_batch *b) {
         if (l > lds) lds = l;
     }
     first[ns] = total4r;
-    const float *logtab = sel[0]->logtab;
     const RainK *rk_d = (const RainK *)b->desc_d;
     const FinK *fk_d = (const FinK *)(b->desc_d + rk_bytes);
     const int *first_d = (const int *)(b->desc_d + rk_bytes + fk_bytes);
     const bool timed = b->tuning;
     // everything the graph depends on besides descriptor contents
     const size_t gsig = (((((size_t)n_items * 131u + (size_t)grid) * 131u + lds) * 131u + (size_t)chunk) * 131u +
-                         (size_t)total4r) * 8u + (size_t)fast * 4u + (size_t)any_adv * 2u + 1u + (desc_bytes << 20) +
-                        (size_t)(uintptr_t)logtab * 7u;
+                         (size_t)total4r) * 8u + (size_t)fast * 4u + (size_t)any_adv * 2u + 1u + (desc_bytes << 20);
     auto enqueue = [&](hipStream_t st) -> hipError_t {
         hipError_t r = hipMemcpyAsync(b->desc_d, b->desc_h[buf], desc_bytes, hipMemcpyHostToDevice, st);
         if (r != hipSuccess) return r;
         if (fast)
             hipLaunchKernelGGL((k_rain_batch<true, true, true>), dim3(grid), dim3(256), lds, st, rk_d,
-                               (const int2 *)b->items_d, n_items, chunk, logtab);
+                               (const int2 *)b->items_d, n_items, chunk);
         else if (any_adv)
             hipLaunchKernelGGL((k_rain_batch<true, true>), dim3(grid), dim3(256), lds, st, rk_d,
-                               (const int2 *)b->items_d, n_items, chunk, logtab);
+                               (const int2 *)b->items_d, n_items, chunk);
         else
             hipLaunchKernelGGL((k_rain_batch<false, true>), dim3(grid), dim3(256), lds, st, rk_d,
-                               (const int2 *)b->items_d, n_items, chunk, logtab);
+                               (const int2 *)b->items_d, n_items, chunk);
         r = hipGetLastError();
         if (r != hipSuccess) return r;
         hipLaunchKernelGGL(k_finalize_batch, dim3(total4r), dim3(256), 0, st, fk_d, first_d, ns);

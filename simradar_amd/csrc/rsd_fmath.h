@@ -7,10 +7,10 @@
 // rint, in a fixed order (the translation unit is compiled with
 // -ffp-contract=off; every fused multiply-add is an explicit __builtin_fmaf):
 //
-//   log / log1p : u = m 2^k with m in [sqrt(1/2), sqrt(2)); s = f / (2 + f);
+//   log         : u = m 2^k with m in [sqrt(1/2), sqrt(2)); s = f / (2 + f);
 //                 log(1 + f) = f - (hfsq - s (hfsq + R)), R = z (2/3 + z (2/5 +
 //                 z (2/7 + z 2/9))), z = s^2; ln2 split into a 16-bit head and a
-//                 tail; log1p adds (x - (u - 1)) / u for the rounding of u = 1 + x.
+//                 tail.
 //   sincos      : three-part Cody-Waite reduction by pi/2, Taylor polynomials of
 //                 sin (to r^9) and cos (to r^10) on |r| <= pi/4.
 //   atan2       : a = min/max, a > tan(pi/8) mapped by (a - 1)/(a + 1) + pi/4,
@@ -74,78 +74,6 @@ __device__ __forceinline__ float rf_logf(float x) {
     float m = rf_split(x, &k);
     float r = rf_log_kernel(m - 1.0f, k, 0.0f);
     return (scale != 0.0f) ? r - scale * 0x1.62e430p-1f : r;
-}
-
-// log1p for the wind index, without division: u = 1 + x = 2^k m, m in [1, 2);
-// the top 7 mantissa bits select g_j = fl(1 / c_j), c_j = 1 + (j + 1/2) / 128,
-// and L_j = -log(g_j) as binary32 head + tail (built on the host with the C
-// library's binary64 log; passed as rows t4[j] = (g_j, head_j, tail_j, 0), one
-// 16-byte read per evaluation; rf_log_rows builds them);
-// r = fma(m, g_j, -1), log(m) = L_j + r + r^2 (-1/2 + r (1/3 - r/4)); the
-// rounding of u adds (x - (u - 1)) g_j 2^-k.  Arguments below 2^-7 use the
-// series directly; negative ones (not on the wind-index path) the divided form.
-#define RF_LOG_TABLE 128
-__device__ __forceinline__ float rf_log1pf(float x, const float4 *t4) {
-    if (x == 0.0f) return x;
-    if (!(x > -1.0f)) return (x == -1.0f) ? -__builtin_inff() : __builtin_nanf("");
-    if (__builtin_isinf(x)) return x;
-    if (x < 0.0f) {
-        float u = 1.0f + x;
-        int k;
-        float m = rf_split(u, &k);
-        return rf_log_kernel(m - 1.0f, k, rf_div(x - (u - 1.0f), u));
-    }
-    if (x < 0x1p-7f) return rf_fma(x * x, rf_fma(x, rf_fma(x, -0.25f, 0x1.555556p-2f), -0.5f), x);
-    float u = 1.0f + x;
-    float tail = (u < 0x1p+24f) ? x - (u - 1.0f) : 0.0f;
-    uint32_t iu = __float_as_uint(u);
-    int k = (int)((iu >> 23) & 0xffu) - 127;
-    int j = (int)((iu >> 16) & 0x7fu);
-    float m = __uint_as_float((iu & 0x007fffffu) | 0x3f800000u);
-    const float4 row = t4[j];
-    float g = row.x;
-    float r = rf_fma(m, g, -1.0f);
-    float p = rf_fma(r * r, rf_fma(r, rf_fma(r, -0.25f, 0x1.555556p-2f), -0.5f), r);
-    float kf = (float)k;
-    float scale = __uint_as_float((uint32_t)(127 - k) << 23);
-    float lo = rf_fma(kf, RF_LN2_LO, row.z) + (tail * g) * scale;
-    float hi = rf_fma(kf, RF_LN2_HI, row.y);
-    return hi + (p + lo);
-}
-
-// rf_log1pf for finite x >= 0 (the stretched wind index): the same series
-// below 2^-7 and the same table form above, without the branches for the
-// cases such an argument cannot take (equal results there: the series returns
-// +0 for +0).  Both forms are evaluated and one selected (the table row read
-// is in range for every such x), so the three axes of an index interleave
-// instead of each waiting on its own table read inside a branch.  The row's
-// fourth word (+0) is added to the series, which leaves it unchanged for x >=
-// 0 and gives the row read a use on both sides of the select, so the compiler
-// keeps the read out of a branch.
-__device__ __forceinline__ float rf_log1p_pos(float x, const float4 *t4) {
-    float u = 1.0f + x;
-    float tail = (u < 0x1p+24f) ? x - (u - 1.0f) : 0.0f;
-    uint32_t iu = __float_as_uint(u);
-    int k = (int)((iu >> 23) & 0xffu) - 127;
-    int j = (int)((iu >> 16) & 0x7fu);
-    float m = __uint_as_float((iu & 0x007fffffu) | 0x3f800000u);
-    const float4 row = t4[j];
-    float g = row.x;
-    float r = rf_fma(m, g, -1.0f);
-    float p = rf_fma(r * r, rf_fma(r, rf_fma(r, -0.25f, 0x1.555556p-2f), -0.5f), r);
-    float kf = (float)k;
-    float scale = __uint_as_float((uint32_t)(127 - k) << 23);
-    float lo = rf_fma(kf, RF_LN2_LO, row.z) + (tail * g) * scale;
-    float hi = rf_fma(kf, RF_LN2_HI, row.y);
-    const float series = rf_fma(x * x, rf_fma(x, rf_fma(x, -0.25f, 0x1.555556p-2f), -0.5f), x) + row.w;
-    return x < 0x1p-7f ? series : hi + (p + lo);
-}
-
-// Rows (g_j, head_j, tail_j, 0) of the log1p table for rf_log1p_pos, copied by
-// the threads of a block from the three arrays g[128], head[128], tail[128].
-__device__ __forceinline__ void rf_log_rows(float4 *dst, const float *tab) {
-    for (int j = threadIdx.x; j < RF_LOG_TABLE; j += blockDim.x)
-        dst[j] = make_float4(tab[j], tab[RF_LOG_TABLE + j], tab[2 * RF_LOG_TABLE + j], 0.0f);
 }
 
 // Faster forms for quantities that only enter the pulse sums (checked against

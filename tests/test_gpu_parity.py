@@ -21,7 +21,6 @@ def _bits(a):
 
 
 @pytest.mark.parametrize("fn,gen", [
-    (0, lambda g: np.concatenate([g.uniform(0, 1, 50000), np.exp(g.uniform(-20, 12, 50000))])),
     (1, lambda g: np.exp(g.uniform(-21, 0, 100000))),
     (2, lambda g: g.uniform(-20, 20, 100000)),
     (3, lambda g: g.uniform(-20, 20, 100000)),
@@ -35,7 +34,7 @@ def test_device_math_matches_oracle(gpu_ready, fn, gen):
     if fn == 6:
         ref = np.sqrt(a)
     else:
-        ref = orc.math(["log1p", "log", "sin", "cos", None, "acos"][fn], a)
+        ref = orc.math({1: "log", 2: "sin", 3: "cos", 5: "acos"}[fn], a)
     bad = int(np.sum(_bits(dev) != _bits(ref)))
     assert bad == 0, "device function %d differs from the oracle at %d of %d inputs" % (fn, bad, a.size)
 
@@ -425,19 +424,6 @@ def test_les_frame_switch_is_stream_ordered(gpu_ready):
     for s in ("pos", "vel", "ori", "tum"):
         assert np.array_equal(_bits(runs[0][s]), _bits(runs[1][s])), s
     assert np.array_equal(runs[0]["iq"], runs[1]["iq"])
-
-
-def test_wind_index_log1p_matches_general_form(gpu_ready):
-    """The wind index's log1p for arguments >= 0 gives the oracle's log1p bit
-    for bit (zero, the series range below 2^-7 and its boundary, the table
-    range up to large arguments)."""
-    g = np.random.default_rng(12)
-    x = np.concatenate([[0.0, 2.0 ** -7, np.nextafter(2.0 ** -7, 0), 1.0, 2.0 ** 24],
-                        g.uniform(0, 2.0 ** -7, 50000), np.exp(g.uniform(-12, 14, 100000))]).astype(np.float32)
-    dev = sa.math_probe(12, x)
-    ref = orc.math("log1p", x)
-    bad = int(np.sum(_bits(dev) != _bits(ref)))
-    assert bad == 0, "differs at %d of %d inputs" % (bad, x.size)
 
 
 def test_file_tables_parity(gpu_ready, tmp_path, monkeypatch):

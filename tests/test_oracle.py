@@ -21,8 +21,6 @@ def ulps(got, ref):
 # ---------------------------------------------------------------- elementary functions
 
 @pytest.mark.parametrize("fn,gen,ref,bound", [
-    ("log1p", lambda g: np.concatenate([g.uniform(0, 1, 100000), np.exp(g.uniform(-24, 14, 100000)),
-                                        -g.uniform(0, 0.99, 20000)]), np.log1p, 1.5),
     ("log", lambda g: np.exp(g.uniform(-22, 8, 200000)), np.log, 1.0),
     ("sin", lambda g: g.uniform(-40, 40, 200000), np.sin, 2.0),
     ("cos", lambda g: g.uniform(-40, 40, 200000), np.cos, 2.0),
@@ -101,11 +99,13 @@ def test_trilinear_clamps_to_edge():
 
 
 def test_stretched_wind_index_maps_grid_nodes_to_texels():
-    # |x| = a (r^m - 1) / (r - 1) maps to coordinate (n - 1)/2 + m (rs.c:3058-3061)
+    # |x| = a (r^m - 1) / (r - 1) maps to coordinate (n - 1)/2 + m (rs.c:3058-3061);
+    # the box holds every node (the index tables cover the box)
     d = orc.les_desc(True, 176, 176, 80, 2.0, 2.0, 2.0, 1.0212, 1.0212, 1.05, 50.0)
     sd = np.zeros(16, np.float32)
-    sd[8], sd[9], sd[12], sd[13] = -100.0, 1000.0, 200.0, 300.0
-    cx, cy = -100.0 + 100.0, 1000.0 + 150.0
+    sd[8], sd[9], sd[12], sd[13] = -500.0, 600.0, 1000.0, 1100.0
+    sd[10], sd[14] = 0.0, 2600.0
+    cx, cy = -500.0 + 500.0, 600.0 + 550.0
     for m in (0, 1, 5, 40, 85):
         x = 2.0 * (1.0212 ** m - 1.0) / 0.0212
         z = 2.0 * (1.05 ** m - 1.0) / 0.05
