@@ -312,6 +312,14 @@ const char *rsd_mlp_last_error(const rsd_mlp *m);
  * network's 12 targets read from ADM table adm_table and RCS table rcs_table
  * exactly as the debris kernels read them; x[n][7] = (R, u), t[n][12]. */
 int rsd_debris_samples(rsd_engine *e, int adm_table, int rcs_table, size_t n, uint64_t seed, float *x, float *t);
+/* Training samples at the population's current states: for the first
+ * min(count, cap) bodies of debris type `type`, x[n][7] = the inputs the debris
+ * kernel forms (R from position and orientation, u from velocity and the wind
+ * of wind_slot at the position, for the box of sim_desc16) and t[n][12] = the
+ * targets read from ADM table adm_table and RCS table rcs_table there; *n_out =
+ * the rows written.  No counterpart in the reference. */
+int rsd_debris_state_samples(rsd_engine *e, int type, int adm_table, int rcs_table, int wind_slot,
+                             const float *sim_desc16, size_t cap, float *x, float *t, size_t *n_out);
 /* Use network m (NULL: the tables) for debris type `type` (>= 1) of engine e:
  * the debris kernel evaluates it once per scatterer and pulse in place of the
  * ADM and RCS lookups.  m must stay alive while it is in use. */

@@ -149,6 +149,11 @@ static inline float of_acosf(float x) {
     return of_atan2f(s, x);
 }
 
+/* acos of a cosine by construction (unit vector or quaternion component, a
+ * coordinate over the norm bounding it): arguments rounded beyond [-1, 1] are
+ * taken as +-1 (the reference's acos would give NaN there); NaN stays NaN. */
+static inline float of_acos_unit(float x) { return of_acosf(x > 1.0f ? 1.0f : (x < -1.0f ? -1.0f : x)); }
+
 /* OpenCL-style helpers with a fixed evaluation order. */
 static inline float of_clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
 static inline float of_mixf(float x, float y, float a) { return x + (y - x) * a; }

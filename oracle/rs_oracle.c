@@ -242,7 +242,7 @@ void orc_quat2abg(of4 q, float *alpha, float *beta, float *gamma) {
     if (beta_arg > 0.999847f || beta_arg < -0.999847f) {
         *alpha = 0.0f;
         *beta = 0.0f;
-        *gamma = (of_signf(q.z) * of_acosf(q.w)) * 2.0f;
+        *gamma = (of_signf(q.z) * of_acos_unit(q.w)) * 2.0f;
     } else {
         *alpha = of_atan2f(q.y * q.z - q.w * q.x, q.x * q.z + q.w * q.y);
         *beta = of_acosf(((q.w * q.w + q.z * q.z) - q.y * q.y) - q.x * q.x);
@@ -551,7 +551,7 @@ void orc_db_atts(of4 *p, of4 *o, of4 *v, of4 *t, of4 *x, ou4 *y, size_t begin, s
             float sqxy = sqrtf(c0 * c0 + c2 * c2);
             float sqxyz = sqrtf((c0 * c0 + c1 * c1) + c2 * c2);
             float sa, ca;
-            of_sincosf(0.5f * of_acosf(c1 / sqxyz), &sa, &ca);
+            of_sincosf(0.5f * of_acos_unit(c1 / sqxyz), &sa, &ca);
             ori.x = (((c0 * ca) * sin_th_2) / sqxyz - ((((c0 * c1) * sin_th_2) / sqxyz) * sa) / sqxy)
                     + ((c2 * cos_th_2) * sa) / sqxy;
             ori.y = (sin_th_2 / sqxyz) * (sqxy * sa + c1 * ca);
@@ -575,7 +575,7 @@ void orc_db_atts(of4 *p, of4 *o, of4 *v, of4 *t, of4 *x, ou4 *y, size_t begin, s
         /* rs.cl:195-261 compute_dudt_dwdt */
         of4 ur = {bg.x - vel.x, bg.y - vel.y, bg.z - vel.z, bg.w - vel.w};
         of4 uh = qrot(norm4(ur), qconj(ori));
-        float beta = of_acosf(uh.x);
+        float beta = of_acos_unit(uh.x);
         float alpha = of_atan2f(uh.z, uh.y);
         if (alpha < 0.0f) {
             alpha = ORC_PI_F + alpha;

@@ -91,7 +91,7 @@ EXPORTED_RSD = [
     "rsd_batch_timer_start", "rsd_batch_timer_stop", "rsd_batch_last_error",
     "rsd_mlp_create", "rsd_mlp_destroy", "rsd_mlp_set_params", "rsd_mlp_get_params", "rsd_mlp_set_output_scaling",
     "rsd_mlp_forward", "rsd_mlp_gradient", "rsd_mlp_set_data", "rsd_mlp_train", "rsd_mlp_sync", "rsd_mlp_last_error",
-    "rsd_debris_samples", "rsd_set_debris_surrogate", "rsd_wind_index_table",
+    "rsd_debris_samples", "rsd_debris_state_samples", "rsd_set_debris_surrogate", "rsd_wind_index_table",
 ]
 
 
@@ -287,6 +287,7 @@ def _declare(lib):
         "rsd_mlp_sync": (I, [P]),
         "rsd_mlp_last_error": (ctypes.c_char_p, [P]),
         "rsd_debris_samples": (I, [P, I, I, S, ctypes.c_uint64, P, P]),
+        "rsd_debris_state_samples": (I, [P, I, I, I, I, P, S, P, P, P]),
         "rsd_set_debris_surrogate": (I, [P, I, P]),
         "rsd_math_probe": (I, [I, I, P, P, P, S]),
         "rsd_wind_index_table": (I, [P, P, P, P, P]),

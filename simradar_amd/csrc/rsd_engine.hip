@@ -296,7 +296,7 @@ __device__ __forceinline__ float4 debris_rcs(float4 pos, float4 ori, const Tex2 
     if (beta_arg > 0.999847f || beta_arg < -0.999847f) {
         alpha = 0.0f;
         beta = 0.0f;
-        gamma = (rf_sign(q.z) * rf_acosf(q.w)) * 2.0f;
+        gamma = (rf_sign(q.z) * rf_acos_unit(q.w)) * 2.0f;
     } else {
         alpha = rf_atan2f(q.y * q.z - q.w * q.x, q.x * q.z + q.w * q.y);
         beta = rf_acosf(((q.w * q.w + q.z * q.z) - q.y * q.y) - q.x * q.x);
@@ -346,7 +346,7 @@ __device__ __forceinline__ float4 radar_rel_quat(float4 pos, float4 ori) {
 // amplitudes hh, vv, hv (re, im) at the radar-relative orientation R.
 __device__ __forceinline__ void table_targets(float4 R, float4 uh, const Tex2 &cdt, const Tex2 &cmt, const F16 &ad,
                                               const Tex2 &rr, const Tex2 &ri, const F16 &d, float t[12]) {
-    float beta = rf_acosf(uh.x);
+    float beta = rf_acos_unit(uh.x);
     float alpha = rf_atan2f(uh.z, uh.y);
     if (alpha < 0.0f) {
         alpha = RSD_PI_F + alpha;
@@ -360,7 +360,7 @@ __device__ __forceinline__ void table_targets(float4 R, float4 uh, const Tex2 &c
     if (beta_arg > 0.999847f || beta_arg < -0.999847f) {
         a = 0.0f;
         b = 0.0f;
-        g = (rf_sign(q.z) * rf_acosf(q.w)) * 2.0f;
+        g = (rf_sign(q.z) * rf_acos_unit(q.w)) * 2.0f;
     } else {
         a = rf_atan2f(q.y * q.z - q.w * q.x, q.x * q.z + q.w * q.y);
         b = rf_acosf(beta_arg);
@@ -1500,7 +1500,7 @@ __device__ __forceinline__ bool debris_move(const DebrisK &a, size_t i, float4 &
         float sqxy = rf_sqrt(c0 * c0 + c2 * c2);
         float sqxyz = rf_sqrt((c0 * c0 + c1 * c1) + c2 * c2);
         float sa, ca;
-        rf_sincosf(0.5f * rf_acosf(rf_div(c1, sqxyz)), &sa, &ca);
+        rf_sincosf(0.5f * rf_acos_unit(rf_div(c1, sqxyz)), &sa, &ca);
         ori.x = (rf_div((c0 * ca) * sin_th_2, sqxyz) - rf_div(rf_div((c0 * c1) * sin_th_2, sqxyz) * sa, sqxy))
                 + rf_div((c2 * cos_th_2) * sa, sqxy);
         ori.y = rf_div(sin_th_2, sqxyz) * (sqxy * sa + c1 * ca);
@@ -1561,7 +1561,7 @@ __device__ __forceinline__ bool debris_advance(const DebrisK &a, size_t i, float
     float4 bg = sample3d(a.wind, c.x, c.y, c.z);
     float4 ur = make_float4(bg.x - vel.x, bg.y - vel.y, bg.z - vel.z, bg.w - vel.w);
     float4 uh = qrot(norm4(ur), qconj(ori));
-    float beta = rf_acosf(uh.x);
+    float beta = rf_acos_unit(uh.x);
     float alpha = rf_atan2f(uh.z, uh.y);
     if (alpha < 0.0f) {
         alpha = RSD_PI_F + alpha;
@@ -1627,7 +1627,7 @@ __device__ __forceinline__ void debris_range(const DebrisK &a, size_t lo, size_t
                 for (int k = 4; k < 7; k++) f[k] = f[k] == f[k] ? f[k] : 0.0f;
             }
             float out[MLP_OUT];
-            mlp_wave_eval(nw, nb, f, nscr + (threadIdx.x >> 6) * 64 * MLP_N4, out);
+            mlp_wave_eval<MlpPadded>(nw, nb, f, nscr + (threadIdx.x >> 6) * 64 * MLP_SCR, out);
             if (ADV && live) {
                 if (need_adv) {
                     debris_forces(a, o, v, t, bg, ur, make_float4(out[0], out[1], out[2], 0.0f),
@@ -1704,13 +1704,13 @@ __device__ __forceinline__ int step_type(const StepDebris &d) {
 template <bool ADV, bool PULSE, bool SURR>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_debris(StepDebris d) {
     extern __shared__ double acc[];
-    __shared__ uint16_t nw[SURR ? MLP_WN : 2];
-    __shared__ float nb[SURR ? MLP_BN : 2];
-    __shared__ float nscr[SURR ? 4 * 64 * MLP_N4 : 2];
+    __shared__ __attribute__((aligned(16))) uint16_t nw[SURR ? MLP_PWN : 2];
+    __shared__ __attribute__((aligned(16))) float nb[SURR ? MLP_BN : 4];
+    __shared__ __attribute__((aligned(16))) float nscr[SURR ? 4 * 64 * MLP_SCR : 4];
     const int j = step_type(d);
     const DebrisK &a = d.t[j];
     if (SURR) {
-        for (int q = threadIdx.x; q < MLP_WN; q += blockDim.x) nw[q] = a.net.w[q];
+        mlp_stage_padded(nw, a.net.w);
         for (int q = threadIdx.x; q < MLP_BN; q += blockDim.x) nb[q] = a.net.b[q];
     }
     if (PULSE) acc_clear(acc, 4 * a.pk.R);
@@ -4515,6 +4515,84 @@ extern "C" int rsd_debris_samples(rsd_engine *e, int adm_table, int rcs_table, s
     hipError_t r = hipMalloc((void **)&a.t, n * 12 * sizeof(float));
     if (r == hipSuccess) {
         hipLaunchKernelGGL(k_debris_samples, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, e->stream, a);
+        r = hipGetLastError();
+    }
+    if (r == hipSuccess) r = hipMemcpyAsync(x, a.x, n * MLP_IN * sizeof(float), hipMemcpyDeviceToHost, e->stream);
+    if (r == hipSuccess) r = hipMemcpyAsync(t, a.t, n * 12 * sizeof(float), hipMemcpyDeviceToHost, e->stream);
+    if (r == hipSuccess) r = hipStreamSynchronize(e->stream);
+    dfree(a.x);
+    dfree(a.t);
+    HIPCHK(e, r);
+    return RSD_OK;
+}
+
+// Training samples at the population's own states: for every body of one
+// debris type, the network inputs the debris kernel would form (R from the
+// position and orientation, u from the velocity and the wind at the position)
+// and the table path's targets at them.  A body in still air relative to the
+// wind (u undefined) gets u = (1, 0, 0) and the targets there.
+struct StateSampK {
+    const float4 *pos, *ori, *vel;
+    unsigned long long begin, n;
+    Tex3 wind;
+    F16 les, sd;
+    WIdx widx;
+    Tex2 cd, cm, rr, ri;
+    F16 ad, rd;
+    float *x, *t;
+};
+
+__global__ void k_debris_state_samples(StateSampK a) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const float4 p = a.pos[a.begin + i], o = a.ori[a.begin + i], v = a.vel[a.begin + i];
+    const float4 c = wind_index(p, a.les, a.sd, a.widx.t, a.widx);
+    const float4 bg = sample3d(a.wind, c.x, c.y, c.z);
+    const float4 ur = make_float4(bg.x - v.x, bg.y - v.y, bg.z - v.z, bg.w - v.w);
+    float4 uh = qrot(norm4(ur), qconj(o));
+    if (!(uh.x == uh.x && uh.y == uh.y && uh.z == uh.z)) uh = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
+    const float4 R = radar_rel_quat(p, o);
+    float t[12];
+    table_targets(R, uh, a.cd, a.cm, a.ad, a.rr, a.ri, a.rd, t);
+    float *x = a.x + i * MLP_IN;
+    x[0] = R.x; x[1] = R.y; x[2] = R.z; x[3] = R.w;
+    x[4] = uh.x; x[5] = uh.y; x[6] = uh.z;
+    for (int k = 0; k < 12; k++) a.t[i * 12 + k] = t[k];
+}
+
+extern "C" int rsd_debris_state_samples(rsd_engine *e, int type, int adm_table, int rcs_table, int wind_slot,
+                                        const float *sim_desc, size_t cap, float *x, float *t, size_t *n_out) {
+    if (!e || !x || !t || !n_out || !sim_desc || type < 1 || type >= RSD_MAX_TYPES) return RSD_ERR_ARG;
+    if (adm_table < 0 || adm_table >= RSD_MAX_TABLES || !e->adm_cd[adm_table]) return fail(e, RSD_ERR_STATE, "ADM table %d not set", adm_table);
+    if (rcs_table < 0 || rcs_table >= RSD_MAX_TABLES || !e->rcs_re[rcs_table]) return fail(e, RSD_ERR_STATE, "RCS table %d not set", rcs_table);
+    if (wind_slot < 0 || wind_slot > 1 || !e->wind_ok[wind_slot]) return fail(e, RSD_ERR_STATE, "wind slot %d not set", wind_slot);
+    if (!e->pos || !e->ori || !e->vel) return fail(e, RSD_ERR_STATE, "state not allocated");
+    HIPCHK(e, hipSetDevice(e->device));
+    int st = flush_pending(e);
+    if (st) return st;
+    const size_t n = e->counts[type] < cap ? e->counts[type] : cap;
+    *n_out = n;
+    if (n == 0) return RSD_OK;
+    StateSampK a;
+    memset(&a, 0, sizeof(a));
+    a.pos = e->pos; a.ori = e->ori; a.vel = e->vel;
+    a.begin = e->origins[type];
+    a.n = n;
+    a.wind = tex3_of(e->wind[wind_slot], e->wnx[wind_slot], e->wny[wind_slot], e->wnz[wind_slot]);
+    a.les = e->les[wind_slot];
+    memcpy(a.sd.s, sim_desc, sizeof(a.sd.s));
+    st = wind_itab(e, wind_slot, sim_desc, &a.widx, NULL);
+    if (st) return st;
+    a.cd.d = e->adm_cd[adm_table]; a.cm.d = e->adm_cm[adm_table];
+    a.cd.nx = a.cm.nx = e->adm_nx[adm_table]; a.cd.ny = a.cm.ny = e->adm_ny[adm_table];
+    a.ad = e->adm_desc[adm_table];
+    a.rr.d = e->rcs_re[rcs_table]; a.ri.d = e->rcs_im[rcs_table];
+    a.rr.nx = a.ri.nx = e->rcs_nx[rcs_table]; a.rr.ny = a.ri.ny = e->rcs_ny[rcs_table];
+    a.rd = e->rcs_desc[rcs_table];
+    HIPCHK(e, hipMalloc((void **)&a.x, n * MLP_IN * sizeof(float)));
+    hipError_t r = hipMalloc((void **)&a.t, n * 12 * sizeof(float));
+    if (r == hipSuccess) {
+        hipLaunchKernelGGL(k_debris_state_samples, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, e->stream, a);
         r = hipGetLastError();
     }
     if (r == hipSuccess) r = hipMemcpyAsync(x, a.x, n * MLP_IN * sizeof(float), hipMemcpyDeviceToHost, e->stream);

@@ -190,6 +190,15 @@ __device__ __forceinline__ float rf_acosf(float x) {
     return rf_atan2f(s, x);
 }
 
+// acos of a quantity that is a cosine by construction (a component of a unit
+// vector or unit quaternion, a coordinate over the norm that bounds it):
+// rounding can carry it a few ulp beyond [-1, 1], where acos is NaN (and a NaN
+// orientation or velocity never leaves the state); such arguments are taken as
+// +-1, a NaN stays NaN.
+__device__ __forceinline__ float rf_acos_unit(float x) {
+    return rf_acosf(x > 1.0f ? 1.0f : (x < -1.0f ? -1.0f : x));
+}
+
 __device__ __forceinline__ float rf_clamp(float x, float lo, float hi) { return __builtin_fminf(__builtin_fmaxf(x, lo), hi); }
 __device__ __forceinline__ float rf_mix(float x, float y, float a) { return x + (y - x) * a; }
 __device__ __forceinline__ float rf_fract(float x, float *ip) {

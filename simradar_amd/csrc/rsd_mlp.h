@@ -50,6 +50,21 @@
 #define MLP_SHIFT (MLP_SCALE + MLP_N4)
 #define MLP_BN (MLP_SHIFT + MLP_N4)                 // 304 floats
 
+// Padded image of the weights for inference out of LDS: rows of W0 at a
+// stride of 20 bf16 (40 bytes) and of W1..W4 at 68 bf16 (136 bytes), so that
+// the 16 rows one A operand load touches fall in distinct LDS banks (at the
+// packed strides of 32 and 128 bytes they share 2 and 8 ways).
+#define MLP_PK0 20
+#define MLP_PH 68
+#define MLP_PW1 (MLP_H * MLP_PK0)
+#define MLP_PW4 (MLP_PW1 + 3 * MLP_H * MLP_PH)
+#define MLP_PWN (MLP_PW4 + MLP_N4 * MLP_PH)         // 15424 bf16
+#define MLP_SCR 12                                  // floats per sample in the output exchange
+
+// Row strides and offsets of a weight image.
+struct MlpPacked { static constexpr int S0 = MLP_K0, SH = MLP_H, O1 = MLP_W1, O4 = MLP_W4; };
+struct MlpPadded { static constexpr int S0 = MLP_PK0, SH = MLP_PH, O1 = MLP_PW1, O4 = MLP_PW4; };
+
 typedef short mlp_s4 __attribute__((ext_vector_type(4)));
 typedef float mlp_f4 __attribute__((ext_vector_type(4)));
 
@@ -152,67 +167,89 @@ __device__ __forceinline__ mlp_s4 mlp_gather_input(const float f[8], int t) {
     return mlp_pack(a);
 }
 
+// Copy the packed weight image w into the padded image dst (LDS) with the
+// threads of a block, four bf16 (8 bytes) at a time: W0 is 64 rows of 4 such
+// chunks, W1..W4 (contiguous in both images) 208 rows of 16.
+__device__ __forceinline__ void mlp_stage_padded(uint16_t *dst, const uint16_t *w) {
+    const uint2 *src = (const uint2 *)w;
+    for (int c = threadIdx.x; c < MLP_H * 4 + (3 * MLP_H + MLP_N4) * 16; c += blockDim.x) {
+        int s, d;
+        if (c < MLP_H * 4) {
+            s = (c >> 2) * MLP_K0 + (c & 3) * 4;
+            d = (c >> 2) * MLP_PK0 + (c & 3) * 4;
+        } else {
+            const int k = c - MLP_H * 4;
+            s = MLP_W1 + (k >> 4) * MLP_H + (k & 15) * 4;
+            d = MLP_PW1 + (k >> 4) * MLP_PH + (k & 15) * 4;
+        }
+        *(uint2 *)(dst + d) = src[s >> 2];
+    }
+}
+
+// ReLU of four values and their bf16 image: rounded first, then max with 0 as
+// signed 16-bit integers (a negative bf16 is a negative int16, a positive one a
+// positive int16), two packed integer maxima in place of four float ones; the
+// same bf16 values as rounding max(v, 0).
+__device__ __forceinline__ mlp_s4 mlp_relu_pack(mlp_f4 v) {
+    const mlp_s4 h = mlp_pack(v);
+    return __builtin_elementwise_max(h, (mlp_s4){0, 0, 0, 0});
+}
+
 // Forward of the wave's four 16-sample tiles together: every weight tile is
-// read once and applied to the four tiles (four independent MFMA chains).
+// read once and applied to the four tiles (four independent MFMA chains).  The
+// bias enters as the accumulator the first MFMA of each output tile starts
+// from.  LAY: the weight image's layout (MlpPacked or MlpPadded).
+template <class LAY>
 __device__ __forceinline__ void mlp_forward4(const uint16_t *w, const float *b, const mlp_s4 x[4], mlp_f4 y[4]) {
     const int l = threadIdx.x & 63, row = l & 15, grp = l >> 4;
     mlp_s4 cur[4][4], nxt[4][4];
 #pragma unroll
     for (int mt = 0; mt < 4; mt++) {
-        const mlp_s4 A = mlp_ld4(w + MLP_W0 + (16 * mt + row) * MLP_K0 + 4 * grp);
+        const mlp_s4 A = mlp_ld4(w + (16 * mt + row) * LAY::S0 + 4 * grp);
         const mlp_f4 bias = *(const mlp_f4 *)(b + MLP_B0 + 16 * mt + 4 * grp);
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
-            mlp_f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-            acc = mlp_mfma(A, x[t], acc);
-#pragma unroll
-            for (int r = 0; r < 4; r++) acc[r] = mlp_act(acc[r] + bias[r]);
-            cur[t][mt] = mlp_pack(acc);
-        }
+        for (int t = 0; t < 4; t++) cur[t][mt] = mlp_relu_pack(mlp_mfma(A, x[t], bias));
     }
 #pragma unroll 1
     for (int L = 1; L < 4; L++) {
-        const uint16_t *W = w + MLP_W1 + (L - 1) * MLP_H * MLP_H;
+        const uint16_t *W = w + LAY::O1 + (L - 1) * MLP_H * LAY::SH;
 #pragma unroll
         for (int mt = 0; mt < 4; mt++) {
+            const mlp_f4 bias = *(const mlp_f4 *)(b + MLP_H * L + 16 * mt + 4 * grp);
             mlp_f4 acc[4];
 #pragma unroll
-            for (int t = 0; t < 4; t++) acc[t] = (mlp_f4){0.0f, 0.0f, 0.0f, 0.0f};
+            for (int t = 0; t < 4; t++) acc[t] = bias;
 #pragma unroll
             for (int kt = 0; kt < 4; kt++) {
-                const mlp_s4 A = mlp_ld4(W + (16 * mt + row) * MLP_H + 16 * kt + 4 * grp);
+                const mlp_s4 A = mlp_ld4(W + (16 * mt + row) * LAY::SH + 16 * kt + 4 * grp);
 #pragma unroll
                 for (int t = 0; t < 4; t++) acc[t] = mlp_mfma(A, cur[t][kt], acc[t]);
             }
-            const mlp_f4 bias = *(const mlp_f4 *)(b + MLP_H * L + 16 * mt + 4 * grp);
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
-#pragma unroll
-                for (int r = 0; r < 4; r++) acc[t][r] = mlp_act(acc[t][r] + bias[r]);
-                nxt[t][mt] = mlp_pack(acc[t]);
-            }
+            for (int t = 0; t < 4; t++) nxt[t][mt] = mlp_relu_pack(acc[t]);
         }
 #pragma unroll
         for (int t = 0; t < 4; t++)
 #pragma unroll
             for (int k = 0; k < 4; k++) cur[t][k] = nxt[t][k];
     }
+    const mlp_f4 bias = *(const mlp_f4 *)(b + MLP_B4 + 4 * grp);
 #pragma unroll
-    for (int t = 0; t < 4; t++) y[t] = (mlp_f4){0.0f, 0.0f, 0.0f, 0.0f};
+    for (int t = 0; t < 4; t++) y[t] = bias;
 #pragma unroll
     for (int kt = 0; kt < 4; kt++) {
-        const mlp_s4 A = mlp_ld4(w + MLP_W4 + row * MLP_H + 16 * kt + 4 * grp);
+        const mlp_s4 A = mlp_ld4(w + LAY::O4 + row * LAY::SH + 16 * kt + 4 * grp);
 #pragma unroll
         for (int t = 0; t < 4; t++) y[t] = mlp_mfma(A, cur[t][kt], y[t]);
     }
-    const mlp_f4 bias = *(const mlp_f4 *)(b + MLP_B4 + 4 * grp);
-#pragma unroll
-    for (int t = 0; t < 4; t++) y[t] += bias;
 }
 
 // Network outputs of every lane's own sample: the four tiles of the wave are
-// evaluated together and their D tiles exchanged through scratch (64 x 16
-// floats of LDS per wave); out[0..11] = scale * y + shift.
+// evaluated together and their D tiles exchanged through scratch (64 x
+// MLP_SCR floats per wave, the 12 outputs of each sample; at 48 bytes a row the
+// 16 rows one store touches fall in distinct banks); out[0..11] = scale * y +
+// shift.
+template <class LAY = MlpPacked>
 __device__ __forceinline__ void mlp_wave_eval(const uint16_t *w, const float *b, const float f[8], float *scratch,
                                               float out[MLP_OUT]) {
     const int l = threadIdx.x & 63, row = l & 15, grp = l >> 4;
@@ -220,13 +257,19 @@ __device__ __forceinline__ void mlp_wave_eval(const uint16_t *w, const float *b,
     mlp_f4 y[4];
 #pragma unroll
     for (int t = 0; t < 4; t++) x[t] = mlp_gather_input(f, t);
-    mlp_forward4(w, b, x, y);
+    mlp_forward4<LAY>(w, b, x, y);
+    if (grp < MLP_OUT / 4) {
 #pragma unroll
-    for (int t = 0; t < 4; t++) *(mlp_f4 *)(scratch + (16 * t + row) * MLP_N4 + 4 * grp) = y[t];
+        for (int t = 0; t < 4; t++) *(mlp_f4 *)(scratch + (16 * t + row) * MLP_SCR + 4 * grp) = y[t];
+    }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
 #pragma unroll
-    for (int k = 0; k < MLP_OUT; k++) out[k] = scratch[l * MLP_N4 + k] * b[MLP_SCALE + k] + b[MLP_SHIFT + k];
+    for (int q = 0; q < MLP_OUT / 4; q++) {
+        const mlp_f4 v = *(const mlp_f4 *)(scratch + l * MLP_SCR + 4 * q);
+#pragma unroll
+        for (int r = 0; r < 4; r++) out[4 * q + r] = v[r] * b[MLP_SCALE + 4 * q + r] + b[MLP_SHIFT + 4 * q + r];
+    }
     __builtin_amdgcn_wave_barrier();
 }
 

@@ -218,7 +218,7 @@ __global__ void k_mlp_adam(AdamK a) {
 }
 
 __global__ __launch_bounds__(64) void k_mlp_infer(MlpDev net, const float *x, float *y, size_t n) {
-    __shared__ float scratch[64 * MLP_N4];
+    __shared__ __attribute__((aligned(16))) float scratch[64 * MLP_SCR];
     const size_t i = (size_t)blockIdx.x * 64 + threadIdx.x;
     float f[8];
 #pragma unroll

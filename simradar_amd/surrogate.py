@@ -75,9 +75,47 @@ def table_samples(radar, debris_type, n, seed):
     return x, t
 
 
-def fit(radar, debris_type, n_samples=1 << 21, epochs=30, batch=4096, lr=2e-3, seed=7):
-    """Train a surrogate of one debris type's tables; returns (Mlp, last batch loss)."""
-    x, t = table_samples(radar, debris_type, n_samples, seed)
+def state_samples(radar, debris_type):
+    """(x[n, 7], t[n, 12]) at the current states of a debris type's bodies:
+    the inputs its advance and pulse would give the network, and the table
+    path's targets there (rsd_debris_state_samples)."""
+    lib = radar.lib
+    adm = rcs = debris_type - 1
+    n = int(radar.info().counts[debris_type])
+    x = np.zeros((max(n, 1), 7), np.float32)
+    t = np.zeros((max(n, 1), 12), np.float32)
+    got = ctypes.c_size_t()
+    inf = radar.info()
+    sd = np.ascontiguousarray(radar.sim_desc(), np.float32)
+    e = radar.engine()
+    if lib.rsd_debris_state_samples(e, debris_type, adm, rcs, int(inf.les_id), _ptr(sd), n, _ptr(x), _ptr(t),
+                                    ctypes.byref(got)):
+        raise RuntimeError(lib.rsd_last_error(e).decode())
+    return x[:got.value], t[:got.value]
+
+
+def training_set(radar, debris_type, n_samples, seed, state_frac=0.0):
+    """n_samples table-path samples: uniform over orientations and flow
+    directions, of which a fraction state_frac is replaced by the population's
+    current states (repeated to fill their share)."""
+    n_state = int(round(n_samples * state_frac))
+    x, t = table_samples(radar, debris_type, n_samples - n_state, seed)
+    if n_state > 0:
+        xs, ts = state_samples(radar, debris_type)
+        if len(xs):
+            idx = np.random.default_rng(seed).integers(0, len(xs), n_state)
+            x = np.concatenate([x, xs[idx]])
+            t = np.concatenate([t, ts[idx]])
+            perm = np.random.default_rng(seed + 1).permutation(len(x))
+            x, t = x[perm], t[perm]
+    return x, t
+
+
+def fit(radar, debris_type, n_samples=1 << 21, epochs=30, batch=4096, lr=2e-3, seed=7, state_frac=0.0):
+    """Train a surrogate of one debris type's tables; returns (Mlp, last batch loss).
+    state_frac > 0 mixes samples at the population's current states into the
+    uniform ones (training_set)."""
+    x, t = training_set(radar, debris_type, n_samples, seed, state_frac)
     mean = t.mean(0)
     std = t.std(0)
     std[std < 1e-12] = 1.0

@@ -4,8 +4,9 @@ DSD drawn from its CDF with unbounded drag, cross-polar debris returns,
 uniform-grid winds (cube27 / cube125), non-triangle range-weight tables in both
 gate folds, the binary32 phase, the signal / aux download streams, the pulse
 test's known-answer inputs (test_make_pulse.c) through the C-ABI, the
-transparent-rain superposition of test_wx_db_compare.sh, the LES volume switch
-and a batched domain.  State streams bit-exact, IQ within 1e-4 of the RMS gate
+transparent-rain superposition of test_wx_db_compare.sh, the LES volume switch,
+a batched domain and a debris pose whose radar-relative quaternion rounds past
+the identity.  State streams bit-exact, IQ within 1e-4 of the RMS gate
 magnitude (tests/harness.py rel_err)."""
 import ctypes
 
@@ -456,4 +457,55 @@ def test_les_time_interpolation_against_the_oracle(gpu_ready, debris):
     assert a.min() < 0.01 and a.max() > 0.99 and np.sum(np.diff(a) < 0) == 2, "two volume boundaries"
     h.check_state(r, o)
     _assert_iq(pg, po)
+    r.free()
+
+
+def _radar_rel_w(p, o):
+    """w of R = ori (x) o_conj(el, az) (rs.cl:287-315) in the oracle's binary32
+    arithmetic."""
+    f = np.float32
+    one = lambda fn, *x: orc.math(fn, *[np.array([v], f) for v in x])[0]
+    el = one("atan2", p[2], np.sqrt(f(f(p[0] * p[0]) + f(p[1] * p[1]))))
+    az = one("atan2", p[0], p[1])
+    a1, a2 = f(f(0.5) * f(el + f(np.pi / 2))), f(f(0.5) * az)
+    se, ce, sa_, ca = one("sin", a1), one("cos", a1), one("sin", a2), one("cos", a2)
+    s2 = f(0.70710677)
+    oc = [f(f(f(-se * ca) + f(ce * sa_)) * s2), f(f(f(ce * ca) + f(se * sa_)) * s2),
+          f(f(f(se * ca) + f(ce * sa_)) * s2), f(f(f(ce * ca) + f(-se * sa_)) * s2)]
+    return orc.qmul(o, oc)[3]
+
+
+def test_debris_at_the_identity_pose_stays_finite(gpu_ready):
+    """A leaf straight ahead of the radar (az ~ 0, el ~ 0) near the orientation
+    RS_populate gives every body: its radar-relative quaternion is the identity
+    up to rounding and w rounds past 1, where the reference's acos (rs.cl:323)
+    gives NaN (and, through the dense pass 1, NaN in every gate).  Engine and
+    oracle take w as 1: the refreshed rcs and the pulse are finite and agree."""
+    pos = np.float32([-0.4720836, 1925.8678, 1.2334673, 0.0])
+    ori = np.float32([0.50010043, -0.49989828, -0.5000516, 0.49994978])
+    assert _radar_rel_w(pos, ori) > 1.0          # the pose exercises the clamp
+    cfg = h.Config(n_rain=4096, debris=[(sa.OBJ_LEAF, 64)], scan="P:3.0,-1:1:0.01")
+    r = sa.Radar(verb=0)
+    h.configure(r, cfg)
+    r.populate()
+    r.check()
+    n0 = int(r.info().counts[0])
+    for s, v in (("pos", pos), ("ori", ori)):
+        r.stream(s, copy=False)[n0] = v
+    r.upload()
+    o = h.OracleRun(r, cfg, h.state_from_radar(r))
+    r.advance_beam()
+    r.make_pulse()                  # the beam moved: debris rcs refreshed at the placed pose
+    sd = r.sim_desc()
+    r.download_pulse_only()
+    r.check()
+    g = r.pulse_f64()
+    o.refresh_debris_rcs(sd)
+    ref = o.pulse(sd, orc.PHASE_F64)
+    r.download()
+    rcs = r.stream("rcs")
+    assert np.all(np.isfinite(rcs)) and np.all(np.isfinite(o.st.rcs))
+    np.testing.assert_array_equal(rcs[n0].view(np.uint32), o.st.rcs[n0].view(np.uint32))
+    assert np.all(np.isfinite(g)) and np.all(np.isfinite(ref))
+    assert h.rel_err(g, ref) < IQ_TOL
     r.free()

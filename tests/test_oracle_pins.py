@@ -410,3 +410,16 @@ def test_time_interpolated_wind_is_linear_in_the_weight():
         np.testing.assert_array_equal(st.vel[0, :3], u0.astype(np.float32))
     finally:
         orc.set_blend(None)
+
+
+def test_quat2abg_unit_component_rounded_past_one():
+    """A radar-relative quaternion within a degree of the identity whose w
+    rounds past 1 (a product of unit quaternions): rs.cl:323's acos(w) would
+    be NaN; the oracle (and engine) take w as 1, so gamma = 0, and any w inside
+    [-1, 1] gives the unclamped result."""
+    w = np.nextafter(np.float32(1.0), np.float32(2.0))
+    a, b, g = orc.quat2abg([1e-5, 0.0, 2e-5, float(w)])
+    assert (a, b) == (0.0, 0.0) and g == 0.0
+    a, b, g = orc.quat2abg([0.0, 0.0, 0.01, 0.99995])
+    want = 2.0 * np.arccos(np.float32(0.99995))
+    assert a == 0.0 and b == 0.0 and abs(g - want) < 2e-6
