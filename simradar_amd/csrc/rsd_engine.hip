@@ -595,13 +595,6 @@ __device__ __forceinline__ void fold_gates(bool on, float r, float4 sa, const Pu
     }
 }
 
-// Lane-private window of RSD_WIN consecutive gates [kb, kb + RSD_WIN): binary32
-// partial sums of the scatterers a lane visits (rs.cl:1182-1217 terms).  With
-// the streams in range order the scatterers of one lane, and of one wave, fall
-// into the same gates, so the terms are summed in registers and the wave adds
-// them to LDS once per launch.  A scatterer whose gates leave the window first
-// flushes the lane's window with per-lane LDS atomics.
-#define RSD_WIN 8
 #define RSD_LDS_DROP_MAX 256      // drop RCS entries staged in LDS by rain pulse launches
 #define RSD_LDS_ANGW_MAX 1024     // angular weight entries staged in LDS
 #define RSD_WIN_EMPTY (-(1 << 24))
@@ -613,17 +606,6 @@ __device__ __forceinline__ void fold_gates(bool on, float r, float4 sa, const Pu
 #ifndef RSD_WWIN
 #define RSD_WWIN 8
 #endif
-
-struct GateWin {
-    float a[RSD_WIN][4];
-    int kb;
-};
-
-__device__ __forceinline__ void win_reset(GateWin &w, int kb) {
-    w.kb = kb;
-#pragma unroll
-    for (int s = 0; s < RSD_WIN; s++) w.a[s][0] = w.a[s][1] = w.a[s][2] = w.a[s][3] = 0.0f;
-}
 
 // Gates whose range offset lies inside the weight's support (klo > khi: none).
 __device__ __forceinline__ void gate_support(float r, const PulseK &k, int &klo, int &khi) {
@@ -641,19 +623,6 @@ __device__ __forceinline__ void gate_support(float r, const PulseK &k, int &klo,
     } else {
         klo = 0x3fffffff;
         khi = -1;
-    }
-}
-
-__device__ __forceinline__ void win_lane_flush(const GateWin &w, const PulseK &k, double *acc) {
-    const int R = k.R;
-#pragma unroll
-    for (int s = 0; s < RSD_WIN; s++) {
-        const int g = w.kb + s;
-        if (g >= 0 && g < R) {
-#pragma unroll
-            for (int c = 0; c < 4; c++)
-                if (w.a[s][c] != 0.0f) atomicAdd(&acc[c * R + g], (double)w.a[s][c]);
-        }
     }
 }
 
@@ -703,68 +672,28 @@ __device__ __forceinline__ void stage_window(double *acc, const PulseK &k) {
     __syncthreads();
 }
 
-__device__ __forceinline__ void win_fold(GateWin &w, bool on, float r, float4 sa, const PulseK &k, double *acc,
-                                         const float2 *wl, const float *grl) {
+// Fold of scatterers in no spatial order (debris): each lane adds its terms for
+// the gates of its support straight to the block's binary64 gate sums in LDS
+// (the range weight from the staged table pairs).
+__device__ __forceinline__ void lane_fold(bool on, float r, float4 sa, const PulseK &k, double *acc, const float2 *wl,
+                                          const float *grl) {
     int klo = 0x3fffffff, khi = -1;
     if (on) gate_support(r, k, klo, khi);
-    on = klo <= khi;
-    if (on && (klo < w.kb || khi >= w.kb + RSD_WIN)) {
-        if (w.kb != RSD_WIN_EMPTY) win_lane_flush(w, k, acc);
-        win_reset(w, klo - 1);     // one gate of room below
-    }
-    // slots any lane of the wave needs (wave-uniform bounds)
-    const int smin = wave_min_i(on ? klo - w.kb : RSD_WIN);
-    const int smax = wave_max_i(on ? khi - w.kb : -1);
-    const int gmax = k.R - 1;
-#pragma unroll
-    for (int s = 0; s < RSD_WIN; s++) {
-        if (s < smin || s > smax) continue;
-        const int g = min(max(w.kb + s, 0), gmax);
-        float wr = lds_weight(r - grl[g], wl, k.rangew_desc);
-        if (!on || w.kb + s != g) wr = 0.0f;
-        w.a[s][0] = __builtin_fmaf(wr, sa.x, w.a[s][0]);
-        w.a[s][1] = __builtin_fmaf(wr, sa.y, w.a[s][1]);
-        w.a[s][2] = __builtin_fmaf(wr, sa.z, w.a[s][2]);
-        w.a[s][3] = __builtin_fmaf(wr, sa.w, w.a[s][3]);
-    }
-}
-
-// End of launch: the wave sums the windows of lanes that share a base gate and
-// one lane adds them to LDS; lanes whose bases spread wider flush alone.
-__device__ __forceinline__ void win_flush(GateWin &w, const PulseK &k, double *acc) {
     const int R = k.R;
-    const int lo = wave_min_i(w.kb != RSD_WIN_EMPTY ? w.kb : 0x3fffffff);
-    const int hi = wave_max_i(w.kb);
-    if (hi == RSD_WIN_EMPTY) return;
-    if (hi - lo >= 4) {
-        if (w.kb != RSD_WIN_EMPTY) win_lane_flush(w, k, acc);
-        return;
-    }
-    const bool lead = (threadIdx.x & 63) == 0;
-    for (int v = lo; v <= hi; v++) {
-        const bool mine = w.kb == v;
-        if (__ballot(mine) == 0) continue;
-        float S[RSD_WIN][4];
-#pragma unroll
-        for (int s = 0; s < RSD_WIN; s++)
-#pragma unroll
-            for (int c = 0; c < 4; c++) S[s][c] = wave_sum32(mine ? w.a[s][c] : 0.0f);
-        if (lead) {
-#pragma unroll
-            for (int s = 0; s < RSD_WIN; s++) {
-                if (v + s >= 0 && v + s < R) {
-#pragma unroll
-                    for (int c = 0; c < 4; c++)
-                        if (S[s][c] != 0.0f) atomicAdd(&acc[c * R + v + s], (double)S[s][c]);
-                }
-            }
+    for (int g = klo; g <= khi; g++) {
+        const float wr = lds_weight(r - grl[g], wl, k.rangew_desc);
+        if (wr != 0.0f) {
+            atomicAdd(&acc[g], (double)(wr * sa.x));
+            atomicAdd(&acc[R + g], (double)(wr * sa.y));
+            atomicAdd(&acc[2 * R + g], (double)(wr * sa.z));
+            atomicAdd(&acc[3 * R + g], (double)(wr * sa.w));
         }
     }
 }
 
 // ---------------------------------------------------------------------------
 // Wave-shared window (rain streams in range order): the 64 lanes of a wave
-// share one window of RSD_WIN consecutive gates [kb, kb + RSD_WIN), kb
+// share one window of RSD_WWIN consecutive gates [kb, kb + RSD_WWIN), kb
 // wave-uniform.  Each lane keeps binary32 partial sums of its scatterers' terms
 // for the 32 (gate, component) slots; the wave reduces them in one transposed
 // pass when the window moves or the launch ends.  Scatterers whose gates fall
@@ -1588,8 +1517,6 @@ __device__ __forceinline__ void debris_range(const DebrisK &a, size_t lo, size_t
     const bool use_win = PULSE && a.pk.fold == RSD_FOLD_WINDOW;
     const float2 *wl = win_weights(acc, a.pk);
     const float *grl = win_gates(acc, a.pk);
-    GateWin win;
-    win_reset(win, RSD_WIN_EMPTY);
     for (size_t base = lo; base < hi; base += blockDim.x) {
         const size_t i = base + threadIdx.x;
         const bool live = i < hi;
@@ -1674,11 +1601,10 @@ __device__ __forceinline__ void debris_range(const DebrisK &a, size_t lo, size_t
             float4 sa = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             float r = 0.0f;
             bool on = live && scat_signal(p, rcs, a.pk, a.pk.angw, sa, r);
-            if (use_win) win_fold(win, on, r, sa, a.pk, acc, wl, grl);
+            if (use_win) lane_fold(on, r, sa, a.pk, acc, wl, grl);
             else fold_gates(on, r, sa, a.pk, acc);
         }
     }
-    if (use_win) win_flush(win, a.pk, acc);
 }
 
 // Debris types of one launch: blocks [first[j], first[j + 1]) take type j.
