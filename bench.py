@@ -29,6 +29,7 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBS = 8000.0
 PROFILE_STRIDE = 10                 # rain launches per HIP-event timed launch
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_rain_c2_traffic.json")
+TRAFFIC_FILE_C3 = os.path.join(ROOT, "profiles", "k_rain_c3_traffic.json")
 TRAIN_SAMPLES = 10_000_000          # table-path samples per debris type for the C5 networks
 REFERENCE_PULSES_PER_S = 222.3   # README.md:58 (instantaneous, 1,998,848 scatterers, GPU not stated)
 
@@ -89,6 +90,8 @@ def main():
     ap.add_argument("--sort", type=int, default=-1,
                     help="pulses between range reorders of the rain streams (-1: the engine default)")
     ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"])
+    ap.add_argument("--c5-samples", type=int, default=TRAIN_SAMPLES,
+                    help="training samples per debris type of the C5 networks (fewer: shorter profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build drives a single MI355X per process "
@@ -251,13 +254,20 @@ def bench_c3(args):
     kern = k_ms / max(1, k_n)
     byts = 96 * cfg.n_rain            # pos + vel + lo read and written
     ach = byts / (kern * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE_C3):
+        with open(TRAFFIC_FILE_C3) as f:
+            t = json.load(f)
+        traffic = t["fetch_bytes"] * t["fetch_factor"] + t["write_bytes"] * t["write_factor"]
     _line("pulses/s & scatterer-updates/s (C3, hi/lo positions)", 1000.0 / ms, "pulses/s", steps, warm, ms,
           "f32 state, f32 hi/lo positions, f64 range-to-phase and gate sums",
           {"workload": "C3: 8,388,608 rain, 3 km x 3 km x 1 km box at 10 km range, %d gates x 15 m" % R,
            "scatterers": cfg.n_rain, "range_gates": R},
           {"scatterer_updates_per_s": 1000.0 / ms * cfg.n_rain,
            "roofline": {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                        "frac": ach / PEAK_HBM_GBS, "traffic": None, "kernel": "k_rain<advance,pulse,hilo>",
+                        "frac": ach / PEAK_HBM_GBS, "traffic": traffic,
+                        "traffic_source": "profiles/k_rain_c3_traffic.json" if traffic else None,
+                        "kernel": "k_rain<advance,pulse,hilo>",
                         "kernel_avg_ms": kern, "bytes_per_launch": byts}})
     r.free()
 
@@ -342,7 +352,7 @@ def bench_c5(args):
     t0 = time.perf_counter()
     nets, errs = [], []
     for k in (1, 2, 3):
-        m, _ = sg.fit(r, k, n_samples=TRAIN_SAMPLES, epochs=16, state_frac=0.25)
+        m, _ = sg.fit(r, k, n_samples=args.c5_samples, epochs=16, state_frac=0.25)
         errs.append(sg.error(m, r, k))
         nets.append(m)
     train_s = time.perf_counter() - t0
@@ -355,7 +365,7 @@ def bench_c5(args):
            "scatterers": cfg.n_total},
           {"scatterer_updates_per_s": 1000.0 / ms_surr * cfg.n_total,
            "table_pulses_per_s": 1000.0 / ms_table, "surrogate_over_table": ms_table / ms_surr,
-           "training_s": train_s, "training_samples_per_type": TRAIN_SAMPLES,
+           "training_s": train_s, "training_samples_per_type": args.c5_samples,
            "training_set": "uniform orientations and flow directions, a quarter at the population's states",
            "heldout_rel_err_cd_cm_amp": errs})
     for m in nets:
