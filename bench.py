@@ -2,8 +2,12 @@
 
 Workload (the configuration the headline metric is quoted on): 1,997,824 rain
 scatterers + 1,024 leaf debris (1,998,848 in total), Marshall-Palmer 10 mm/h DSD,
-suctvort-shaped synthetic LES 176x176x80, concept D|B|U, 15 m gates (40 gates),
-1 deg beam, 0.2 us pulse, PRT 1 ms, lambda 0.1 m, PPI 3 deg -12..12 in 0.01 deg.
+suctvort-shaped synthetic LES 176x176x80 with its 10 volumes interpolated in
+time (two volumes resident, the wind blended at the pulse's time), concept
+D|B|U, 15 m gates (40 gates), 1 deg beam, 0.2 us pulse, PRT 1 ms, lambda 0.1 m,
+PPI 3 deg -12..12 in 0.01 deg.  The same workload with the reference's
+piecewise-constant volume switch (one volume read per step) is timed beside it
+(--no-alt skips it; --les-step makes it the headline).
 A step is one pulse: the advance of every scatterer by one PRT and the per-gate
 complex H/V sums (RS_advance_time + RS_advance_beam + RS_make_pulse), with the
 state resident in HBM.  Prints one JSON line.
@@ -38,11 +42,19 @@ def cpu_baseline(r, cfg, pulses, threads, warmup=5, runs=3):
     """The oracle (CPU restatement of the reference kernels: the same kernel
     sequence, dense pass 1 over all gates) timed on a bounded sample of the same
     workload, each population split over `threads` host threads: `warmup`
-    untimed pulses, then `runs` runs of `pulses` pulses; the median rate."""
+    untimed pulses, then `runs` runs of `pulses` pulses; the median rate.  With
+    the LES volumes interpolated in time the oracle reads both volumes too."""
+    from oracle import oracle as orc
     from oracle import run as orun
     st = orun.state_from_radar(r)
     o = orun.OracleRun(r, cfg, st, threads=threads)
     sd = r.sim_desc()
+    if cfg.les_interp:
+        inf = r.info()
+        vn = int(inf.vel_idx)
+        o.set_les_frame(r.les_frame(vn - 2))
+        nxt = orc.tex(r.les_frame(vn - 1))
+        orc.set_blend(nxt[0], None, float(inf.wind_alpha))
 
     def one():
         o.advance(sd)
@@ -56,6 +68,8 @@ def cpu_baseline(r, cfg, pulses, threads, warmup=5, runs=3):
         for _ in range(pulses):
             one()
         rates.append(pulses / (time.perf_counter() - t0))
+    if cfg.les_interp:
+        orc.set_blend(None)
     return sorted(rates)[len(rates) // 2], rates
 
 
@@ -75,6 +89,50 @@ def timed_pulses(requested, interval):
     return (n + interval - 1) // interval * interval
 
 
+def time_c2(r, warmup, steps, e2e_steps):
+    """One step = one PRT: RS_advance_time + RS_advance_beam + RS_make_pulse.
+    The advance is deferred into the pulse's launch, so every step is one fused
+    advance + pulse launch and nothing is left pending between steps.  Returns
+    (event ms over the timed steps, wall s, sampled kernel ms, samples,
+    end-to-end pulses/s with the per-pulse IQ download or None)."""
+    def step():
+        r.advance_time()
+        r.advance_beam()
+        r.make_pulse()
+
+    for _ in range(warmup):
+        step()
+    r.sync()
+    r.check()
+    # HIP events around every PROFILE_STRIDE-th rain launch of the timed loop
+    # (events on every launch add a few us of queue packets per pulse)
+    r.profile(True, stride=PROFILE_STRIDE)
+    print("bench: timing %d pulses (whole range-reorder intervals)" % steps, file=sys.stderr, flush=True)
+    r.sync()
+    t0 = time.perf_counter()
+    r.timer_start()
+    for _ in range(steps):
+        step()
+    ev_ms = r.timer_stop()
+    r.sync()
+    wall = time.perf_counter() - t0
+    r.check()
+    k_ms, k_n = r.profile_read(3)
+    r.profile(False)
+    # End to end: the same pulse loop with the blocking per-pulse IQ download
+    # that the reference's driver performs when writing IQ (simradar.c:1057-1059).
+    e2e = None
+    if e2e_steps > 0:
+        r.sync()
+        t1 = time.perf_counter()
+        for _ in range(e2e_steps):
+            step()
+            r.download_pulse_only()
+        e2e = e2e_steps / (time.perf_counter() - t1)
+        r.check()
+    return ev_ms, wall, k_ms, k_n, e2e
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +148,10 @@ def main():
     ap.add_argument("--sort", type=int, default=-1,
                     help="pulses between range reorders of the rain streams (-1: the engine default)")
     ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"])
+    ap.add_argument("--les-step", action="store_true",
+                    help="C2 headline with the reference's piecewise-constant LES volume switch instead of the "
+                         "time interpolation")
+    ap.add_argument("--no-alt", action="store_true", help="C2: skip timing the other LES volume mode")
     ap.add_argument("--c5-samples", type=int, default=TRAIN_SAMPLES,
                     help="training samples per debris type of the C5 networks (fewer: shorter profiling runs)")
     args = ap.parse_args()
@@ -108,6 +170,7 @@ def main():
     from simradar_amd import workloads as wl
 
     cfg = wl.c2()
+    cfg.les_interp = not args.les_step
     print("bench: populating %s (%d scatterers)" % (cfg.name, cfg.n_total), file=sys.stderr, flush=True)
     r = wl.make(cfg)
     if args.block or args.max_blocks:
@@ -129,47 +192,22 @@ def main():
                          "workload, oracle C restatement (dense pass 1 over all gates), scatterers split over "
                          "%d host threads" % (args.cpu_pulses, cfg.n_total, threads)}
 
-    # One step = one PRT: RS_advance_time + RS_advance_beam + RS_make_pulse.  The
-    # advance is deferred into the pulse's launch, so every step is one fused
-    # advance + pulse launch and nothing is left pending between steps.
-    def step():
-        r.advance_time()
-        r.advance_beam()
-        r.make_pulse()
-
-    for _ in range(args.warmup):
-        step()
-    r.sync()
-    r.check()
     steps = timed_pulses(args.steps, r.sort_interval())
-    # HIP events around every PROFILE_STRIDE-th rain launch of the timed loop
-    # (events on every launch add a few us of queue packets per pulse)
-    r.profile(True, stride=PROFILE_STRIDE)
-    print("bench: timing %d pulses (%d requested, whole range-reorder intervals)" % (steps, args.steps),
-          file=sys.stderr, flush=True)
-    r.sync()
-    t0 = time.perf_counter()
-    r.timer_start()
-    for _ in range(steps):
-        step()
-    ev_ms = r.timer_stop()
-    r.sync()
-    wall = time.perf_counter() - t0
-    r.check()
-    k_ms, k_n = r.profile_read(3)
-    r.profile(False)
-
-    # End to end: the same pulse loop with the blocking per-pulse IQ download
-    # that the reference's driver performs when writing IQ (simradar.c:1057-1059).
-    e2e = None
-    if args.e2e_steps > 0:
-        r.sync()
-        t1 = time.perf_counter()
-        for _ in range(args.e2e_steps):
-            step()
-            r.download_pulse_only()
-        e2e = args.e2e_steps / (time.perf_counter() - t1)
-        r.check()
+    ev_ms, wall, k_ms, k_n, e2e = time_c2(r, args.warmup, steps, args.e2e_steps)
+    alt = None
+    if not args.no_alt:
+        r.free()
+        acfg = wl.c2()
+        acfg.les_interp = not cfg.les_interp
+        print("bench: the %s mode" % ("time-interpolated" if acfg.les_interp else "volume-switch"),
+              file=sys.stderr, flush=True)
+        r = wl.make(acfg)
+        if args.sort >= 0:
+            r.set_sort(args.sort)
+        a_ms, _, ak_ms, ak_n, _ = time_c2(r, args.warmup, steps, 0)
+        alt = {"les_mode": "interpolated" if acfg.les_interp else "volume switch (reference)",
+               "pulses_per_s": 1000.0 * steps / a_ms, "kernel_avg_ms": ak_ms / max(1, ak_n),
+               "roofline_frac": cfg.bytes_per_pulse() / (ak_ms / max(1, ak_n) * 1e-3) / 1e9 / PEAK_HBM_GBS}
     ms_per_step = ev_ms / steps
     pps = 1000.0 / ms_per_step
     n = cfg.n_total
@@ -204,8 +242,10 @@ def main():
         "dtype": "f32 state, f64 phase and gate sums",
         "data": "synthetic (seeded LES / DSD / ADM / RCS tables)",
         "config": {"workload": "C2: 1,997,824 rain + 1,024 leaf debris, %d gates x 15 m, PPI 3 deg -12:12:0.01, "
-                               "suctvort LES 176x176x80, concept DBU" % R,
+                               "suctvort LES 176x176x80 (10 volumes), concept DBU" % R,
+                   "les_mode": "interpolated in time" if cfg.les_interp else "volume switch every tp (reference)",
                    "scatterers": n, "range_gates": R, "algorithmic_bytes_per_pulse": cfg.bytes_per_pulse()},
+        "other_les_mode": alt,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "k_rain<advance,pulse> (rain + leaf debris blocks)", "kernel_avg_ms": kern_ms,

@@ -1660,8 +1660,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 #ifndef RSD_RAIN_WAVES
 #define RSD_RAIN_WAVES 4    // waves per SIMD the fused rain kernel is compiled for
 #endif
+// The time-interpolated variants (TI: eight more texels of the next volume in
+// flight) take three waves per SIMD: at four they spill 44 VGPRs to scratch.
 template <bool ADV, bool PULSE, bool HILO, bool TI = false, bool FAST = false>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RSD_RAIN_WAVES))) void k_rain(RainK a, StepDebris d) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TI ? 3 : RSD_RAIN_WAVES))) void k_rain(RainK a, StepDebris d) {
     extern __shared__ double acc[];
     __shared__ float4 itab[RSD_ITAB_ROWS];
     const unsigned nd = (unsigned)d.first[d.nt];
@@ -2533,14 +2535,19 @@ static int blocks_for(const rsd_engine *e, size_t n) {
 // (occupancy of the launch's variant and LDS size, times the compute units).
 // Launches of a multiple of that many equal blocks run in whole rounds (no
 // block waits alone for a free slot at the end).
-static int rain_resident_blocks(rsd_engine *e, bool hilo, bool fast, size_t lds) {
-    const int key = (int)hilo | ((int)fast << 1);
+static int rain_resident_blocks(rsd_engine *e, bool hilo, bool fast, bool ti, size_t lds) {
+    const int key = (int)hilo | ((int)fast << 1) | ((int)ti << 2);
     if (!e->rain_per_cu || e->rain_occ_hilo != key || e->rain_occ_lds != lds) {
         int nb = 0;
-        const void *k = hilo ? (fast ? reinterpret_cast<const void *>(&k_rain<true, true, true, false, true>)
-                                     : reinterpret_cast<const void *>(&k_rain<true, true, true>))
-                             : (fast ? reinterpret_cast<const void *>(&k_rain<true, true, false, false, true>)
-                                     : reinterpret_cast<const void *>(&k_rain<true, true, false>));
+        const void *k;
+        if (ti) k = hilo ? (fast ? reinterpret_cast<const void *>(&k_rain<true, true, true, true, true>)
+                                 : reinterpret_cast<const void *>(&k_rain<true, true, true, true>))
+                         : (fast ? reinterpret_cast<const void *>(&k_rain<true, true, false, true, true>)
+                                 : reinterpret_cast<const void *>(&k_rain<true, true, false, true>));
+        else k = hilo ? (fast ? reinterpret_cast<const void *>(&k_rain<true, true, true, false, true>)
+                              : reinterpret_cast<const void *>(&k_rain<true, true, true>))
+                      : (fast ? reinterpret_cast<const void *>(&k_rain<true, true, false, false, true>)
+                              : reinterpret_cast<const void *>(&k_rain<true, true, false>));
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, e->block, lds) != hipSuccess || nb < 1) nb = 1;
         e->rain_per_cu = nb;
         e->rain_occ_hilo = key;
@@ -2942,7 +2949,7 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p, con
         const size_t lds_est = (size_t)4 * R * sizeof(double) + (size_t)e->rangew_n * sizeof(float2) +
                                (size_t)R * sizeof(float) + 16 + (size_t)RSD_LDS_DROP_MAX * sizeof(float4) +
                                (size_t)RSD_LDS_ANGW_MAX * sizeof(float);
-        const int res = rain_resident_blocks(e, e->plo != NULL, rain_fast(e, s, p), lds_est);
+        const int res = rain_resident_blocks(e, e->plo != NULL, rain_fast(e, s, p), s->wind_blend != 0, lds_est);
         // riders taking a round or more: the rain keeps the default grid
         const int round = ride_blocks < res ? 2 * res - ride_blocks : rain_blocks;
         const size_t need = (e->n_rain + 63) / 64;   // at least a wave per block
