@@ -145,6 +145,28 @@ __device__ __forceinline__ void gather3d(const Tex3 &t, float u, float v, float 
     g.t[6] = b3[0]; g.t[7] = b3[1];
 }
 
+// Trilinear texels of two volumes stored as texel pairs (pair[2 t] = volume 1,
+// pair[2 t + 1] = volume 2, the padded layout of t): the eight corners of both
+// volumes in eight 32-byte reads, each pair within one cache line, so the
+// second volume's texels come with the first's.
+__device__ __forceinline__ void gather3d_pair(const Tex3 &t, const float4 *pair, float u, float v, float w, Gather3 &g,
+                                              Gather3 &g2) {
+    const uint32_t i = tex_axis_padded(u, t.fx, g.a);
+    const uint32_t j = tex_axis_padded(v, t.fy, g.b);
+    const uint32_t k = tex_axis_padded(w, t.fz, g.c);
+    g2.a = g.a;
+    g2.b = g.b;
+    g2.c = g.c;
+    const float4 *b0 = pair + 2u * (k * t.pxy + j * t.px + i);
+    const float4 *b1 = b0 + 2u * t.px;
+    const float4 *b2 = b0 + 2u * t.pxy;
+    const float4 *b3 = b2 + 2u * t.px;
+    g.t[0] = b0[0]; g2.t[0] = b0[1]; g.t[1] = b0[2]; g2.t[1] = b0[3];
+    g.t[2] = b1[0]; g2.t[2] = b1[1]; g.t[3] = b1[2]; g2.t[3] = b1[3];
+    g.t[4] = b2[0]; g2.t[4] = b2[1]; g.t[5] = b2[2]; g2.t[5] = b2[3];
+    g.t[6] = b3[0]; g2.t[6] = b3[1]; g.t[7] = b3[2]; g2.t[7] = b3[3];
+}
+
 __device__ __forceinline__ float4 lerp3d(const Gather3 &g) {
     float4 x00 = lerp4(g.t[0], g.t[1], g.a);
     float4 x10 = lerp4(g.t[2], g.t[3], g.a);
@@ -906,6 +928,7 @@ struct RainK {
     F4 drop_desc;
     uint32_t dragged;
     WIdx widx;              // stretched-grid index tables of the wind slot
+    const float4 *wpair;    // time-interpolated wind: both volumes as texel pairs (specialised loop)
     PulseK pk;
     // Pulse launches stage the drop RCS table (drop_n entries) and the angular
     // weight table (angw_n entries) in LDS at byte offset tab_off; tab_lds = 0:
@@ -1017,6 +1040,10 @@ __device__ __forceinline__ bool rain_move(const RainK &a, size_t i, float4 &pos,
 template <bool TI = false, bool FAST = false>
 __device__ __forceinline__ void rain_gather(const RainK &a, float4 pos, const float4 *itab, Gather3 &g, Gather3 *g2 = nullptr) {
     float4 c = wind_index<FAST>(pos, a.les, a.sd, itab, a.widx, a.cx, a.cy);
+    if (TI && FAST) {
+        gather3d_pair(a.wind, a.wpair, c.x, c.y, c.z, g, *g2);
+        return;
+    }
     gather3d(a.wind, c.x, c.y, c.z, g);
     if (TI) {
         Tex3 t2 = a.wind;
@@ -2083,6 +2110,15 @@ static pthread_mutex_t scratch_lock = PTHREAD_MUTEX_INITIALIZER;
 // Padded copy of a 3-D table (see Tex3): texel (i, j, k) of the padded volume is
 // texel (clamp(i - RSD_PAD_LO), clamp(j - RSD_PAD_LO), clamp(k - RSD_PAD_LO)) of
 // the table.
+// Texel pairs of two volumes of one padded shape: out[2 t] = a[t], out[2 t + 1] = b[t].
+__global__ void k_wind_pair(const float4 *__restrict__ a, const float4 *__restrict__ b, float4 *__restrict__ out,
+                            unsigned long long n) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    out[2 * t] = a[t];
+    out[2 * t + 1] = b[t];
+}
+
 __global__ void k_pad3d(const float4 *__restrict__ src, float4 *__restrict__ dst, int nx, int ny, int nz) {
     const uint32_t px = (uint32_t)nx + RSD_PAD_LO + RSD_PAD_HI, py = (uint32_t)ny + RSD_PAD_LO + RSD_PAD_HI;
     const uint32_t pz = (uint32_t)nz + RSD_PAD_LO + RSD_PAD_HI;
@@ -2227,6 +2263,13 @@ struct rsd_engine {
     // (rebuilt when either changes) and copied stream-ordered from pinned
     // staging.
     float4 *itab_d[2], *itab_h[2];
+    // Time-interpolated wind: the volumes of slot wpair_slot and the other slot
+    // as texel pairs (k_wind_pair), rebuilt when either slot's volume changes
+    // (wgen counts each slot's uploads).
+    float4 *wpair;
+    size_t wpair_cap;
+    int wpair_ok, wpair_slot;
+    unsigned wgen[2], wpair_gen[2];
     hipEvent_t ev_itab[2];
     float itab_key[2][22];
     int itab_ok[2];
@@ -2454,6 +2497,7 @@ extern "C" void rsd_close(rsd_engine *e) {
     }
     dfree(e->drop); dfree(e->dff[0]); dfree(e->dff[1]); dfree(e->rangew); dfree(e->angw); dfree(e->gate_r); dfree(e->partials); dfree(e->partials2);
     dfree(e->ring);
+    dfree(e->wpair);
     dfree(e->mom_d);
     if (e->pulse_h) (void)hipHostFree(e->pulse_h);
     if (e->prof_ev) {
@@ -2741,6 +2785,38 @@ static int pad_raw(rsd_engine *e, float4 *dst, int nx, int ny, int nz) {
 }
 
 // Time interpolation towards the other LES slot (same shape), weight alpha.
+// The texel pairs of slot `slot` (volume k) and the other slot (k + 1),
+// rebuilt on the engine's stream after an upload into either; null when they
+// cannot be allocated (the launch then reads the two volumes apart).
+static const float4 *wind_pair(rsd_engine *e, int slot) {
+    const int o = 1 - slot;
+    const size_t pt = padded_texels(e->wnx[slot], e->wny[slot], e->wnz[slot]);
+    if (!e->wpair || e->wpair_cap < 2 * pt) {
+        if (e->wpair) {
+            if (hipStreamSynchronize(e->stream) != hipSuccess) return NULL;
+            dfree(e->wpair);
+        }
+        e->wpair = NULL;
+        e->wpair_cap = 0;
+        e->wpair_ok = 0;
+        if (hipMalloc((void **)&e->wpair, 2 * pt * sizeof(float4)) != hipSuccess) {
+            e->wpair = NULL;
+            return NULL;
+        }
+        e->wpair_cap = 2 * pt;
+    }
+    if (!e->wpair_ok || e->wpair_slot != slot || e->wpair_gen[0] != e->wgen[slot] || e->wpair_gen[1] != e->wgen[o]) {
+        hipLaunchKernelGGL(k_wind_pair, dim3((unsigned)((pt + 255) / 256)), dim3(256), 0, e->stream,
+                           (const float4 *)e->wind[slot], (const float4 *)e->wind[o], e->wpair, (unsigned long long)pt);
+        if (hipGetLastError() != hipSuccess) return NULL;
+        e->wpair_ok = 1;
+        e->wpair_slot = slot;
+        e->wpair_gen[0] = e->wgen[slot];
+        e->wpair_gen[1] = e->wgen[o];
+    }
+    return e->wpair;
+}
+
 static int blend_slot(rsd_engine *e, int slot, Tex3 *t, float alpha) {
     const int o = 1 - slot;
     if (!e->wind_ok[o] || e->wnx[o] != e->wnx[slot] || e->wny[o] != e->wny[slot] || e->wnz[o] != e->wnz[slot])
@@ -2794,6 +2870,7 @@ static int fill_rain(rsd_engine *e, const rsd_step *s, RainK &a) {
     if (s && s->wind_blend) {
         int st = blend_slot(e, slot, &a.wind, s->wind_alpha);
         if (st) return st;
+        a.wpair = wind_pair(e, slot);
     }
     a.les = e->les[slot];
     if (s) memcpy(a.sd.s, s->sim_desc, sizeof(a.sd.s));
@@ -3093,7 +3170,7 @@ static int launch_step(rsd_engine *e, const rsd_step *s, const rsd_pulse *p, con
         // queue's processing of separate event packets.
         hipEvent_t t0 = timed ? e->prof_ev[2 * e->prof_n] : nullptr, t1 = timed ? e->prof_ev[2 * e->prof_n + 1] : nullptr;
         const bool ti = a.wind.d2 != nullptr;
-        const bool fast = rain_fast(e, s, p) && a.tab_lds;
+        const bool fast = rain_fast(e, s, p) && a.tab_lds && (!ti || a.wpair);
         if (fast) {
             if (e->plo && ti) hipExtLaunchKernelGGL((k_rain<true, true, true, true, true>), grid, dim3(e->block), lds_rain, e->stream, t0, t1, 0, a, d);
             else if (ti) hipExtLaunchKernelGGL((k_rain<true, true, false, true, true>), grid, dim3(e->block), lds_rain, e->stream, t0, t1, 0, a, d);
@@ -3563,6 +3640,7 @@ extern "C" int rsd_set_wind(rsd_engine *e, int slot, const float *uvwt, int nx, 
     }
     e->wnx[slot] = nx; e->wny[slot] = ny; e->wnz[slot] = nz;
     e->wind_ok[slot] = 1;
+    e->wgen[slot]++;
     memcpy(e->les[slot].s, desc, sizeof(e->les[slot].s));
     return RSD_OK;
 }
