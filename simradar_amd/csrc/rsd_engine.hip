@@ -52,6 +52,16 @@
 #include "rsd_mlp.h"
 #include "rsd_mlp_host.h"
 
+// Sample tiles of 16 the debris kernel's network forward carries at once (1, 2
+// or 4): fewer tiles hold fewer activation registers and read each weight tile
+// from LDS more often.
+#ifndef RSD_MLP_TILES
+#define RSD_MLP_TILES 4
+#endif
+#ifndef RSD_DEBRIS_SURR_WAVES
+#define RSD_DEBRIS_SURR_WAVES 3     // waves per SIMD the surrogate debris kernel is compiled for
+#endif
+
 #define RSD_MIN_HEIGHT 2.0f
 #define RSD_PI_F 0x1.921fb6p+1f
 #define RSD_PI_2_F 0x1.921fb6p+0f
@@ -1581,7 +1591,7 @@ __device__ __forceinline__ void debris_range(const DebrisK &a, size_t lo, size_t
                 for (int k = 4; k < 7; k++) f[k] = f[k] == f[k] ? f[k] : 0.0f;
             }
             float out[MLP_OUT];
-            mlp_wave_eval<MlpPadded>(nw, nb, f, nscr + (threadIdx.x >> 6) * 64 * MLP_SCR, out);
+            mlp_wave_eval<MlpPadded, RSD_MLP_TILES>(nw, nb, f, nscr + (threadIdx.x >> 6) * 64 * MLP_SCR, out);
             if (ADV && live) {
                 if (need_adv) {
                     debris_forces(a, o, v, t, bg, ur, make_float4(out[0], out[1], out[2], 0.0f),
@@ -1655,7 +1665,7 @@ __device__ __forceinline__ int step_type(const StepDebris &d) {
 // 168 VGPRs without spilling (two waves at its natural 244; 4,421 against
 // 3,887 pulses/s at C5).
 template <bool ADV, bool PULSE, bool SURR>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_debris(StepDebris d) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SURR ? RSD_DEBRIS_SURR_WAVES : 3))) void k_debris(StepDebris d) {
     extern __shared__ double acc[];
     __shared__ __attribute__((aligned(16))) uint16_t nw[SURR ? MLP_PWN : 2];
     __shared__ __attribute__((aligned(16))) float nb[SURR ? MLP_BN : 4];

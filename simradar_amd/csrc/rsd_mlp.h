@@ -17,7 +17,9 @@
 // (lane l holds features 16 k + 4 (l / 16) + 0..3 of sample l % 16) and D in
 // the same lane layout as B (features 16 m + 4 (l / 16) + 0..3 of sample
 // l % 16).  The output of one layer is therefore, tile for tile, the B operand
-// of the next: activations never leave the registers.
+// of the next: activations never leave the registers.  Inference
+// (mlp_forward4) runs the 64-wide layers on v_mfma_f32_16x16x32_bf16 instead,
+// two such B tiles forming one K = 32 operand.
 #ifndef RSD_MLP_H
 #define RSD_MLP_H
 
@@ -50,23 +52,48 @@
 #define MLP_SHIFT (MLP_SCALE + MLP_N4)
 #define MLP_BN (MLP_SHIFT + MLP_N4)                 // 304 floats
 
-// Padded image of the weights for inference out of LDS: rows of W0 at a
-// stride of 20 bf16 (40 bytes) and of W1..W4 at 68 bf16 (136 bytes), so that
-// the 16 rows one A operand load touches fall in distinct LDS banks (at the
-// packed strides of 32 and 128 bytes they share 2 and 8 ways).
+// Padded image of the weights for inference out of LDS.  W0: rows at a stride
+// of 20 bf16 (40 bytes).  W1..W4: rows at 72 bf16 (144 bytes), each row's 64
+// inputs reordered for v_mfma_f32_16x16x32_bf16 (mlp_mfma32): position
+// 32 c + 8 g + j holds input 32 c + 4 g + j for j < 4 and 32 c + 16 + 4 g +
+// (j - 4) for j >= 4, so one A operand is one aligned 16-byte read, and the 16
+// rows one such read touches in a quarter wave cover the 64 LDS banks once (at
+// the packed strides of 32 and 128 bytes they share 2 and 8 ways).
 #define MLP_PK0 20
-#define MLP_PH 68
+#define MLP_PH 72
 #define MLP_PW1 (MLP_H * MLP_PK0)
 #define MLP_PW4 (MLP_PW1 + 3 * MLP_H * MLP_PH)
-#define MLP_PWN (MLP_PW4 + MLP_N4 * MLP_PH)         // 15424 bf16
+#define MLP_PWN (MLP_PW4 + MLP_N4 * MLP_PH)         // 16256 bf16
 #define MLP_SCR 12                                  // floats per sample in the output exchange
 
-// Row strides and offsets of a weight image.
-struct MlpPacked { static constexpr int S0 = MLP_K0, SH = MLP_H, O1 = MLP_W1, O4 = MLP_W4; };
-struct MlpPadded { static constexpr int S0 = MLP_PK0, SH = MLP_PH, O1 = MLP_PW1, O4 = MLP_PW4; };
-
 typedef short mlp_s4 __attribute__((ext_vector_type(4)));
+typedef short mlp_s8 __attribute__((ext_vector_type(8)));
 typedef float mlp_f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 mlp_bf8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ mlp_s4 mlp_ld4(const uint16_t *p) {  // 4 consecutive bf16 (8-byte aligned)
+    return *(const mlp_s4 *)p;
+}
+
+__device__ __forceinline__ mlp_s8 mlp_cat(mlp_s4 lo, mlp_s4 hi) {
+    return __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+}
+
+// Row strides and offsets of a weight image, and the K = 32 A operand of lane
+// group g for input chunk c of a W1..W4 row (inputs 32 c + 4 g + 0..3, then
+// 32 c + 16 + 4 g + 0..3).
+struct MlpPacked {
+    static constexpr int S0 = MLP_K0, SH = MLP_H, O1 = MLP_W1, O4 = MLP_W4;
+    static __device__ __forceinline__ mlp_s8 a8(const uint16_t *row, int c, int g) {
+        return mlp_cat(mlp_ld4(row + 32 * c + 4 * g), mlp_ld4(row + 32 * c + 16 + 4 * g));
+    }
+};
+struct MlpPadded {
+    static constexpr int S0 = MLP_PK0, SH = MLP_PH, O1 = MLP_PW1, O4 = MLP_PW4;
+    static __device__ __forceinline__ mlp_s8 a8(const uint16_t *row, int c, int g) {
+        return *(const mlp_s8 *)(row + 32 * c + 8 * g);
+    }
+};
 
 // Device-side view of a network.
 struct MlpDev {
@@ -99,11 +126,17 @@ __device__ __forceinline__ mlp_f4 mlp_mfma(mlp_s4 a, mlp_s4 b, mlp_f4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
-__device__ __forceinline__ float mlp_act(float x) { return __builtin_fmaxf(x, 0.0f); }   // ReLU
-
-__device__ __forceinline__ mlp_s4 mlp_ld4(const uint16_t *p) {  // 4 consecutive bf16 (8-byte aligned)
-    return *(const mlp_s4 *)p;
+// D = A B + C on 16 x 32 tiles: lane l holds A[l % 16][8 (l / 16) + 0..7] and
+// B[8 (l / 16) + 0..7][l % 16]; D as for the 16x16x16 form.  Only the pairing
+// of A's and B's K entries within a lane matters, so the K order is free:
+// chunk c of a 64-wide layer input pairs the 16x16x16 B tiles 2 c and 2 c + 1
+// (features 32 c + 4 g + 0..3, then 32 c + 16 + 4 g + 0..3).
+__device__ __forceinline__ mlp_f4 mlp_mfma32(mlp_s8 a, mlp_s8 b, mlp_f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(mlp_bf8, a), __builtin_bit_cast(mlp_bf8, b), c,
+                                                    0, 0, 0);
 }
+
+__device__ __forceinline__ float mlp_act(float x) { return __builtin_fmaxf(x, 0.0f); }   // ReLU
 
 // Forward of one 16-sample tile.  x: layer-0 B operand (lanes with l / 16 >= 2
 // hold zeros); w, b: weight and bias images (LDS or global).  Returns the
@@ -168,21 +201,18 @@ __device__ __forceinline__ mlp_s4 mlp_gather_input(const float f[8], int t) {
 }
 
 // Copy the packed weight image w into the padded image dst (LDS) with the
-// threads of a block, four bf16 (8 bytes) at a time: W0 is 64 rows of 4 such
-// chunks, W1..W4 (contiguous in both images) 208 rows of 16.
+// threads of a block: W0 as 64 rows of four 8-byte chunks; W1..W4 (contiguous
+// in both images) as 208 rows of eight 16-byte A operands, each joining the
+// two 8-byte runs of its row that MlpPadded::a8 pairs.
 __device__ __forceinline__ void mlp_stage_padded(uint16_t *dst, const uint16_t *w) {
     const uint2 *src = (const uint2 *)w;
-    for (int c = threadIdx.x; c < MLP_H * 4 + (3 * MLP_H + MLP_N4) * 16; c += blockDim.x) {
-        int s, d;
-        if (c < MLP_H * 4) {
-            s = (c >> 2) * MLP_K0 + (c & 3) * 4;
-            d = (c >> 2) * MLP_PK0 + (c & 3) * 4;
-        } else {
-            const int k = c - MLP_H * 4;
-            s = MLP_W1 + (k >> 4) * MLP_H + (k & 15) * 4;
-            d = MLP_PW1 + (k >> 4) * MLP_PH + (k & 15) * 4;
-        }
-        *(uint2 *)(dst + d) = src[s >> 2];
+    for (int c = threadIdx.x; c < MLP_H * 4; c += blockDim.x)
+        *(uint2 *)(dst + (c >> 2) * MLP_PK0 + (c & 3) * 4) = src[((c >> 2) * MLP_K0 + (c & 3) * 4) >> 2];
+    for (int c = threadIdx.x; c < (3 * MLP_H + MLP_N4) * 8; c += blockDim.x) {
+        const int r = c >> 3, ch = (c >> 2) & 1, g = c & 3;
+        const int s = MLP_W1 + r * MLP_H + 32 * ch + 4 * g;
+        const uint2 lo = src[s >> 2], hi = src[(s + 16) >> 2];
+        *(uint4 *)(dst + MLP_PW1 + r * MLP_PH + 32 * ch + 8 * g) = make_uint4(lo.x, lo.y, hi.x, hi.y);
     }
 }
 
@@ -198,49 +228,55 @@ __device__ __forceinline__ mlp_s4 mlp_relu_pack(mlp_f4 v) {
 // Forward of the wave's four 16-sample tiles together: every weight tile is
 // read once and applied to the four tiles (four independent MFMA chains).  The
 // bias enters as the accumulator the first MFMA of each output tile starts
-// from.  LAY: the weight image's layout (MlpPacked or MlpPadded).
-template <class LAY>
+// from.  Layer 0 (K = 16 for the 7 inputs) runs on the 16x16x16 form, the
+// 64-wide layers on the 16x16x32 one (two MFMA per output tile and sample
+// tile instead of four).  LAY: the weight image's layout (MlpPacked or
+// MlpPadded).
+template <class LAY, int P = 4>
 __device__ __forceinline__ void mlp_forward4(const uint16_t *w, const float *b, const mlp_s4 x[4], mlp_f4 y[4]) {
     const int l = threadIdx.x & 63, row = l & 15, grp = l >> 4;
-    mlp_s4 cur[4][4], nxt[4][4];
 #pragma unroll
-    for (int mt = 0; mt < 4; mt++) {
-        const mlp_s4 A = mlp_ld4(w + (16 * mt + row) * LAY::S0 + 4 * grp);
-        const mlp_f4 bias = *(const mlp_f4 *)(b + MLP_B0 + 16 * mt + 4 * grp);
-#pragma unroll
-        for (int t = 0; t < 4; t++) cur[t][mt] = mlp_relu_pack(mlp_mfma(A, x[t], bias));
-    }
-#pragma unroll 1
-    for (int L = 1; L < 4; L++) {
-        const uint16_t *W = w + LAY::O1 + (L - 1) * MLP_H * LAY::SH;
+    for (int t0 = 0; t0 < 4; t0 += P) {
+        mlp_s4 cur[P][4], nxt[P][4];
 #pragma unroll
         for (int mt = 0; mt < 4; mt++) {
-            const mlp_f4 bias = *(const mlp_f4 *)(b + MLP_H * L + 16 * mt + 4 * grp);
-            mlp_f4 acc[4];
+            const mlp_s4 A = mlp_ld4(w + (16 * mt + row) * LAY::S0 + 4 * grp);
+            const mlp_f4 bias = *(const mlp_f4 *)(b + MLP_B0 + 16 * mt + 4 * grp);
 #pragma unroll
-            for (int t = 0; t < 4; t++) acc[t] = bias;
+            for (int t = 0; t < P; t++) cur[t][mt] = mlp_relu_pack(mlp_mfma(A, x[t0 + t], bias));
+        }
+#pragma unroll 1
+        for (int L = 1; L < 4; L++) {
+            const uint16_t *W = w + LAY::O1 + (L - 1) * MLP_H * LAY::SH;
 #pragma unroll
-            for (int kt = 0; kt < 4; kt++) {
-                const mlp_s4 A = mlp_ld4(W + (16 * mt + row) * LAY::SH + 16 * kt + 4 * grp);
+            for (int mt = 0; mt < 4; mt++) {
+                const mlp_f4 bias = *(const mlp_f4 *)(b + MLP_H * L + 16 * mt + 4 * grp);
+                mlp_f4 acc[P];
 #pragma unroll
-                for (int t = 0; t < 4; t++) acc[t] = mlp_mfma(A, cur[t][kt], acc[t]);
+                for (int t = 0; t < P; t++) acc[t] = bias;
+#pragma unroll
+                for (int c = 0; c < 2; c++) {
+                    const mlp_s8 A = LAY::a8(W + (16 * mt + row) * LAY::SH, c, grp);
+#pragma unroll
+                    for (int t = 0; t < P; t++) acc[t] = mlp_mfma32(A, mlp_cat(cur[t][2 * c], cur[t][2 * c + 1]), acc[t]);
+                }
+#pragma unroll
+                for (int t = 0; t < P; t++) nxt[t][mt] = mlp_relu_pack(acc[t]);
             }
 #pragma unroll
-            for (int t = 0; t < 4; t++) nxt[t][mt] = mlp_relu_pack(acc[t]);
+            for (int t = 0; t < P; t++)
+#pragma unroll
+                for (int k = 0; k < 4; k++) cur[t][k] = nxt[t][k];
         }
+        const mlp_f4 bias = *(const mlp_f4 *)(b + MLP_B4 + 4 * grp);
 #pragma unroll
-        for (int t = 0; t < 4; t++)
+        for (int t = 0; t < P; t++) y[t0 + t] = bias;
 #pragma unroll
-            for (int k = 0; k < 4; k++) cur[t][k] = nxt[t][k];
-    }
-    const mlp_f4 bias = *(const mlp_f4 *)(b + MLP_B4 + 4 * grp);
+        for (int c = 0; c < 2; c++) {
+            const mlp_s8 A = LAY::a8(w + LAY::O4 + row * LAY::SH, c, grp);
 #pragma unroll
-    for (int t = 0; t < 4; t++) y[t] = bias;
-#pragma unroll
-    for (int kt = 0; kt < 4; kt++) {
-        const mlp_s4 A = mlp_ld4(w + LAY::O4 + row * LAY::SH + 16 * kt + 4 * grp);
-#pragma unroll
-        for (int t = 0; t < 4; t++) y[t] = mlp_mfma(A, cur[t][kt], y[t]);
+            for (int t = 0; t < P; t++) y[t0 + t] = mlp_mfma32(A, mlp_cat(cur[t][2 * c], cur[t][2 * c + 1]), y[t0 + t]);
+        }
     }
 }
 
@@ -249,7 +285,7 @@ __device__ __forceinline__ void mlp_forward4(const uint16_t *w, const float *b, 
 // MLP_SCR floats per wave, the 12 outputs of each sample; at 48 bytes a row the
 // 16 rows one store touches fall in distinct banks); out[0..11] = scale * y +
 // shift.
-template <class LAY = MlpPacked>
+template <class LAY = MlpPacked, int P = 4>
 __device__ __forceinline__ void mlp_wave_eval(const uint16_t *w, const float *b, const float f[8], float *scratch,
                                               float out[MLP_OUT]) {
     const int l = threadIdx.x & 63, row = l & 15, grp = l >> 4;
@@ -257,7 +293,7 @@ __device__ __forceinline__ void mlp_wave_eval(const uint16_t *w, const float *b,
     mlp_f4 y[4];
 #pragma unroll
     for (int t = 0; t < 4; t++) x[t] = mlp_gather_input(f, t);
-    mlp_forward4<LAY>(w, b, x, y);
+    mlp_forward4<LAY, P>(w, b, x, y);
     if (grp < MLP_OUT / 4) {
 #pragma unroll
         for (int t = 0; t < 4; t++) *(mlp_f4 *)(scratch + (16 * t + row) * MLP_SCR + 4 * grp) = y[t];
