@@ -58,6 +58,9 @@
 #ifndef RSD_MLP_TILES
 #define RSD_MLP_TILES 4
 #endif
+#ifndef RSD_MLP_XR
+#define RSD_MLP_XR 4                // network outputs exchanged per sample per round (12 or 4)
+#endif
 #ifndef RSD_DEBRIS_SURR_WAVES
 #define RSD_DEBRIS_SURR_WAVES 3     // waves per SIMD the surrogate debris kernel is compiled for
 #endif
@@ -1591,7 +1594,7 @@ __device__ __forceinline__ void debris_range(const DebrisK &a, size_t lo, size_t
                 for (int k = 4; k < 7; k++) f[k] = f[k] == f[k] ? f[k] : 0.0f;
             }
             float out[MLP_OUT];
-            mlp_wave_eval<MlpPadded, RSD_MLP_TILES>(nw, nb, f, nscr + (threadIdx.x >> 6) * 64 * MLP_SCR, out);
+            mlp_wave_eval<MlpPadded, RSD_MLP_TILES, RSD_MLP_XR>(nw, nb, f, nscr + (threadIdx.x >> 6) * 64 * RSD_MLP_XR, out);
             if (ADV && live) {
                 if (need_adv) {
                     debris_forces(a, o, v, t, bg, ur, make_float4(out[0], out[1], out[2], 0.0f),
@@ -1663,13 +1666,14 @@ __device__ __forceinline__ int step_type(const StepDebris &d) {
 // types in one grid, so their blocks share the device instead of queueing
 // one type after another.  Three waves per SIMD: the surrogate variant fits
 // 168 VGPRs without spilling (two waves at its natural 244; 4,421 against
-// 3,887 pulses/s at C5).
+// 3,887 pulses/s at C5).  Four (128 VGPRs, LDS under 40 KB with the outputs
+// exchanged in three rounds) measured no faster.
 template <bool ADV, bool PULSE, bool SURR>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SURR ? RSD_DEBRIS_SURR_WAVES : 3))) void k_debris(StepDebris d) {
     extern __shared__ double acc[];
     __shared__ __attribute__((aligned(16))) uint16_t nw[SURR ? MLP_PWN : 2];
     __shared__ __attribute__((aligned(16))) float nb[SURR ? MLP_BN : 4];
-    __shared__ __attribute__((aligned(16))) float nscr[SURR ? 4 * 64 * MLP_SCR : 4];
+    __shared__ __attribute__((aligned(16))) float nscr[SURR ? 4 * 64 * RSD_MLP_XR : 4];
     const int j = step_type(d);
     const DebrisK &a = d.t[j];
     if (SURR) {

@@ -281,32 +281,38 @@ __device__ __forceinline__ void mlp_forward4(const uint16_t *w, const float *b, 
 }
 
 // Network outputs of every lane's own sample: the four tiles of the wave are
-// evaluated together and their D tiles exchanged through scratch (64 x
-// MLP_SCR floats per wave, the 12 outputs of each sample; at 48 bytes a row the
-// 16 rows one store touches fall in distinct banks); out[0..11] = scale * y +
-// shift.
-template <class LAY = MlpPacked, int P = 4>
+// evaluated together and their D tiles exchanged through scratch, XR floats
+// per sample per round: XR = 12 (MLP_SCR) in one round (64 x 12 floats per
+// wave; at 48 bytes a row the 16 rows one store touches fall in distinct
+// banks), XR = 4 in three rounds of one lane group's four outputs (64 x 4
+// floats per wave).  out[0..11] = scale * y + shift.
+template <class LAY = MlpPacked, int P = 4, int XR = MLP_SCR>
 __device__ __forceinline__ void mlp_wave_eval(const uint16_t *w, const float *b, const float f[8], float *scratch,
                                               float out[MLP_OUT]) {
+    static_assert(XR == MLP_SCR || XR == 4, "exchange of 12 or 4 floats per sample");
     const int l = threadIdx.x & 63, row = l & 15, grp = l >> 4;
     mlp_s4 x[4];
     mlp_f4 y[4];
 #pragma unroll
     for (int t = 0; t < 4; t++) x[t] = mlp_gather_input(f, t);
     mlp_forward4<LAY, P>(w, b, x, y);
-    if (grp < MLP_OUT / 4) {
 #pragma unroll
-        for (int t = 0; t < 4; t++) *(mlp_f4 *)(scratch + (16 * t + row) * MLP_SCR + 4 * grp) = y[t];
+    for (int q0 = 0; q0 < MLP_OUT / 4; q0 += XR / 4) {
+        if (XR == MLP_SCR ? grp < MLP_OUT / 4 : grp == q0) {
+#pragma unroll
+            for (int t = 0; t < 4; t++) *(mlp_f4 *)(scratch + (16 * t + row) * XR + (XR == MLP_SCR ? 4 * grp : 0)) = y[t];
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+#pragma unroll
+        for (int q = q0; q < q0 + XR / 4; q++) {
+            const mlp_f4 v = *(const mlp_f4 *)(scratch + l * XR + 4 * (q - q0));
+#pragma unroll
+            for (int r = 0; r < 4; r++) out[4 * q + r] = v[r] * b[MLP_SCALE + 4 * q + r] + b[MLP_SHIFT + 4 * q + r];
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-#pragma unroll
-    for (int q = 0; q < MLP_OUT / 4; q++) {
-        const mlp_f4 v = *(const mlp_f4 *)(scratch + l * MLP_SCR + 4 * q);
-#pragma unroll
-        for (int r = 0; r < 4; r++) out[4 * q + r] = v[r] * b[MLP_SCALE + 4 * q + r] + b[MLP_SHIFT + 4 * q + r];
-    }
-    __builtin_amdgcn_wave_barrier();
 }
 
 #endif

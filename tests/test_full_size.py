@@ -1,6 +1,7 @@
 """Each workload configuration at its full size on the GPU against the oracle:
 
-C2  1,997,824 rain + 1,024 leaves: 3 pulses, state bit-exact, IQ within 1e-4
+C2  1,997,824 rain + 1,024 leaves: 3 pulses, state bit-exact, IQ within 1e-4,
+    with the LES volume switch and with the volumes interpolated in time
 C3  8,388,608 rain in the 10 km box with hi/lo positions: 3 pulses against the
     binary64 oracle (IQ within 1e-4, Z within 0.1 dB, V within 0.05 m/s), the
     all-binary32 variant far worse
@@ -39,6 +40,12 @@ def _full(cfg, pulses):
 
 def test_c2_full_size(gpu_ready):
     _full(wl.c2(scan="P:3.0,-1:1:0.01"), 3)
+
+
+def test_c2_full_size_les_interpolated_in_time(gpu_ready):
+    cfg = wl.c2(scan="P:3.0,-1:1:0.01")
+    cfg.les_interp = True
+    _full(cfg, 3)
 
 
 def test_c5_table_path_full_size(gpu_ready):
