@@ -152,6 +152,9 @@ def main():
                     help="C2 headline with the reference's piecewise-constant LES volume switch instead of the "
                          "time interpolation")
     ap.add_argument("--no-alt", action="store_true", help="C2: skip timing the other LES volume mode")
+    ap.add_argument("--c5-serial-train", action="store_true",
+                    help="C5: train the three networks one after another (rocprofv3's tracer has crashed on the "
+                         "concurrent training threads)")
     ap.add_argument("--c5-samples", type=int, default=TRAIN_SAMPLES,
                     help="training samples per debris type of the C5 networks (fewer: shorter profiling runs)")
     args = ap.parse_args()
@@ -390,12 +393,13 @@ def bench_c5(args):
         return r.timer_stop() / steps
     ms_table = timed()
     t0 = time.perf_counter()
-    nets, errs = [], []
-    for k in (1, 2, 3):
-        m, _ = sg.fit(r, k, n_samples=args.c5_samples, epochs=16, state_frac=0.25)
-        errs.append(sg.error(m, r, k))
-        nets.append(m)
+    kw = dict(n_samples=args.c5_samples, epochs=16, state_frac=0.25)
+    if args.c5_serial_train:
+        nets = [sg.fit(r, k, **kw)[0] for k in (1, 2, 3)]
+    else:
+        nets = [m for m, _ in sg.fit_many(r, (1, 2, 3), **kw)]
     train_s = time.perf_counter() - t0
+    errs = [sg.error(m, r, k) for k, m in zip((1, 2, 3), nets)]
     for k, m in zip((1, 2, 3), nets):
         r.set_debris_surrogate(k, m)
     ms_surr = timed()

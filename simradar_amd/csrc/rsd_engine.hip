@@ -2434,8 +2434,8 @@ extern "C" int rsd_open(int device_index, rsd_engine **out) {
         hipMalloc((void **)&e->itab_d[1], RSD_ITAB_ROWS * sizeof(float4)) != hipSuccess ||
         hipHostMalloc((void **)&e->itab_h[0], RSD_ITAB_ROWS * sizeof(float4), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void **)&e->itab_h[1], RSD_ITAB_ROWS * sizeof(float4), hipHostMallocDefault) != hipSuccess ||
-        hipMemset(e->itab_d[0], 0, RSD_ITAB_ROWS * sizeof(float4)) != hipSuccess ||
-        hipMemset(e->itab_d[1], 0, RSD_ITAB_ROWS * sizeof(float4)) != hipSuccess) {
+        hipMemsetAsync(e->itab_d[0], 0, RSD_ITAB_ROWS * sizeof(float4), e->stream) != hipSuccess ||
+        hipMemsetAsync(e->itab_d[1], 0, RSD_ITAB_ROWS * sizeof(float4), e->stream) != hipSuccess) {
         free(e);
         return RSD_ERR_HIP;
     }
@@ -2668,7 +2668,9 @@ static int ensure_gates(rsd_engine *e, float start, float delta, int R) {
     dfree(e->gate_r);
     e->gate_r = NULL;
     hipError_t st = hipMalloc((void **)&e->gate_r, (size_t)R * sizeof(float));
-    if (st == hipSuccess) st = hipMemcpy(e->gate_r, h, (size_t)R * sizeof(float), hipMemcpyHostToDevice);
+    // on the engine's (non-blocking) stream, so the kernels after it see it
+    if (st == hipSuccess) st = hipMemcpyAsync(e->gate_r, h, (size_t)R * sizeof(float), hipMemcpyHostToDevice, e->stream);
+    if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
     free(h);
     if (st != hipSuccess) return fail(e, RSD_ERR_HIP, "gate table: %s", hipGetErrorString(st));
     e->gate_R = R;
@@ -4284,7 +4286,8 @@ extern "C" int rsd_batch_launch(rsd_batch *b) {
             const int rows = (int)((sel[k]->n_rain + chunk - 1) / chunk);
             for (int c = 0; c < rows; c++) h[t++] = make_int2(k, c);
         }
-        hipError_t r = hipMemcpy(b->items_d, h, (size_t)n_items * sizeof(int2), hipMemcpyHostToDevice);
+        hipError_t r = hipMemcpyAsync(b->items_d, h, (size_t)n_items * sizeof(int2), hipMemcpyHostToDevice, b->stream);
+        if (r == hipSuccess) r = hipStreamSynchronize(b->stream);
         free(h);
         if (r != hipSuccess) return bfail(b, RSD_ERR_HIP, "items upload");
         b->items_sig = sig;

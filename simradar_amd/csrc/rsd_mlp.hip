@@ -62,13 +62,16 @@ __device__ __forceinline__ void store_fm(uint16_t *base, int B, int feat0, int s
     base[(size_t)(feat0 + 3) * B + s] = (uint16_t)v.w;
 }
 
+static_assert(MLP_WN % 8 == 0 && MLP_TN % 8 == 0 && MLP_BN % 4 == 0, "images staged in 16-byte pieces");
+
 __global__ __launch_bounds__(256) void k_mlp_fwd_bwd(TrainK a) {
-    __shared__ uint16_t w[MLP_WN];
-    __shared__ uint16_t wt[MLP_TN];
-    __shared__ float b[MLP_BN];
-    for (int i = threadIdx.x; i < MLP_WN; i += blockDim.x) w[i] = a.net.w[i];
-    for (int i = threadIdx.x; i < MLP_TN; i += blockDim.x) wt[i] = a.net.wt[i];
-    for (int i = threadIdx.x; i < MLP_BN; i += blockDim.x) b[i] = a.net.b[i];
+    __shared__ __attribute__((aligned(16))) uint16_t w[MLP_WN];
+    __shared__ __attribute__((aligned(16))) uint16_t wt[MLP_TN];
+    __shared__ __attribute__((aligned(16))) float b[MLP_BN];
+    // the images in 16-byte pieces (their device copies are 16-byte aligned)
+    for (int i = threadIdx.x; i < MLP_WN / 8; i += blockDim.x) ((uint4 *)w)[i] = ((const uint4 *)a.net.w)[i];
+    for (int i = threadIdx.x; i < MLP_TN / 8; i += blockDim.x) ((uint4 *)wt)[i] = ((const uint4 *)a.net.wt)[i];
+    for (int i = threadIdx.x; i < MLP_BN / 4; i += blockDim.x) ((float4 *)b)[i] = ((const float4 *)a.net.b)[i];
     __syncthreads();
     const int l = threadIdx.x & 63, row = l & 15, grp = l >> 4;
     const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -261,6 +264,18 @@ static void mfree(void *p) {
     if (p) (void)hipFree(p);
 }
 
+// Every copy and fill of a network's memory goes on the network's own stream
+// (non-blocking, so work there is not ordered after the legacy default
+// stream's): a host-side copy waits for the stream, a fill is ordered before
+// the kernels that follow it.
+static hipError_t mcopy(rsd_mlp *m, void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+    hipError_t r = hipMemcpyAsync(dst, src, bytes, kind, m->stream);
+    if (r == hipSuccess) r = hipStreamSynchronize(m->stream);
+    return r;
+}
+
+static hipError_t mzero(rsd_mlp *m, void *dst, size_t bytes) { return hipMemsetAsync(dst, 0, bytes, m->stream); }
+
 // Position of public parameter k (flat, unpadded) in the padded image, or -1.
 static int mlp_image_index(int k) {
     if (k < 0) return -1;
@@ -330,9 +345,9 @@ extern "C" int rsd_mlp_create(int device, uint64_t seed, rsd_mlp **out) {
         h[i] = (float)((2.0 * u - 1.0) * lim);
     }
     for (int k = 0; k < MLP_OUT; k++) h[MLP_WN + MLP_SCALE + k] = 1.0f;
-    if (hipMemcpy(m->master, h, pb, hipMemcpyHostToDevice) != hipSuccess || hipMemset(m->m1, 0, pb) != hipSuccess ||
-        hipMemset(m->m2, 0, pb) != hipSuccess || hipMemset(m->w, 0, MLP_WN * sizeof(uint16_t)) != hipSuccess ||
-        hipMemset(m->wt, 0, MLP_TN * sizeof(uint16_t)) != hipSuccess) {
+    if (mcopy(m, m->master, h, pb, hipMemcpyHostToDevice) != hipSuccess || mzero(m, m->m1, pb) != hipSuccess ||
+        mzero(m, m->m2, pb) != hipSuccess || mzero(m, m->w, MLP_WN * sizeof(uint16_t)) != hipSuccess ||
+        mzero(m, m->wt, MLP_TN * sizeof(uint16_t)) != hipSuccess) {
         free(h);
         rsd_mlp_destroy(m);
         return RSD_ERR_HIP;
@@ -361,16 +376,16 @@ extern "C" int rsd_mlp_set_params(rsd_mlp *m, const float *p) {
     MCHK(m, hipSetDevice(m->device));
     float *h = (float *)malloc(MLP_PN * sizeof(float));
     if (!h) return RSD_ERR_ALLOC;
-    hipError_t r = hipMemcpy(h, m->master, MLP_PN * sizeof(float), hipMemcpyDeviceToHost);
+    hipError_t r = mcopy(m, h, m->master, MLP_PN * sizeof(float), hipMemcpyDeviceToHost);
     if (r == hipSuccess) {
         for (int k = 0; k < RSD_MLP_PARAMS; k++) h[mlp_image_index(k)] = p[k];
-        r = hipMemcpy(m->master, h, MLP_PN * sizeof(float), hipMemcpyHostToDevice);
+        r = mcopy(m, m->master, h, MLP_PN * sizeof(float), hipMemcpyHostToDevice);
     }
     free(h);
     MCHK(m, r);
     m->step = 0;
-    MCHK(m, hipMemset(m->m1, 0, MLP_PN * sizeof(float)));
-    MCHK(m, hipMemset(m->m2, 0, MLP_PN * sizeof(float)));
+    MCHK(m, mzero(m, m->m1, MLP_PN * sizeof(float)));
+    MCHK(m, mzero(m, m->m2, MLP_PN * sizeof(float)));
     int st = mlp_repack(m);
     if (st) return st;
     MCHK(m, hipStreamSynchronize(m->stream));
@@ -380,7 +395,7 @@ extern "C" int rsd_mlp_set_params(rsd_mlp *m, const float *p) {
 static int get_image(rsd_mlp *m, const float *src, float *p) {
     float *h = (float *)malloc(MLP_PN * sizeof(float));
     if (!h) return RSD_ERR_ALLOC;
-    hipError_t r = hipMemcpy(h, src, MLP_PN * sizeof(float), hipMemcpyDeviceToHost);
+    hipError_t r = mcopy(m, h, src, MLP_PN * sizeof(float), hipMemcpyDeviceToHost);
     if (r == hipSuccess)
         for (int k = 0; k < RSD_MLP_PARAMS; k++) p[k] = h[mlp_image_index(k)];
     free(h);
@@ -403,12 +418,13 @@ extern "C" int rsd_mlp_set_output_scaling(rsd_mlp *m, const float *scale, const 
         v[k] = scale[k];
         v[MLP_N4 + k] = shift[k];
     }
-    MCHK(m, hipMemcpy(m->master + MLP_WN + MLP_SCALE, v, sizeof(v), hipMemcpyHostToDevice));
+    MCHK(m, mcopy(m, m->master + MLP_WN + MLP_SCALE, v, sizeof(v), hipMemcpyHostToDevice));
     return RSD_OK;
 }
 
 static int ensure_batch(rsd_mlp *m, int B) {
     if (B <= m->batch_cap) return RSD_OK;
+    MCHK(m, hipStreamSynchronize(m->stream));
     mfree(m->act); mfree(m->dz); mfree(m->loss);
     m->act = m->dz = NULL;
     m->loss = NULL;
@@ -416,8 +432,8 @@ static int ensure_batch(rsd_mlp *m, int B) {
     MCHK(m, hipMalloc((void **)&m->dz, (size_t)5 * 64 * B * sizeof(uint16_t)));
     MCHK(m, hipMalloc((void **)&m->loss, (size_t)(B / 16) * sizeof(float)));
     // rows of the padded features are never written by the kernels: zero them once
-    MCHK(m, hipMemset(m->act, 0, (size_t)5 * 64 * B * sizeof(uint16_t)));
-    MCHK(m, hipMemset(m->dz, 0, (size_t)5 * 64 * B * sizeof(uint16_t)));
+    MCHK(m, mzero(m, m->act, (size_t)5 * 64 * B * sizeof(uint16_t)));
+    MCHK(m, mzero(m, m->dz, (size_t)5 * 64 * B * sizeof(uint16_t)));
     m->batch_cap = B;
     return RSD_OK;
 }
@@ -477,11 +493,11 @@ static int upload_set(rsd_mlp *m, const float *x7, const float *t, size_t n, flo
         h[i * 8 + 7] = 0.0f;
     }
     hipError_t r = hipMalloc((void **)xd, n * 8 * sizeof(float));
-    if (r == hipSuccess) r = hipMemcpy(*xd, h, n * 8 * sizeof(float), hipMemcpyHostToDevice);
+    if (r == hipSuccess) r = mcopy(m, *xd, h, n * 8 * sizeof(float), hipMemcpyHostToDevice);
     free(h);
     if (r == hipSuccess && t) {
         r = hipMalloc((void **)td, n * MLP_OUT * sizeof(float));
-        if (r == hipSuccess) r = hipMemcpy(*td, t, n * MLP_OUT * sizeof(float), hipMemcpyHostToDevice);
+        if (r == hipSuccess) r = mcopy(m, *td, t, n * MLP_OUT * sizeof(float), hipMemcpyHostToDevice);
     }
     MCHK(m, r);
     return RSD_OK;
@@ -498,7 +514,7 @@ extern "C" int rsd_mlp_gradient(rsd_mlp *m, const float *x, const float *t, size
     if (!st) st = get_image(m, m->grad, grad);
     if (!st && loss) {
         float *l = (float *)malloc((n / 16) * sizeof(float));
-        if (l && hipMemcpy(l, m->loss, (n / 16) * sizeof(float), hipMemcpyDeviceToHost) == hipSuccess) {
+        if (l && mcopy(m, l, m->loss, (n / 16) * sizeof(float), hipMemcpyDeviceToHost) == hipSuccess) {
             double s = 0.0;
             for (size_t i = 0; i < n / 16; i++) s += l[i];
             *loss = s / (12.0 * (double)n);
@@ -513,6 +529,7 @@ extern "C" int rsd_mlp_gradient(rsd_mlp *m, const float *x, const float *t, size
 extern "C" int rsd_mlp_set_data(rsd_mlp *m, const float *x, const float *t, size_t n) {
     if (!m || !x || !t || n < 256) return RSD_ERR_ARG;
     MCHK(m, hipSetDevice(m->device));
+    MCHK(m, hipStreamSynchronize(m->stream));
     mfree(m->x);
     mfree(m->t);
     m->x = m->t = NULL;
@@ -546,8 +563,7 @@ extern "C" int rsd_mlp_train(rsd_mlp *m, int epochs, int batch, float lr, double
         const int nb = batch / 16;
         float *l = (float *)malloc(nb * sizeof(float));
         if (!l) return RSD_ERR_ALLOC;
-        MCHK(m, hipStreamSynchronize(m->stream));
-        hipError_t r = hipMemcpy(l, m->loss, nb * sizeof(float), hipMemcpyDeviceToHost);
+        hipError_t r = mcopy(m, l, m->loss, nb * sizeof(float), hipMemcpyDeviceToHost);
         double s = 0.0;
         for (int i = 0; i < nb; i++) s += l[i];
         free(l);

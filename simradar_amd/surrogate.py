@@ -125,6 +125,28 @@ def fit(radar, debris_type, n_samples=1 << 21, epochs=30, batch=4096, lr=2e-3, s
     return m, loss
 
 
+def fit_many(radar, debris_types, n_samples=1 << 21, epochs=30, batch=4096, lr=2e-3, seed=7, state_frac=0.0):
+    """fit() for several debris types: the training sets are drawn one after
+    another (they use the engine), then the networks train at the same time,
+    each on its own stream from its own host thread (a 4096-sample step fills
+    a quarter of the device, so the three types' steps overlap).  Each network
+    comes out as fit() would make it.  Returns [(Mlp, last batch loss)]."""
+    from concurrent.futures import ThreadPoolExecutor
+    jobs = []
+    for k in debris_types:
+        x, t = training_set(radar, k, n_samples, seed, state_frac)
+        mean = t.mean(0)
+        std = t.std(0)
+        std[std < 1e-12] = 1.0
+        m = Mlp(seed=seed)
+        m.set_output_scaling(std, mean)
+        jobs.append((m, x, (t - mean) / std))
+    with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
+        futs = [ex.submit(m.train, x, t, epochs, batch, lr) for m, x, t in jobs]
+        losses = [f.result() for f in futs]
+    return [(m, loss) for (m, _, _), loss in zip(jobs, losses)]
+
+
 def error(m, radar, debris_type, n=65536, seed=99):
     """Relative RMS error of the network against the tables on held-out samples:
     (drag cd, moment cm, scattering amplitudes), each RMS(y - t) / RMS(t)."""

@@ -129,3 +129,21 @@ def test_surrogate_reflectivity_matches_the_tables(gpu_ready):
     for m in nets:
         m.free()
     r.free()
+
+
+def test_networks_trained_together_equal_those_trained_alone(gpu_ready):
+    """fit_many trains the types' networks at once on their own streams; each
+    comes out bit for bit as fit() makes it alone."""
+    cfg = h.Config(n_rain=4096, debris=[(sa.OBJ_LEAF, 512), (sa.OBJ_METAL_SHEET, 512), (sa.OBJ_BRICK, 512)],
+                   scan="P:3.0,-1:1:0.01")
+    r = wl.make(cfg)
+    kw = dict(n_samples=16384, epochs=3, batch=4096, seed=11, state_frac=0.25)
+    many = sg.fit_many(r, (1, 2, 3), **kw)
+    alone, loss = sg.fit(r, 2, **kw)
+    assert np.array_equal(many[1][0].params(), alone.params())
+    assert many[1][1] == loss
+    assert not np.array_equal(many[0][0].params(), many[1][0].params())
+    for m, _ in many:
+        m.free()
+    alone.free()
+    r.free()
