@@ -110,16 +110,25 @@ __device__ __forceinline__ uint16_t mlp_bf16(float f) {   // round to nearest ev
 
 __device__ __forceinline__ float mlp_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-typedef __bf16 mlp_bf4 __attribute__((ext_vector_type(4)));
+typedef __bf16 mlp_bf2 __attribute__((ext_vector_type(2)));
+typedef short mlp_s2 __attribute__((ext_vector_type(2)));
 
-// Four values to bf16 with the hardware conversion (round to nearest even).
+// Two values to bf16 in one v_cvt_pk_bf16_f32 (round to nearest even).  The
+// empty asm pins the packed pair in a register: without it the compiler merges
+// two pairs into one four-wide conversion and lowers that as four single
+// conversions and two permutes.
+__device__ __forceinline__ mlp_s2 mlp_pack2(float a, float b) {
+    mlp_bf2 h;
+    h.x = (__bf16)a;
+    h.y = (__bf16)b;
+    uint32_t u = __builtin_bit_cast(uint32_t, h);
+    asm("" : "+v"(u));
+    return __builtin_bit_cast(mlp_s2, u);
+}
+
+// Four values to bf16 (round to nearest even).
 __device__ __forceinline__ mlp_s4 mlp_pack(mlp_f4 v) {
-    mlp_bf4 h;
-    h.x = (__bf16)v.x;
-    h.y = (__bf16)v.y;
-    h.z = (__bf16)v.z;
-    h.w = (__bf16)v.w;
-    return __builtin_bit_cast(mlp_s4, h);
+    return __builtin_shufflevector(mlp_pack2(v.x, v.y), mlp_pack2(v.z, v.w), 0, 1, 2, 3);
 }
 
 __device__ __forceinline__ mlp_f4 mlp_mfma(mlp_s4 a, mlp_s4 b, mlp_f4 c) {
