@@ -61,6 +61,7 @@
 #ifndef RSD_MLP_XR
 #define RSD_MLP_XR 4                // network outputs exchanged per sample per round (12 or 4)
 #endif
+#define RSD_MLP_SCRW (RSD_MLP_XR > 8 ? RSD_MLP_XR : 8)   // scratch floats per sample (inputs: 8)
 #ifndef RSD_DEBRIS_SURR_WAVES
 #define RSD_DEBRIS_SURR_WAVES 3     // waves per SIMD the surrogate debris kernel is compiled for
 #endif
@@ -1594,7 +1595,7 @@ __device__ __forceinline__ void debris_range(const DebrisK &a, size_t lo, size_t
                 for (int k = 4; k < 7; k++) f[k] = f[k] == f[k] ? f[k] : 0.0f;
             }
             float out[MLP_OUT];
-            mlp_wave_eval<MlpPadded, RSD_MLP_TILES, RSD_MLP_XR>(nw, nb, f, nscr + (threadIdx.x >> 6) * 64 * RSD_MLP_XR, out);
+            mlp_wave_eval<MlpPadded, RSD_MLP_TILES, RSD_MLP_XR>(nw, nb, f, nscr + (threadIdx.x >> 6) * 64 * RSD_MLP_SCRW, out);
             if (ADV && live) {
                 if (need_adv) {
                     debris_forces(a, o, v, t, bg, ur, make_float4(out[0], out[1], out[2], 0.0f),
@@ -1673,7 +1674,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SURR ? RSD_
     extern __shared__ double acc[];
     __shared__ __attribute__((aligned(16))) uint16_t nw[SURR ? MLP_PWN : 2];
     __shared__ __attribute__((aligned(16))) float nb[SURR ? MLP_BN : 4];
-    __shared__ __attribute__((aligned(16))) float nscr[SURR ? 4 * 64 * RSD_MLP_XR : 4];
+    __shared__ __attribute__((aligned(16))) float nscr[SURR ? 4 * 64 * RSD_MLP_SCRW : 4];
     const int j = step_type(d);
     const DebrisK &a = d.t[j];
     if (SURR) {

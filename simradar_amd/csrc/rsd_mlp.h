@@ -193,22 +193,6 @@ __device__ __forceinline__ mlp_f4 mlp_tile_forward(const uint16_t *w, const floa
     return y;
 }
 
-// Layer-0 B operand of tile t (samples 16 t .. 16 t + 15 = lanes of the wave)
-// from every lane's own 8 input features f[0..7] (f[7] = 0).
-__device__ __forceinline__ mlp_s4 mlp_gather_input(const float f[8], int t) {
-    const int l = threadIdx.x & 63, grp = l >> 4;
-    const int src = 16 * t + (l & 15);
-    float v[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) v[k] = __shfl(f[k], src, 64);
-    mlp_f4 a;
-    a.x = grp == 0 ? v[0] : (grp == 1 ? v[4] : 0.0f);
-    a.y = grp == 0 ? v[1] : (grp == 1 ? v[5] : 0.0f);
-    a.z = grp == 0 ? v[2] : (grp == 1 ? v[6] : 0.0f);
-    a.w = grp == 0 ? v[3] : (grp == 1 ? v[7] : 0.0f);
-    return mlp_pack(a);
-}
-
 // Copy the packed weight image w into the padded image dst (LDS) with the
 // threads of a block: W0 as 64 rows of four 8-byte chunks; W1..W4 (contiguous
 // in both images) as 208 rows of eight 16-byte A operands, each joining the
@@ -289,12 +273,14 @@ __device__ __forceinline__ void mlp_forward4(const uint16_t *w, const float *b, 
     }
 }
 
-// Network outputs of every lane's own sample: the four tiles of the wave are
-// evaluated together and their D tiles exchanged through scratch, XR floats
-// per sample per round: XR = 12 (MLP_SCR) in one round (64 x 12 floats per
-// wave; at 48 bytes a row the 16 rows one store touches fall in distinct
-// banks), XR = 4 in three rounds of one lane group's four outputs (64 x 4
-// floats per wave).  out[0..11] = scale * y + shift.
+// Network outputs of every lane's own sample.  The inputs reach the layer-0 B
+// operands through scratch (each lane's 8 features as two float4, stored
+// [half][lane] so that both the stores and the tile reads cover the banks
+// once); the four tiles of the wave are evaluated together and their D tiles
+// exchanged through scratch, XR floats per sample per round: XR = 12
+// (MLP_SCR) in one round (at 48 bytes a row the 16 rows one store touches fall
+// in distinct banks), XR = 4 in three rounds of one lane group's four outputs.
+// scratch: 64 x max(8, XR) floats per wave.  out[0..11] = scale * y + shift.
 template <class LAY = MlpPacked, int P = 4, int XR = MLP_SCR>
 __device__ __forceinline__ void mlp_wave_eval(const uint16_t *w, const float *b, const float f[8], float *scratch,
                                               float out[MLP_OUT]) {
@@ -302,8 +288,18 @@ __device__ __forceinline__ void mlp_wave_eval(const uint16_t *w, const float *b,
     const int l = threadIdx.x & 63, row = l & 15, grp = l >> 4;
     mlp_s4 x[4];
     mlp_f4 y[4];
+    mlp_f4 *sx = (mlp_f4 *)scratch;
+    sx[l] = (mlp_f4){f[0], f[1], f[2], f[3]};
+    sx[64 + l] = (mlp_f4){f[4], f[5], f[6], f[7]};
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
 #pragma unroll
-    for (int t = 0; t < 4; t++) x[t] = mlp_gather_input(f, t);
+    for (int t = 0; t < 4; t++) {   // lanes with l / 16 >= 2 hold zeros (read in bounds, then dropped)
+        const mlp_f4 v = sx[(grp & 1) * 64 + 16 * t + row];
+        x[t] = grp < 2 ? mlp_pack(v) : (mlp_s4){0, 0, 0, 0};
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     mlp_forward4<LAY, P>(w, b, x, y);
 #pragma unroll
     for (int q0 = 0; q0 < MLP_OUT / 4; q0 += XR / 4) {
