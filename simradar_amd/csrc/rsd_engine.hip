@@ -363,13 +363,37 @@ __device__ __forceinline__ float4 debris_rcs(float4 pos, float4 ori, const Tex2 
 }
 
 // Radar-relative orientation R = ori (x) o_conj(el, az) of rs.cl:287-315, with
-// the sign fixed so that R.w >= 0 (R and -R are the same rotation).
+// the sign fixed so that R.w >= 0 (R and -R are the same rotation).  The
+// half-angle cosines and sines come from the position's direction without
+// atan2 and sincos: with rho = |(x, y)|, r = |p| and theta = el + pi / 2,
+// (cos, sin)(theta / 2) is along (r - z, rho) = along (rho, r + z), and with
+// phi = az = atan2(x, y), (cos, sin)(phi / 2) is along (rho + y, x) = along
+// (|x|, sign(x) (rho - y)); of each pair the form without cancellation is
+// taken.  The surrogate's inputs only (the table path keeps rs.cl's angles).
+__device__ __forceinline__ void half_angle(float u, float v, float &c, float &s) {
+    const float n = rf_len2(u, v);
+    c = rf_div(u, n);
+    s = rf_div(v, n);
+}
+
 __device__ __forceinline__ float4 radar_rel_quat(float4 pos, float4 ori) {
-    float el = rf_atan2f(pos.z, rf_len2(pos.x, pos.y));
-    float az = rf_atan2f(pos.x, pos.y);
+    const float rho = rf_len2(pos.x, pos.y), r = rf_len2(rho, pos.z);
     float se, ce, sa, ca;
-    rf_sincosf(0.5f * (el + RSD_PI_2_F), &se, &ce);
-    rf_sincosf(0.5f * az, &sa, &ca);
+    if (r == 0.0f) {            // at the radar: el = az = 0
+        ce = se = RSD_SQRT1_2_F;
+    } else if (pos.z <= 0.0f) {
+        half_angle(r - pos.z, rho, ce, se);
+    } else {
+        half_angle(rho, r + pos.z, ce, se);
+    }
+    if (rho == 0.0f) {          // on the vertical: az = atan2(x, +-0) = +-0
+        ca = 1.0f;
+        sa = __builtin_copysignf(0.0f, pos.x);
+    } else if (pos.y >= 0.0f) {
+        half_angle(rho + pos.y, pos.x, ca, sa);
+    } else {
+        half_angle(__builtin_fabsf(pos.x), __builtin_copysignf(rho - pos.y, pos.x), ca, sa);
+    }
     float4 oc = make_float4((-se * ca + ce * sa) * RSD_SQRT1_2_F, (ce * ca + se * sa) * RSD_SQRT1_2_F,
                             (se * ca + ce * sa) * RSD_SQRT1_2_F, (ce * ca + (-se) * sa) * RSD_SQRT1_2_F);
     float4 R = qmul(ori, oc);
